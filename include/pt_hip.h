@@ -1,0 +1,222 @@
+/*
+ * pt_hip.h — C ABI of libpt_hip.so, the MI355X (gfx950) trajectory and
+ * lag-statistics library for particle_transport.
+ *
+ * The reference has no plugin API: its Julia functions call CUDA.jl directly
+ * (@cuda at phase_space_gpu_runner.jl:569, multimode_multienergy_cache_pipeline.jl:1474,1484,
+ * compute_delta_mu2_curve.jl:203,355) and run the pair statistics on host
+ * threads.  Each entry point below replaces one of those call sites; the
+ * comment on each names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns int status: PT_OK (0) or a PT_ERR_* code.  No
+ *     C++ exception crosses this boundary.  pt_last_error() returns the
+ *     message of the last failure on the calling thread.
+ *   - Arrays follow the reference's Julia layouts: column-major, first index
+ *     fastest.  A Julia array (a,b,c) is the C array [c][b][a].
+ *   - Lag steps are the reference's 1-based cached-step offsets (>= 1).
+ *     Particle indices passed in selection lists are 0-based.
+ *   - Host pointers belong to the caller and are only read/written during
+ *     the call.  Device memory belongs to the handles.  "on_device" arguments
+ *     say whether a pointer is a HIP device pointer (1) or host memory (0).
+ *   - A context owns one HIP stream; every call on that context is ordered on
+ *     it and returns after the stream has drained.  A context is not
+ *     thread-safe: one host thread drives it, as one Julia task does today.
+ */
+#ifndef PT_HIP_H
+#define PT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_ABI_VERSION 1
+
+enum pt_status {
+    PT_OK = 0,
+    PT_ERR_INVALID = 1,  /* bad argument (shape, range, null pointer) */
+    PT_ERR_HIP = 2,      /* HIP runtime failure */
+    PT_ERR_MEMORY = 3,   /* device allocation failed or over budget */
+    PT_ERR_STATE = 4     /* call not valid in the handle's current state */
+};
+
+enum pt_dtype { PT_F64 = 0, PT_F32 = 1 };
+
+/* Field storage.  PLANAR = six separate (nx,ny,nz) column-major arrays, the
+ * reference layout (phase_space_gpu_runner.jl:515-520).  BRICK = the six
+ * components of a node interleaved in one 16-byte (BF16/FP16) or 32-byte
+ * (FP32) record, nodes grouped in 8x8x8 bricks; BF16/FP16 bricks store
+ * (value - offset_c) / scale_c per component c. */
+enum pt_storage {
+    PT_STORE_F64_PLANAR = 0,
+    PT_STORE_F32_PLANAR = 1,
+    PT_STORE_F32_BRICK = 2,
+    PT_STORE_BF16_BRICK = 3,
+    PT_STORE_F16_BRICK = 4
+};
+
+/* Pusher kinds.
+ *  REF_F64:    FP64 state (position in metres, momentum in SI), reference
+ *              operation order; needs a PT_STORE_F64_PLANAR field.
+ *  PLAIN_F32:  reference structure in FP32 with split position (int32 cell +
+ *              FP32 offset); state in HBM, six planar FP32 arrays, all 48
+ *              corner values gathered every step, one launch per step.
+ *  FAST:       FP32 pusher with split position; many steps per launch with
+ *              state and the current cell's interpolation coefficients in
+ *              registers; any BRICK field (or F32 planar). */
+enum pt_pusher { PT_PUSH_REF_F64 = 0, PT_PUSH_PLAIN_F32 = 1, PT_PUSH_FAST = 2 };
+
+/* Cadence output kinds (cache_mode in the reference's [run] section). */
+enum pt_out { PT_OUT_NONE = 0, PT_OUT_PHASE = 1, PT_OUT_MU = 2 };
+
+typedef struct pt_ctx pt_ctx;
+typedef struct pt_field pt_field;
+typedef struct pt_ens pt_ens;
+
+/* ---- context ------------------------------------------------------------ */
+
+int pt_abi_version(void);
+const char *pt_last_error(void);
+
+/* Replaces CUDA.jl's implicit device/stream setup (the reference uses the
+ * default device and stream).  device: HIP ordinal. */
+int pt_ctx_create(int device, pt_ctx **out);
+int pt_ctx_destroy(pt_ctx *ctx);
+/* Device bytes currently held by the context's fields and ensembles. */
+int pt_ctx_bytes_in_use(const pt_ctx *ctx, int64_t *bytes);
+
+/* ---- fields ------------------------------------------------------------- */
+
+/* Replaces the CuArray uploads of the six field components
+ * (phase_space_gpu_runner.jl:515-520; multimode_multienergy_cache_pipeline.jl:1419-1424)
+ * after load_static_fields (phase_space_gpu_runner.jl:95-123) has converted them to
+ * SI.  comps = {Bx, By, Bz (tesla), ux, uy, uz (m/s)}, each Julia (nx,ny,nz).
+ * box_length_m = box_length_pc * PC_TO_M; grid nodes x_i = range(0, L, length=n)
+ * (:125-130).  The FP64 mean |B| of the source values (reference_B0_T,
+ * time_units.jl:18-20) is computed on the device at creation. */
+int pt_field_create(pt_ctx *ctx, int64_t nx, int64_t ny, int64_t nz, double box_length_m,
+                    int storage, int src_dtype, int src_on_device, const void *const comps[6],
+                    pt_field **out);
+int pt_field_destroy(pt_field *field);
+
+/* reference_B0_T (time_units.jl:18-20): mean over all nodes of
+ * sqrt(Bx^2+By^2+Bz^2), summed in FP64 on the device. */
+int pt_field_mean_abs_b(const pt_field *field, double *B0_T);
+/* Same reduction on caller arrays (total-field clock for decomposed modes,
+ * README.md:132-148). */
+int pt_mean_abs_b(pt_ctx *ctx, int64_t n, int src_dtype, int src_on_device, const void *bx,
+                  const void *by, const void *bz, double *B0_T);
+
+/* ---- ensembles ---------------------------------------------------------- */
+
+/* Replaces the particle-state CuArray uploads and CUDA.fill(true, n) of
+ * run_gpu_ensemble (phase_space_gpu_runner.jl:506-513) and run_mu_cache_ensemble
+ * (multimode_multienergy_cache_pipeline.jl:1410-1417).  x/p: three host or device
+ * arrays of n values each (positions in metres, momenta in kg m/s), as
+ * init_particles returns them (phase_space_gpu_runner.jl:196-245).  The field
+ * fixes the geometry and must outlive the ensemble.  periodic: boundary
+ * "periodic" (1) or "open" (0). */
+int pt_ens_create(pt_ctx *ctx, const pt_field *field, int pusher, int64_t n, int src_dtype,
+                  int src_on_device, const void *const x[3], const void *const p[3], int periodic,
+                  pt_ens **out);
+int pt_ens_destroy(pt_ens *ens);
+
+typedef struct {
+    double dt_s;          /* dt = trajectory_time.dt_s (phase_space_gpu_runner.jl:498) */
+    double charge_C;      /* Q_E for the reference's protons (:9) */
+    double mass_kg;       /* M_P (:10) */
+    int64_t first_step;   /* 1-based si of the first step of this call */
+    int64_t n_steps;      /* steps run by this call: si = first_step .. first_step+n_steps-1 */
+    int64_t total_steps;  /* n_integration_steps; no push at si == total_steps (:568) */
+    int64_t save_stride;  /* cadence: save at si with (si-1) % save_stride == 0 (:393-396) */
+    int32_t out_kind;     /* PT_OUT_* */
+    int32_t steps_per_launch; /* FAST pusher: steps per kernel launch (0 = library default) */
+} pt_advance_params;
+
+/* Replaces the step loop of run_gpu_ensemble (phase_space_gpu_runner.jl:562-591) and
+ * run_mu_cache_ensemble (multimode_multienergy_cache_pipeline.jl:1467-1498): the
+ * per-step advance_particles_kernel! launch, the per-step sum(dalive), the
+ * reconstruct_mu_step_kernel! launch at save steps and the save-step D2H.
+ *   out_a/out_b: PT_OUT_PHASE -> positions, momenta, each Julia (n,3,nsave_call)
+ *                (C [save][3][n]); PT_OUT_MU -> mu in out_a, Julia (n,nsave_call)
+ *                (C [save][n]); NaN for dead particles as the reference writes.
+ *   out_dtype:   PT_F32 or PT_F64 (trajectory_output_precision /
+ *                mu_cache_output_precision).
+ *   out_capacity_saves: number of save slots the buffers hold.
+ *   alive_counts (host, optional): n_steps values, sum(dalive) after each step.
+ *   n_saved (optional): saves written by this call. */
+int pt_ens_advance(pt_ens *ens, const pt_field *field, const pt_advance_params *params,
+                   void *out_a, void *out_b, int out_dtype, int out_on_device,
+                   int64_t out_capacity_saves, int64_t *alive_counts, int64_t *n_saved);
+
+/* Download the current state (positions in metres, momenta in kg m/s) and
+ * alive flags.  No reference counterpart (the reference never stores the
+ * final state, exact_final_state_stored=false, phase_space_gpu_runner.jl:436); used
+ * by parity tests and checkpointing. */
+int pt_ens_get_state(pt_ens *ens, double *x[3], double *p[3], uint8_t *alive);
+
+/* Kernel time (ms) of the last pt_ens_advance, measured with HIP events on
+ * the context's stream around the launches, and the number of launches. */
+int pt_ens_last_timing(const pt_ens *ens, double *kernel_ms, int64_t *launches);
+
+/* ---- mu reconstruction -------------------------------------------------- */
+
+/* Replaces reconstruct_mu_chunk_gpu / reconstruct_mu_kernel!
+ * (compute_delta_mu2_curve.jl:151-225): mu for every (step, particle) of a
+ * phase-space chunk, with B trilinear from a PLANAR field, in the field's
+ * precision (compute_precision).  positions/momenta: Julia (chunk,3,nsave)
+ * (C [s][3][chunk]) of src_dtype.  mu_out: Julia (nsave, chunk)
+ * (time fastest) of out_dtype. */
+int pt_mu_from_phase(pt_ctx *ctx, const pt_field *field, int64_t chunk, int64_t nsave,
+                     int src_dtype, int src_on_device, const void *positions,
+                     const void *momenta, int out_dtype, int out_on_device, void *mu_out);
+
+/* ---- lag statistics ----------------------------------------------------- */
+
+/* Replaces process_mu_chunk_dmumu_cpu! (compute_delta_mu2_dmumu_full.jl:656-704) and
+ * its two bodies, process_mu_chunk_full_pairs_cpu! (:457-560, start_mode 0 =
+ * sliding) and process_mu_chunk_injection_pairs_cpu! (:562-654, start_mode 1 =
+ * injection).  mu is float32; element (s, j) of the chunk (save s, selected
+ * particle j) is at mu[s*stride_s + col_j*stride_p], where col_j =
+ * particle_index[j] if particle_index is non-null, else j.  The reference's
+ * chunk is Julia (nsave, chunk): stride_s = 1, stride_p = nsave.  A device
+ * mu cache written by pt_ens_advance is [save][n]: stride_s = n, stride_p = 1.
+ * Accumulators are host arrays in the reference's shapes and are updated in
+ * place: counts/bin sums added, the per-lag (count, mean, M2) merged with
+ * Chan's formula (combine_welford_stats!, :439-455).
+ *   per lag (nlags): particle_counts, particle_means, particle_m2s,
+ *                    pair_sum_squares, pair_counts
+ *   Julia (n_bins, n_lags): dmumu_counts, dmumu_sum_delta, dmumu_sum_delta2
+ * mu_edges: n_bins+1 FP64 edges (build_mu_edges_full, :289-302). */
+int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t nsave,
+                        int64_t n_sel, int64_t stride_s, int64_t stride_p,
+                        const int64_t *particle_index, const int64_t *lag_steps, int64_t nlags,
+                        const double *mu_edges, int64_t n_bins, int start_mode, int bin_abs,
+                        int64_t *particle_counts, double *particle_means, double *particle_m2s,
+                        double *pair_sum_squares, int64_t *pair_counts, int64_t *dmumu_counts,
+                        double *dmumu_sum_delta, double *dmumu_sum_delta2);
+
+/* Replaces process_momenta_chunk_dpp_cpu! (compute_dpp_full.jl:247-288).
+ * momenta: float32, Julia (total,3,nsave) (C [s][3][total]) with the chunk's
+ * selected columns given by particle_index (or 0..n_sel-1).  |p| in FP64
+ * (momentum_magnitude, :232-234); sums in FP64; updated in place. */
+int pt_dpp_accumulate(pt_ctx *ctx, const float *momenta, int on_device, int64_t nsave,
+                      int64_t total, int64_t n_sel, const int64_t *particle_index,
+                      const int64_t *lag_steps, int64_t nlags, double p0, int64_t *counts,
+                      double *sum_delta_p, double *sum_delta_p2, double *sum_delta_p_norm,
+                      double *sum_delta_p_norm2);
+
+/* Replaces fill_energy_snapshots! (compute_dpp_full.jl:290-299): kinetic energy
+ * in GeV of each selected particle at each snapshot save index (0-based),
+ * written into out, Julia (k, out_cols) column-major, at column col0 + j. */
+int pt_energy_snapshots(pt_ctx *ctx, const float *momenta, int on_device, int64_t nsave,
+                        int64_t total, int64_t n_sel, const int64_t *particle_index,
+                        const int64_t *snapshot_idx, int64_t k, double *out, int64_t col0);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PT_HIP_H */

@@ -1,0 +1,114 @@
+"""ctypes binding of libpt_hip.so (the C ABI declared in include/pt_hip.h).
+
+The library must exist (build it with ``python -m particle_transport_amd.build``):
+there is no CPU fallback, and every call raises ``PTError`` with the
+library's own message when a status is non-zero.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpt_hip.so")
+
+PT_OK, PT_ERR_INVALID, PT_ERR_HIP, PT_ERR_MEMORY, PT_ERR_STATE = 0, 1, 2, 3, 4
+PT_F64, PT_F32 = 0, 1
+STORAGE = {"f64_planar": 0, "f32_planar": 1, "f32_brick": 2, "bf16_brick": 3, "f16_brick": 4}
+PUSHER = {"ref_f64": 0, "plain_f32": 1, "fast": 2}
+OUT = {"none": 0, "phase": 1, "phase-space": 1, "mu": 2}
+
+# Every symbol include/pt_hip.h declares.
+EXPORTS = (
+    "pt_abi_version", "pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_ctx_bytes_in_use",
+    "pt_field_create", "pt_field_destroy", "pt_field_mean_abs_b", "pt_mean_abs_b",
+    "pt_ens_create", "pt_ens_destroy", "pt_ens_advance", "pt_ens_get_state", "pt_ens_last_timing",
+    "pt_mu_from_phase", "pt_dmumu_accumulate", "pt_dpp_accumulate", "pt_energy_snapshots",
+)
+
+
+class PTError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libpt_hip error {code}: {msg}")
+        self.code = code
+
+
+class AdvanceParams(ctypes.Structure):
+    _fields_ = [("dt_s", ctypes.c_double), ("charge_C", ctypes.c_double), ("mass_kg", ctypes.c_double),
+                ("first_step", ctypes.c_int64), ("n_steps", ctypes.c_int64), ("total_steps", ctypes.c_int64),
+                ("save_stride", ctypes.c_int64), ("out_kind", ctypes.c_int32),
+                ("steps_per_launch", ctypes.c_int32)]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_int = ctypes.c_int
+
+
+def load():
+    """Load libpt_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: build it with `python -m particle_transport_amd.build` "
+            "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "pt_abi_version": (_int, []),
+        "pt_last_error": (ctypes.c_char_p, []),
+        "pt_ctx_create": (_int, [_int, ctypes.POINTER(_vp)]),
+        "pt_ctx_destroy": (_int, [_vp]),
+        "pt_ctx_bytes_in_use": (_int, [_vp, _i64p]),
+        "pt_field_create": (_int, [_vp, _i64, _i64, _i64, ctypes.c_double, _int, _int, _int,
+                                   ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+        "pt_field_destroy": (_int, [_vp]),
+        "pt_field_mean_abs_b": (_int, [_vp, _dp]),
+        "pt_mean_abs_b": (_int, [_vp, _i64, _int, _int, _vp, _vp, _vp, _dp]),
+        "pt_ens_create": (_int, [_vp, _vp, _int, _i64, _int, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                 _int, ctypes.POINTER(_vp)]),
+        "pt_ens_destroy": (_int, [_vp]),
+        "pt_ens_advance": (_int, [_vp, _vp, ctypes.POINTER(AdvanceParams), _vp, _vp, _int, _int, _i64,
+                                  _i64p, _i64p]),
+        "pt_ens_get_state": (_int, [_vp, ctypes.POINTER(_dp), ctypes.POINTER(_dp), ctypes.POINTER(ctypes.c_uint8)]),
+        "pt_ens_last_timing": (_int, [_vp, _dp, _i64p]),
+        "pt_mu_from_phase": (_int, [_vp, _vp, _i64, _i64, _int, _int, _vp, _vp, _int, _int, _vp]),
+        "pt_dmumu_accumulate": (_int, [_vp, _fp, _int, _i64, _i64, _i64, _i64, _i64p, _i64p, _i64, _dp, _i64,
+                                       _int, _int, _i64p, _dp, _dp, _dp, _i64p, _i64p, _dp, _dp]),
+        "pt_dpp_accumulate": (_int, [_vp, _fp, _int, _i64, _i64, _i64, _i64p, _i64p, _i64, ctypes.c_double,
+                                     _i64p, _dp, _dp, _dp, _dp]),
+        "pt_energy_snapshots": (_int, [_vp, _fp, _int, _i64, _i64, _i64, _i64p, _i64p, _i64, _dp, _i64]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != PT_OK:
+        msg = load().pt_last_error()
+        raise PTError(status, msg.decode() if msg else "unknown error")
+
+
+def ptr(a: np.ndarray, t=_vp):
+    if a is None:
+        return None
+    if t is _vp:
+        return ctypes.c_void_p(a.ctypes.data)
+    return a.ctypes.data_as(t)
+
+
+def exported_symbols():
+    """Names of the C-ABI symbols present in the library (no GPU needed)."""
+    L = ctypes.CDLL(LIB_PATH)
+    return [n for n in EXPORTS if hasattr(L, n)]

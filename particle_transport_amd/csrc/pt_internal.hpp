@@ -1,0 +1,170 @@
+// Internal declarations shared by the libpt_hip.so translation units.
+// Host handles, error plumbing, launch entry points of each kernel file and
+// small device helpers (brick addressing, BF16/FP16 decode).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/pt_hip.h"
+
+namespace pt {
+
+// Physical constants of the reference (phase_space_gpu_runner.jl:8-10).
+constexpr double kC = 2.99792458e8;
+constexpr double kQe = 1.602176634e-19;
+constexpr double kMp = 1.67262192369e-27;
+constexpr double kGeVtoJ = 1e9 * 1.602176634e-19;
+
+// ---------------------------------------------------------------- errors ----
+void set_error(const std::string &msg);
+struct Error {
+    int code;
+    std::string msg;
+};
+#define PT_HIP_CHECK(expr)                                                                 \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            throw ::pt::Error{PT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
+    } while (0)
+#define PT_REQUIRE(cond, msg)                                  \
+    do {                                                       \
+        if (!(cond)) throw ::pt::Error{PT_ERR_INVALID, (msg)}; \
+    } while (0)
+
+// ------------------------------------------------------------ geometry -----
+// Grid of the reference: n nodes on [0, L] per axis, x_i = range(0, L, length=n)
+// (phase_space_gpu_runner.jl:125-130), n-1 cells.  Bricks group nodes in 8^3 blocks.
+struct Geom {
+    int64_t n[3];      // nodes per axis
+    double L;          // box length (m)
+    double lo[3];      // xmin (0)
+    double hi[3];      // xmax (L)
+    double d[3];       // node spacing x[2]-x[1]
+    int64_t nb[3];     // bricks per axis = ceil(n/8)
+};
+
+struct FieldScale {
+    float scale[6];    // decoded value = stored * scale + offset
+    float offset[6];
+};
+
+}  // namespace pt
+
+struct pt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int64_t bytes_in_use = 0;
+    void *scratch = nullptr;     // reusable device scratch (grown on demand)
+    size_t scratch_bytes = 0;
+};
+
+struct pt_field {
+    pt_ctx *ctx = nullptr;
+    pt::Geom g{};
+    int storage = 0;
+    void *data = nullptr;        // planar: 6 consecutive component arrays; brick: records
+    size_t bytes = 0;
+    pt::FieldScale sc{};         // brick storages only
+    double B0 = 0.0;             // FP64 mean |B| of the source values
+};
+
+struct pt_ens {
+    pt_ctx *ctx = nullptr;
+    const pt_field *field = nullptr;
+    int pusher = 0;
+    int64_t n = 0;
+    int periodic = 1;
+    int64_t n_alive = 0;
+    void *state = nullptr;       // see SplitState / RefState below
+    size_t state_bytes = 0;
+    uint8_t *alive = nullptr;
+    int64_t *deaths = nullptr;   // device per-step death counters
+    int64_t deaths_cap = 0;
+    double kernel_ms = 0.0;
+    int64_t launches = 0;
+};
+
+namespace pt {
+
+// Split-precision particle state (structure of arrays, n elements each):
+// int32 cell[3], float off[3] in [0,1) cell units, float P[3] = p / (M_P c).
+struct SplitState {
+    int32_t *cell[3];
+    float *off[3];
+    float *P[3];
+};
+// FP64 reference-structured state: x in metres, p in kg m/s.
+struct RefState {
+    double *x[3];
+    double *p[3];
+};
+
+inline SplitState split_state(pt_ens *e) {
+    SplitState s;
+    char *b = static_cast<char *>(e->state);
+    size_t n = static_cast<size_t>(e->n);
+    for (int c = 0; c < 3; ++c) s.cell[c] = reinterpret_cast<int32_t *>(b + c * n * 4);
+    for (int c = 0; c < 3; ++c) s.off[c] = reinterpret_cast<float *>(b + (3 + c) * n * 4);
+    for (int c = 0; c < 3; ++c) s.P[c] = reinterpret_cast<float *>(b + (6 + c) * n * 4);
+    return s;
+}
+inline RefState ref_state(pt_ens *e) {
+    RefState s;
+    char *b = static_cast<char *>(e->state);
+    size_t n = static_cast<size_t>(e->n);
+    for (int c = 0; c < 3; ++c) s.x[c] = reinterpret_cast<double *>(b + c * n * 8);
+    for (int c = 0; c < 3; ++c) s.p[c] = reinterpret_cast<double *>(b + (3 + c) * n * 8);
+    return s;
+}
+
+// Per-call push parameters after host-side derivation.
+struct PushArgs {
+    int64_t first_step, n_steps, total_steps, stride;
+    int64_t save_base;       // global 0-based index of the first save at/after first_step
+    int out_kind;
+    int out_f64;             // output element type: 1 double, 0 float
+    void *out_a, *out_b;     // device output buffers [save][3][n] / [save][n]
+    int64_t out_cap;         // save slots in out buffers
+    double dt, q, m;
+    int steps_per_launch;
+};
+
+// Launchers (one per kernel file).  Each enqueues on e->ctx->stream and
+// returns the number of kernel launches.
+int64_t launch_push_ref_f64(pt_ens *e, const pt_field *f, const PushArgs &a);
+int64_t launch_push_plain_f32(pt_ens *e, const pt_field *f, const PushArgs &a);
+int64_t launch_push_fast(pt_ens *e, const pt_field *f, const PushArgs &a);
+
+void launch_pack_field(pt_field *f, const void *const src[6], int src_f64, hipStream_t s);
+void launch_mean_abs_b(pt_ctx *ctx, int64_t n, int src_f64, const void *bx, const void *by,
+                       const void *bz, double *out_host);
+void launch_component_stats(pt_ctx *ctx, int64_t n, int src_f64, const void *src, double *mean,
+                            double *maxdev);
+void launch_split_from_si(pt_ens *e, const double *dx, const double *dp);
+void launch_split_to_si(pt_ens *e, double *dx, double *dp);
+
+// Device scratch owned by the context, grown on demand (never shrinks).
+void *ctx_scratch(pt_ctx *ctx, size_t bytes);
+
+// -------------------------------------------------------- device helpers ---
+__host__ __device__ inline int64_t brick_node_index(const int64_t nb[3], int64_t i, int64_t j,
+                                                    int64_t k) {
+    int64_t brick = ((k >> 3) * nb[1] + (j >> 3)) * nb[0] + (i >> 3);
+    return (brick << 9) + (((k & 7) << 6) | ((j & 7) << 3) | (i & 7));
+}
+
+__device__ inline float bf16_bits_to_float(uint16_t h) {
+    return __uint_as_float(static_cast<uint32_t>(h) << 16);
+}
+__device__ inline float f16_bits_to_float(uint16_t h) {
+    return __half2float(__ushort_as_half(h));
+}
+
+}  // namespace pt

@@ -1,0 +1,738 @@
+// Host runtime of libpt_hip.so: contexts, fields, ensembles and the
+// pt_ens_advance driver, plus the set-up kernels they need (field packing,
+// FP64 mean |B|, split-state conversion).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pt_internal.hpp"
+
+namespace pt {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+template <class F>
+static int guarded(F &&fn) {
+    try {
+        fn();
+        return PT_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return PT_ERR_MEMORY;
+    } catch (...) {
+        set_error("unknown C++ exception");
+        return PT_ERR_STATE;
+    }
+}
+
+static void *dev_alloc(pt_ctx *ctx, size_t bytes) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{PT_ERR_MEMORY, "hipMalloc(" + std::to_string(bytes) + " bytes) failed: " +
+                                       hipGetErrorString(e)};
+    }
+    ctx->bytes_in_use += static_cast<int64_t>(bytes);
+    return p;
+}
+static void dev_free(pt_ctx *ctx, void *p, size_t bytes) {
+    if (p) {
+        (void)hipFree(p);
+        ctx->bytes_in_use -= static_cast<int64_t>(bytes);
+    }
+}
+
+void *ctx_scratch(pt_ctx *ctx, size_t bytes) {
+    if (bytes > ctx->scratch_bytes) {
+        PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        dev_free(ctx, ctx->scratch, ctx->scratch_bytes);
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+        size_t grow = std::max(bytes, static_cast<size_t>(1) << 20);
+        ctx->scratch = dev_alloc(ctx, grow);
+        ctx->scratch_bytes = grow;
+    }
+    return ctx->scratch;
+}
+
+// ------------------------------------------------------------- kernels -----
+
+// Planar copy/convert of one component into the field's storage array.
+template <class Src, class Dst>
+__global__ void k_copy_convert(const Src *__restrict__ src, Dst *__restrict__ dst, int64_t n) {
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = static_cast<Dst>(src[i]);
+}
+
+// Brick packing of component c: node (i,j,k) of the Julia (nx,ny,nz) array
+// goes to record brick_node_index(i,j,k), slot c.  rec_words = 8 elements.
+template <class Src>
+__global__ void k_pack_brick_f32(const Src *__restrict__ src, float *__restrict__ dst, int c,
+                                 int64_t nx, int64_t ny, int64_t nz, int64_t nbx, int64_t nby,
+                                 int64_t nbz) {
+    int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t n = nx * ny * nz;
+    if (idx >= n) return;
+    int64_t i = idx % nx, j = (idx / nx) % ny, k = idx / (nx * ny);
+    int64_t nb[3] = {nbx, nby, nbz};
+    dst[brick_node_index(nb, i, j, k) * 8 + c] = static_cast<float>(src[idx]);
+}
+
+template <class Src, bool BF16>
+__global__ void k_pack_brick_16(const Src *__restrict__ src, uint16_t *__restrict__ dst, int c,
+                                int64_t nx, int64_t ny, int64_t nz, int64_t nbx, int64_t nby,
+                                int64_t nbz, float inv_scale, double offset) {
+    int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t n = nx * ny * nz;
+    if (idx >= n) return;
+    int64_t i = idx % nx, j = (idx / nx) % ny, k = idx / (nx * ny);
+    int64_t nb[3] = {nbx, nby, nbz};
+    float v = static_cast<float>((static_cast<double>(src[idx]) - offset)) * inv_scale;
+    uint16_t bits;
+    if (BF16) {
+        bits = __bfloat16_as_ushort(__float2bfloat16(v));  // round to nearest even
+    } else {
+        bits = __half_as_ushort(__float2half_rn(v));
+    }
+    dst[brick_node_index(nb, i, j, k) * 8 + c] = bits;
+}
+
+// Block-level FP64 sum of |B| with a fixed tree; per-block partials are then
+// summed in block order by k_sum_partials, so the result is deterministic.
+template <class Src>
+__global__ void k_abs_b_partials(const Src *__restrict__ bx, const Src *__restrict__ by,
+                                 const Src *__restrict__ bz, int64_t n, double *__restrict__ part) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double x = bx[i], y = by[i], z = bz[i];
+        acc += sqrt(x * x + y * y + z * z);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_partials(const double *__restrict__ part, int nparts, double *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < nparts; ++i) s += part[i];
+        out[0] = s;
+    }
+}
+
+// Per-component mean (FP64) and min/max for BF16/FP16 scale selection.
+template <class Src>
+__global__ void k_comp_partials(const Src *__restrict__ v, int64_t n, double *__restrict__ psum,
+                                double *__restrict__ pmin, double *__restrict__ pmax) {
+    __shared__ double rs[256], rmin[256], rmax[256];
+    double s = 0.0, lo = INFINITY, hi = -INFINITY;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double x = v[i];
+        s += x;
+        lo = fmin(lo, x);
+        hi = fmax(hi, x);
+    }
+    rs[threadIdx.x] = s;
+    rmin[threadIdx.x] = lo;
+    rmax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            rs[threadIdx.x] += rs[threadIdx.x + w];
+            rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + w]);
+            rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        psum[blockIdx.x] = rs[0];
+        pmin[blockIdx.x] = rmin[0];
+        pmax[blockIdx.x] = rmax[0];
+    }
+}
+
+// SI state -> split state.  cell = floor(x/d), off = x/d - cell rounded to
+// FP32 (an offset that rounds up to 1 moves to the next cell at offset 0).
+__global__ void k_split_from_si(const double *__restrict__ x, const double *__restrict__ p,
+                                int64_t n, SplitState s, double d0, double d1, double d2,
+                                double pnorm) {
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d[3] = {d0, d1, d2};
+    for (int c = 0; c < 3; ++c) {
+        double u = x[c * n + i] / d[c];
+        double fl = floor(u);
+        float off = static_cast<float>(u - fl);
+        int32_t cell = static_cast<int32_t>(fl);
+        if (off >= 1.0f) {
+            off = 0.0f;
+            cell += 1;
+        }
+        s.cell[c][i] = cell;
+        s.off[c][i] = off;
+        s.P[c][i] = static_cast<float>(p[c * n + i] / pnorm);
+    }
+}
+
+__global__ void k_split_to_si(SplitState s, int64_t n, double *__restrict__ x,
+                              double *__restrict__ p, double d0, double d1, double d2,
+                              double pnorm) {
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d[3] = {d0, d1, d2};
+    for (int c = 0; c < 3; ++c) {
+        x[c * n + i] = (static_cast<double>(s.cell[c][i]) + static_cast<double>(s.off[c][i])) * d[c];
+        p[c * n + i] = static_cast<double>(s.P[c][i]) * pnorm;
+    }
+}
+
+static inline unsigned grid_for(int64_t n, int block = 256) {
+    return static_cast<unsigned>((n + block - 1) / block);
+}
+
+void launch_pack_field(pt_field *f, const void *const src[6], int src_f64, hipStream_t s) {
+    const int64_t nx = f->g.n[0], ny = f->g.n[1], nz = f->g.n[2];
+    const int64_t n = nx * ny * nz;
+    for (int c = 0; c < 6; ++c) {
+        if (!src[c]) continue;  // components not present in this pass
+        switch (f->storage) {
+            case PT_STORE_F64_PLANAR: {
+                double *dst = static_cast<double *>(f->data) + c * n;
+                if (src_f64)
+                    k_copy_convert<<<grid_for(n), 256, 0, s>>>(static_cast<const double *>(src[c]), dst, n);
+                else
+                    k_copy_convert<<<grid_for(n), 256, 0, s>>>(static_cast<const float *>(src[c]), dst, n);
+                break;
+            }
+            case PT_STORE_F32_PLANAR: {
+                float *dst = static_cast<float *>(f->data) + c * n;
+                if (src_f64)
+                    k_copy_convert<<<grid_for(n), 256, 0, s>>>(static_cast<const double *>(src[c]), dst, n);
+                else
+                    k_copy_convert<<<grid_for(n), 256, 0, s>>>(static_cast<const float *>(src[c]), dst, n);
+                break;
+            }
+            case PT_STORE_F32_BRICK: {
+                float *dst = static_cast<float *>(f->data);
+                if (src_f64)
+                    k_pack_brick_f32<<<grid_for(n), 256, 0, s>>>(static_cast<const double *>(src[c]), dst, c,
+                                                                nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2]);
+                else
+                    k_pack_brick_f32<<<grid_for(n), 256, 0, s>>>(static_cast<const float *>(src[c]), dst, c,
+                                                                nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2]);
+                break;
+            }
+            case PT_STORE_BF16_BRICK:
+            case PT_STORE_F16_BRICK: {
+                uint16_t *dst = static_cast<uint16_t *>(f->data);
+                float inv = 1.0f / f->sc.scale[c];
+                double off = f->sc.offset[c];
+                bool bf = f->storage == PT_STORE_BF16_BRICK;
+                if (src_f64) {
+                    auto *sp = static_cast<const double *>(src[c]);
+                    if (bf)
+                        k_pack_brick_16<double, true><<<grid_for(n), 256, 0, s>>>(sp, dst, c, nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2], inv, off);
+                    else
+                        k_pack_brick_16<double, false><<<grid_for(n), 256, 0, s>>>(sp, dst, c, nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2], inv, off);
+                } else {
+                    auto *sp = static_cast<const float *>(src[c]);
+                    if (bf)
+                        k_pack_brick_16<float, true><<<grid_for(n), 256, 0, s>>>(sp, dst, c, nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2], inv, off);
+                    else
+                        k_pack_brick_16<float, false><<<grid_for(n), 256, 0, s>>>(sp, dst, c, nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2], inv, off);
+                }
+                break;
+            }
+        }
+        PT_HIP_CHECK(hipGetLastError());
+    }
+}
+
+void launch_mean_abs_b(pt_ctx *ctx, int64_t n, int src_f64, const void *bx, const void *by,
+                       const void *bz, double *out_host) {
+    const int nparts = 1024;
+    double *part = static_cast<double *>(ctx_scratch(ctx, (nparts + 1) * sizeof(double)));
+    if (src_f64)
+        k_abs_b_partials<<<nparts, 256, 0, ctx->stream>>>(static_cast<const double *>(bx),
+                                                          static_cast<const double *>(by),
+                                                          static_cast<const double *>(bz), n, part);
+    else
+        k_abs_b_partials<<<nparts, 256, 0, ctx->stream>>>(static_cast<const float *>(bx),
+                                                          static_cast<const float *>(by),
+                                                          static_cast<const float *>(bz), n, part);
+    PT_HIP_CHECK(hipGetLastError());
+    k_sum_partials<<<1, 64, 0, ctx->stream>>>(part, nparts, part + nparts);
+    PT_HIP_CHECK(hipGetLastError());
+    double sum = 0.0;
+    PT_HIP_CHECK(hipMemcpyAsync(&sum, part + nparts, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out_host = sum / static_cast<double>(n);
+}
+
+void launch_component_stats(pt_ctx *ctx, int64_t n, int src_f64, const void *src, double *mean,
+                            double *maxdev) {
+    const int nparts = 512;
+    double *part = static_cast<double *>(ctx_scratch(ctx, 3 * nparts * sizeof(double)));
+    if (src_f64)
+        k_comp_partials<<<nparts, 256, 0, ctx->stream>>>(static_cast<const double *>(src), n, part,
+                                                         part + nparts, part + 2 * nparts);
+    else
+        k_comp_partials<<<nparts, 256, 0, ctx->stream>>>(static_cast<const float *>(src), n, part,
+                                                         part + nparts, part + 2 * nparts);
+    PT_HIP_CHECK(hipGetLastError());
+    std::vector<double> h(3 * nparts);
+    PT_HIP_CHECK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    double s = 0.0, lo = INFINITY, hi = -INFINITY;
+    for (int i = 0; i < nparts; ++i) {
+        s += h[i];
+        lo = std::min(lo, h[nparts + i]);
+        hi = std::max(hi, h[2 * nparts + i]);
+    }
+    *mean = s / static_cast<double>(n);
+    *maxdev = std::max(hi - *mean, *mean - lo);
+}
+
+void launch_split_from_si(pt_ens *e, const double *dx, const double *dp) {
+    SplitState s = split_state(e);
+    const Geom &g = e->field->g;
+    k_split_from_si<<<grid_for(e->n), 256, 0, e->ctx->stream>>>(dx, dp, e->n, s, g.d[0], g.d[1],
+                                                               g.d[2], kMp * kC);
+    PT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_split_to_si(pt_ens *e, double *dx, double *dp) {
+    SplitState s = split_state(e);
+    const Geom &g = e->field->g;
+    k_split_to_si<<<grid_for(e->n), 256, 0, e->ctx->stream>>>(s, e->n, dx, dp, g.d[0], g.d[1], g.d[2],
+                                                             kMp * kC);
+    PT_HIP_CHECK(hipGetLastError());
+}
+
+static size_t storage_bytes(int storage, const Geom &g) {
+    const size_t n = static_cast<size_t>(g.n[0] * g.n[1] * g.n[2]);
+    const size_t nbrick = static_cast<size_t>(g.nb[0] * g.nb[1] * g.nb[2]) * 512;
+    switch (storage) {
+        case PT_STORE_F64_PLANAR: return 6 * n * 8;
+        case PT_STORE_F32_PLANAR: return 6 * n * 4;
+        case PT_STORE_F32_BRICK: return nbrick * 32;
+        case PT_STORE_BF16_BRICK:
+        case PT_STORE_F16_BRICK: return nbrick * 16;
+    }
+    throw Error{PT_ERR_INVALID, "unknown field storage " + std::to_string(storage)};
+}
+
+static int64_t saves_in_range(int64_t first, int64_t count, int64_t stride) {
+    // number of si in [first, first+count) with (si-1) % stride == 0
+    if (count <= 0) return 0;
+    int64_t a = first - 1, b = first + count - 2;  // 0-based range [a, b]
+    int64_t lo = (a + stride - 1) / stride;
+    int64_t hi = b / stride;
+    return hi >= lo ? hi - lo + 1 : 0;
+}
+
+}  // namespace pt
+
+using namespace pt;
+
+extern "C" {
+
+int pt_abi_version(void) { return PT_ABI_VERSION; }
+
+const char *pt_last_error(void) { return pt::g_last_error.c_str(); }
+
+int pt_ctx_create(int device, pt_ctx **out) {
+    return guarded([&] {
+        PT_REQUIRE(out != nullptr, "pt_ctx_create: out is null");
+        int count = 0;
+        PT_HIP_CHECK(hipGetDeviceCount(&count));
+        PT_REQUIRE(device >= 0 && device < count, "pt_ctx_create: device ordinal out of range");
+        PT_HIP_CHECK(hipSetDevice(device));
+        auto *c = new pt_ctx();
+        c->device = device;
+        // Default-priority, non-blocking stream owned by the context.
+        PT_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        PT_HIP_CHECK(hipEventCreate(&c->ev0));
+        PT_HIP_CHECK(hipEventCreate(&c->ev1));
+        *out = c;
+    });
+}
+
+int pt_ctx_destroy(pt_ctx *ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->scratch) (void)hipFree(ctx->scratch);
+        (void)hipEventDestroy(ctx->ev0);
+        (void)hipEventDestroy(ctx->ev1);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int pt_ctx_bytes_in_use(const pt_ctx *ctx, int64_t *bytes) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && bytes, "pt_ctx_bytes_in_use: null argument");
+        *bytes = ctx->bytes_in_use;
+    });
+}
+
+int pt_field_create(pt_ctx *ctx, int64_t nx, int64_t ny, int64_t nz, double box_length_m,
+                    int storage, int src_dtype, int src_on_device, const void *const comps[6],
+                    pt_field **out) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && comps && out, "pt_field_create: null argument");
+        PT_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "pt_field_create: need at least 2 nodes per axis");
+        PT_REQUIRE(nx < (1LL << 30) && ny < (1LL << 30) && nz < (1LL << 30), "pt_field_create: grid too large");
+        PT_REQUIRE(std::isfinite(box_length_m) && box_length_m > 0, "pt_field_create: box length must be finite and positive");
+        PT_REQUIRE(src_dtype == PT_F64 || src_dtype == PT_F32, "pt_field_create: src_dtype must be PT_F64 or PT_F32");
+        for (int c = 0; c < 6; ++c) PT_REQUIRE(comps[c] != nullptr, "pt_field_create: null component");
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        auto *f = new pt_field();
+        f->ctx = ctx;
+        f->storage = storage;
+        int64_t nn[3] = {nx, ny, nz};
+        for (int c = 0; c < 3; ++c) {
+            f->g.n[c] = nn[c];
+            f->g.lo[c] = 0.0;
+            f->g.hi[c] = box_length_m;
+            f->g.d[c] = box_length_m / static_cast<double>(nn[c] - 1);
+            f->g.nb[c] = (nn[c] + 7) / 8;
+        }
+        f->g.L = box_length_m;
+        void *staging = nullptr;
+        size_t staging_bytes = 0;
+        try {
+            f->bytes = storage_bytes(storage, f->g);
+            const int64_t n = nx * ny * nz;
+            const size_t esz = src_dtype == PT_F64 ? 8 : 4;
+            // Host sources are staged three components at a time in a
+            // temporary device buffer of their own (the context scratch is
+            // used by the reductions below).
+            if (!src_on_device) {
+                staging_bytes = 3 * static_cast<size_t>(n) * esz;
+                staging = dev_alloc(ctx, staging_bytes);
+            }
+            f->data = dev_alloc(ctx, f->bytes);
+            PT_HIP_CHECK(hipMemsetAsync(f->data, 0, f->bytes, ctx->stream));
+            for (int half = 0; half < 2; ++half) {
+                const void *dsrc[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+                for (int c = 0; c < 3; ++c) {
+                    const int comp = 3 * half + c;
+                    if (src_on_device) {
+                        dsrc[comp] = comps[comp];
+                    } else {
+                        void *dst = static_cast<char *>(staging) + c * n * esz;
+                        PT_HIP_CHECK(hipMemcpyAsync(dst, comps[comp], n * esz, hipMemcpyHostToDevice, ctx->stream));
+                        dsrc[comp] = dst;
+                    }
+                }
+                if (half == 0)
+                    launch_mean_abs_b(ctx, n, src_dtype == PT_F64, dsrc[0], dsrc[1], dsrc[2], &f->B0);
+                for (int c = 0; c < 3; ++c) {
+                    const int comp = 3 * half + c;
+                    if (storage == PT_STORE_BF16_BRICK || storage == PT_STORE_F16_BRICK) {
+                        double mean = 0, maxdev = 0;
+                        launch_component_stats(ctx, n, src_dtype == PT_F64, dsrc[comp], &mean, &maxdev);
+                        f->sc.offset[comp] = static_cast<float>(mean);
+                        f->sc.scale[comp] = maxdev > 0 ? static_cast<float>(maxdev) : 1.0f;
+                    } else {
+                        f->sc.offset[comp] = 0.0f;
+                        f->sc.scale[comp] = 1.0f;
+                    }
+                }
+                launch_pack_field(f, dsrc, src_dtype == PT_F64, ctx->stream);
+                PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            }
+            dev_free(ctx, staging, staging_bytes);
+            staging = nullptr;
+        } catch (...) {
+            (void)hipStreamSynchronize(ctx->stream);
+            dev_free(ctx, staging, staging_bytes);
+            if (f->data) dev_free(ctx, f->data, f->bytes);
+            delete f;
+            throw;
+        }
+        *out = f;
+    });
+}
+
+int pt_field_destroy(pt_field *field) {
+    return guarded([&] {
+        if (!field) return;
+        (void)hipStreamSynchronize(field->ctx->stream);
+        dev_free(field->ctx, field->data, field->bytes);
+        delete field;
+    });
+}
+
+int pt_field_mean_abs_b(const pt_field *field, double *B0_T) {
+    return guarded([&] {
+        PT_REQUIRE(field && B0_T, "pt_field_mean_abs_b: null argument");
+        *B0_T = field->B0;
+    });
+}
+
+int pt_mean_abs_b(pt_ctx *ctx, int64_t n, int src_dtype, int src_on_device, const void *bx,
+                  const void *by, const void *bz, double *B0_T) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && bx && by && bz && B0_T, "pt_mean_abs_b: null argument");
+        PT_REQUIRE(n > 0, "pt_mean_abs_b: n must be positive");
+        PT_REQUIRE(src_dtype == PT_F64 || src_dtype == PT_F32, "pt_mean_abs_b: bad dtype");
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        const size_t esz = src_dtype == PT_F64 ? 8 : 4;
+        if (src_on_device) {
+            launch_mean_abs_b(ctx, n, src_dtype == PT_F64, bx, by, bz, B0_T);
+            return;
+        }
+        // Stage after the reduction's partials (scratch head is used by the reduction).
+        const size_t head = 4096 * sizeof(double);
+        char *st = static_cast<char *>(ctx_scratch(ctx, head + 3 * n * esz)) + head;
+        PT_HIP_CHECK(hipMemcpyAsync(st, bx, n * esz, hipMemcpyHostToDevice, ctx->stream));
+        PT_HIP_CHECK(hipMemcpyAsync(st + n * esz, by, n * esz, hipMemcpyHostToDevice, ctx->stream));
+        PT_HIP_CHECK(hipMemcpyAsync(st + 2 * n * esz, bz, n * esz, hipMemcpyHostToDevice, ctx->stream));
+        launch_mean_abs_b(ctx, n, src_dtype == PT_F64, st, st + n * esz, st + 2 * n * esz, B0_T);
+    });
+}
+
+int pt_ens_create(pt_ctx *ctx, const pt_field *field, int pusher, int64_t n, int src_dtype,
+                  int src_on_device, const void *const x[3], const void *const p[3], int periodic,
+                  pt_ens **out) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && field && x && p && out, "pt_ens_create: null argument");
+        PT_REQUIRE(n >= 1 && n < (1LL << 31), "pt_ens_create: particle count out of range");
+        PT_REQUIRE(src_dtype == PT_F64 || src_dtype == PT_F32, "pt_ens_create: bad dtype");
+        PT_REQUIRE(pusher == PT_PUSH_REF_F64 || pusher == PT_PUSH_PLAIN_F32 || pusher == PT_PUSH_FAST,
+                   "pt_ens_create: unknown pusher");
+        if (pusher == PT_PUSH_REF_F64)
+            PT_REQUIRE(field->storage == PT_STORE_F64_PLANAR, "PT_PUSH_REF_F64 needs a PT_STORE_F64_PLANAR field");
+        if (pusher == PT_PUSH_PLAIN_F32)
+            PT_REQUIRE(field->storage == PT_STORE_F32_PLANAR, "PT_PUSH_PLAIN_F32 needs a PT_STORE_F32_PLANAR field");
+        if (pusher == PT_PUSH_FAST)
+            PT_REQUIRE(field->storage != PT_STORE_F64_PLANAR, "PT_PUSH_FAST needs an FP32 or 16-bit field");
+        for (int c = 0; c < 3; ++c) PT_REQUIRE(x[c] && p[c], "pt_ens_create: null state component");
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+
+        // Gather the state as FP64 on the host, check it, stage it on the device.
+        const size_t N = static_cast<size_t>(n);
+        std::vector<double> h(6 * N);
+        const size_t esz = src_dtype == PT_F64 ? 8 : 4;
+        std::vector<char> tmp(src_on_device ? N * esz : 0);
+        for (int c = 0; c < 6; ++c) {
+            const void *srcp = c < 3 ? x[c] : p[c - 3];
+            const void *hsrc = srcp;
+            if (src_on_device) {
+                PT_HIP_CHECK(hipMemcpy(tmp.data(), srcp, N * esz, hipMemcpyDeviceToHost));
+                hsrc = tmp.data();
+            }
+            for (size_t i = 0; i < N; ++i)
+                h[c * N + i] = src_dtype == PT_F64 ? static_cast<const double *>(hsrc)[i]
+                                                   : static_cast<const float *>(hsrc)[i];
+        }
+        const Geom &g = field->g;
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < N; ++i) {
+                double v = h[c * N + i];
+                PT_REQUIRE(std::isfinite(v) && v >= g.lo[c] && v <= g.hi[c],
+                           "pt_ens_create: initial position outside the simulation box");
+                PT_REQUIRE(std::isfinite(h[(3 + c) * N + i]), "pt_ens_create: non-finite momentum");
+            }
+
+        auto *e = new pt_ens();
+        e->ctx = ctx;
+        e->field = field;
+        e->pusher = pusher;
+        e->n = n;
+        e->periodic = periodic ? 1 : 0;
+        e->n_alive = n;
+        try {
+            e->state_bytes = pusher == PT_PUSH_REF_F64 ? 6 * N * 8 : 9 * N * 4;
+            e->state = dev_alloc(ctx, e->state_bytes);
+            e->alive = static_cast<uint8_t *>(dev_alloc(ctx, N));
+            PT_HIP_CHECK(hipMemsetAsync(e->alive, 1, N, ctx->stream));
+            if (pusher == PT_PUSH_REF_F64) {
+                PT_HIP_CHECK(hipMemcpyAsync(e->state, h.data(), 6 * N * 8, hipMemcpyHostToDevice, ctx->stream));
+            } else {
+                double *st = static_cast<double *>(ctx_scratch(ctx, 6 * N * 8));
+                PT_HIP_CHECK(hipMemcpyAsync(st, h.data(), 6 * N * 8, hipMemcpyHostToDevice, ctx->stream));
+                launch_split_from_si(e, st, st + 3 * N);
+            }
+            PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            dev_free(ctx, e->state, e->state_bytes);
+            dev_free(ctx, e->alive, N);
+            delete e;
+            throw;
+        }
+        *out = e;
+    });
+}
+
+int pt_ens_destroy(pt_ens *ens) {
+    return guarded([&] {
+        if (!ens) return;
+        pt_ctx *ctx = ens->ctx;
+        (void)hipStreamSynchronize(ctx->stream);
+        dev_free(ctx, ens->state, ens->state_bytes);
+        dev_free(ctx, ens->alive, static_cast<size_t>(ens->n));
+        dev_free(ctx, ens->deaths, static_cast<size_t>(ens->deaths_cap) * 8);
+        delete ens;
+    });
+}
+
+int pt_ens_advance(pt_ens *ens, const pt_field *field, const pt_advance_params *params,
+                   void *out_a, void *out_b, int out_dtype, int out_on_device,
+                   int64_t out_capacity_saves, int64_t *alive_counts, int64_t *n_saved) {
+    return guarded([&] {
+        PT_REQUIRE(ens && field && params, "pt_ens_advance: null argument");
+        PT_REQUIRE(field->g.n[0] == ens->field->g.n[0] && field->g.n[1] == ens->field->g.n[1] &&
+                       field->g.n[2] == ens->field->g.n[2] && field->g.L == ens->field->g.L,
+                   "pt_ens_advance: field geometry differs from the ensemble's");
+        if (ens->pusher == PT_PUSH_REF_F64)
+            PT_REQUIRE(field->storage == PT_STORE_F64_PLANAR, "PT_PUSH_REF_F64 needs a PT_STORE_F64_PLANAR field");
+        if (ens->pusher == PT_PUSH_PLAIN_F32)
+            PT_REQUIRE(field->storage == PT_STORE_F32_PLANAR, "PT_PUSH_PLAIN_F32 needs a PT_STORE_F32_PLANAR field");
+        if (ens->pusher == PT_PUSH_FAST)
+            PT_REQUIRE(field->storage != PT_STORE_F64_PLANAR, "PT_PUSH_FAST needs an FP32 or 16-bit field");
+        const pt_advance_params &P = *params;
+        PT_REQUIRE(std::isfinite(P.dt_s) && P.dt_s > 0, "pt_ens_advance: dt must be finite and positive");
+        PT_REQUIRE(std::isfinite(P.mass_kg) && P.mass_kg > 0, "pt_ens_advance: mass must be positive");
+        PT_REQUIRE(std::isfinite(P.charge_C) && P.charge_C != 0, "pt_ens_advance: charge must be non-zero");
+        PT_REQUIRE(P.first_step >= 1 && P.n_steps >= 0 && P.total_steps >= 1,
+                   "pt_ens_advance: step indices must be >= 1");
+        PT_REQUIRE(P.first_step + P.n_steps - 1 <= P.total_steps, "pt_ens_advance: steps past total_steps");
+        PT_REQUIRE(P.save_stride >= 1, "trajectory_time_stride must be >= 1");
+        PT_REQUIRE(P.out_kind == PT_OUT_NONE || P.out_kind == PT_OUT_PHASE || P.out_kind == PT_OUT_MU,
+                   "pt_ens_advance: unknown out_kind");
+        PT_REQUIRE(out_dtype == PT_F32 || out_dtype == PT_F64, "pt_ens_advance: out_dtype must be PT_F32 or PT_F64");
+        pt_ctx *ctx = ens->ctx;
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+
+        const int64_t nsv = saves_in_range(P.first_step, P.n_steps, P.save_stride);
+        const size_t N = static_cast<size_t>(ens->n);
+        const size_t esz = out_dtype == PT_F64 ? 8 : 4;
+        const size_t comps = P.out_kind == PT_OUT_PHASE ? 3 : 1;
+        void *da = nullptr, *db = nullptr;
+        char *staging = nullptr;
+        if (P.out_kind != PT_OUT_NONE && nsv > 0) {
+            PT_REQUIRE(out_a != nullptr && (P.out_kind != PT_OUT_PHASE || out_b != nullptr),
+                       "pt_ens_advance: output buffer is null");
+            PT_REQUIRE(out_capacity_saves >= nsv, "pt_ens_advance: output buffers hold " +
+                                                      std::to_string(out_capacity_saves) + " saves, call writes " +
+                                                      std::to_string(nsv));
+            if (out_on_device) {
+                da = out_a;
+                db = out_b;
+            } else {
+                size_t per = static_cast<size_t>(nsv) * comps * N * esz;
+                staging = static_cast<char *>(ctx_scratch(ctx, per * (P.out_kind == PT_OUT_PHASE ? 2 : 1)));
+                da = staging;
+                db = P.out_kind == PT_OUT_PHASE ? staging + per : nullptr;
+            }
+        }
+        if (ens->deaths_cap < std::max<int64_t>(P.n_steps, 1)) {
+            dev_free(ctx, ens->deaths, static_cast<size_t>(ens->deaths_cap) * 8);
+            ens->deaths = nullptr;
+            ens->deaths_cap = 0;
+            int64_t cap = std::max<int64_t>(P.n_steps, 1024);
+            ens->deaths = static_cast<int64_t *>(dev_alloc(ctx, static_cast<size_t>(cap) * 8));
+            ens->deaths_cap = cap;
+        }
+        PT_HIP_CHECK(hipMemsetAsync(ens->deaths, 0, static_cast<size_t>(std::max<int64_t>(P.n_steps, 1)) * 8, ctx->stream));
+
+        PushArgs a{};
+        a.first_step = P.first_step;
+        a.n_steps = P.n_steps;
+        a.total_steps = P.total_steps;
+        a.stride = P.save_stride;
+        a.save_base = (P.first_step - 1 + P.save_stride - 1) / P.save_stride;
+        a.out_kind = (nsv > 0) ? P.out_kind : PT_OUT_NONE;
+        a.out_f64 = out_dtype == PT_F64;
+        a.out_a = da;
+        a.out_b = db;
+        a.out_cap = nsv;
+        a.dt = P.dt_s;
+        a.q = P.charge_C;
+        a.m = P.mass_kg;
+        a.steps_per_launch = P.steps_per_launch;
+
+        PT_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+        int64_t launches = 0;
+        if (P.n_steps > 0) {
+            switch (ens->pusher) {
+                case PT_PUSH_REF_F64: launches = launch_push_ref_f64(ens, field, a); break;
+                case PT_PUSH_PLAIN_F32: launches = launch_push_plain_f32(ens, field, a); break;
+                case PT_PUSH_FAST: launches = launch_push_fast(ens, field, a); break;
+            }
+        }
+        PT_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+        PT_HIP_CHECK(hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        PT_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ens->kernel_ms = ms;
+        ens->launches = launches;
+
+        std::vector<int64_t> deaths(static_cast<size_t>(std::max<int64_t>(P.n_steps, 1)));
+        PT_HIP_CHECK(hipMemcpy(deaths.data(), ens->deaths, deaths.size() * 8, hipMemcpyDeviceToHost));
+        int64_t alive = ens->n_alive;
+        for (int64_t k = 0; k < P.n_steps; ++k) {
+            alive -= deaths[static_cast<size_t>(k)];
+            if (alive_counts) alive_counts[k] = alive;
+        }
+        ens->n_alive = alive;
+
+        if (staging) {
+            size_t per = static_cast<size_t>(nsv) * comps * N * esz;
+            PT_HIP_CHECK(hipMemcpy(out_a, da, per, hipMemcpyDeviceToHost));
+            if (P.out_kind == PT_OUT_PHASE) PT_HIP_CHECK(hipMemcpy(out_b, db, per, hipMemcpyDeviceToHost));
+        }
+        if (n_saved) *n_saved = nsv;
+    });
+}
+
+int pt_ens_get_state(pt_ens *ens, double *x[3], double *p[3], uint8_t *alive) {
+    return guarded([&] {
+        PT_REQUIRE(ens, "pt_ens_get_state: null ensemble");
+        pt_ctx *ctx = ens->ctx;
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        const size_t N = static_cast<size_t>(ens->n);
+        std::vector<double> h(6 * N);
+        if (ens->pusher == PT_PUSH_REF_F64) {
+            PT_HIP_CHECK(hipMemcpyAsync(h.data(), ens->state, 6 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+        } else {
+            double *st = static_cast<double *>(ctx_scratch(ctx, 6 * N * 8));
+            launch_split_to_si(ens, st, st + 3 * N);
+            PT_HIP_CHECK(hipMemcpyAsync(h.data(), st, 6 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+        }
+        if (alive) PT_HIP_CHECK(hipMemcpyAsync(alive, ens->alive, N, hipMemcpyDeviceToHost, ctx->stream));
+        PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (int c = 0; c < 3; ++c) {
+            if (x && x[c]) std::memcpy(x[c], h.data() + c * N, N * 8);
+            if (p && p[c]) std::memcpy(p[c], h.data() + (3 + c) * N, N * 8);
+        }
+    });
+}
+
+int pt_ens_last_timing(const pt_ens *ens, double *kernel_ms, int64_t *launches) {
+    return guarded([&] {
+        PT_REQUIRE(ens, "pt_ens_last_timing: null ensemble");
+        if (kernel_ms) *kernel_ms = ens->kernel_ms;
+        if (launches) *launches = ens->launches;
+    });
+}
+
+}  // extern "C"

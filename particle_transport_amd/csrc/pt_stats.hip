@@ -1,0 +1,757 @@
+// Lag statistics and mu reconstruction on gfx950.
+//
+//  D_mumu (sliding and injection starts) and D_pp replace the host-threaded
+//  accumulators of compute_delta_mu2_dmumu_full.jl:457-654 and
+//  compute_dpp_full.jl:247-288.  One lane walks one particle's time series;
+//  a workgroup covers 256 particles x G lags.  The mu-bin of the pair start
+//  changes rarely along a trajectory, so each lane keeps a run accumulator
+//  (count, sum dmu, sum dmu^2) per lag for its current bin in registers and
+//  flushes it into LDS-privatised bins (ds_add) only when the bin changes.
+//  Per-particle means are merged across particles with Chan's formula in a
+//  fixed tree (wavefront shuffles, then LDS), per-workgroup partials are
+//  merged in workgroup order, so the Welford statistics are deterministic.
+//  Integer counts are exact; sums are FP64.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pt_internal.hpp"
+
+namespace pt {
+namespace {
+
+constexpr int kTPB = 256;
+constexpr int kG = 8;          // lags per workgroup
+constexpr int kMaxBins = 256;  // LDS budget: kG * bins * 24 B
+
+struct Welford {
+    double n, mean, m2;
+};
+
+__device__ __forceinline__ Welford chan(Welford a, Welford b) {
+    if (b.n == 0.0) return a;
+    if (a.n == 0.0) return b;
+    const double n = a.n + b.n;
+    const double delta = b.mean - a.mean;
+    Welford r;
+    r.n = n;
+    r.mean = a.mean + delta * b.n / n;
+    r.m2 = a.m2 + b.m2 + delta * delta * a.n * b.n / n;
+    return r;
+}
+
+// Block-wide ordered reduction helpers (256 threads = 4 wavefronts of 64).
+__device__ __forceinline__ Welford block_welford(Welford w, Welford *sh) {
+    for (int o = 32; o > 0; o >>= 1) {
+        Welford other;
+        other.n = __shfl_down(w.n, o, 64);
+        other.mean = __shfl_down(w.mean, o, 64);
+        other.m2 = __shfl_down(w.m2, o, 64);
+        w = chan(w, other);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = w;
+    __syncthreads();
+    Welford r{0, 0, 0};
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kTPB / 64; ++i) r = chan(r, sh[i]);
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kTPB / 64; ++i) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ long long block_sum_i(long long v, long long *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    long long r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kTPB / 64; ++i) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------- D_mumu ---
+//
+// Bins: bin b (0-based) holds float v with lo[b] <= v < lo[b+1] (hi_last for
+// the last bin).  lo[b] is the smallest float >= edge b, and hi_last the
+// successor of the largest float <= the last edge, so for every float mu these
+// comparisons equal mu_bin_index_full's FP64 comparisons
+// (compute_delta_mu2_dmumu_full.jl:308-318), right-closed last bin included.
+__global__ void __launch_bounds__(kTPB)
+k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, int64_t sp,
+        const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags,
+        const float *__restrict__ lo_g, int nbins, float hi_last, int bin_abs, int injection,
+        unsigned long long *__restrict__ g_cnt, double *__restrict__ g_s1, double *__restrict__ g_s2,
+        double *__restrict__ part_d, long long *__restrict__ part_i) {
+    extern __shared__ unsigned char smem[];
+    unsigned long long *b_cnt = reinterpret_cast<unsigned long long *>(smem);
+    double *b_s1 = reinterpret_cast<double *>(b_cnt + kG * nbins);
+    double *b_s2 = b_s1 + kG * nbins;
+    float *lo = reinterpret_cast<float *>(b_s2 + kG * nbins);
+    __shared__ Welford sh_w[kTPB / 64];
+    __shared__ double sh_d[kTPB / 64];
+    __shared__ long long sh_i[kTPB / 64];
+
+    for (int i = threadIdx.x; i < kG * nbins; i += kTPB) {
+        b_cnt[i] = 0;
+        b_s1[i] = 0.0;
+        b_s2[i] = 0.0;
+    }
+    for (int i = threadIdx.x; i < nbins; i += kTPB) lo[i] = lo_g[i];
+    __syncthreads();
+
+    const int gy = blockIdx.y;
+    int64_t lag[kG];
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+        const int li = gy * kG + g;
+        lag[g] = li < nlags ? lags[li] : INT64_MAX / 4;
+    }
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * kTPB + threadIdx.x;
+    const bool active = j < n_sel;
+
+    unsigned rc[kG];
+    double r1[kG], r2[kG], ps[kG];
+    unsigned pc[kG];
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+        rc[g] = 0;
+        pc[g] = 0;
+        r1[g] = r2[g] = ps[g] = 0.0;
+    }
+    int cur_bin = -1;
+    float cur_lo = INFINITY, cur_hi = -INFINITY;
+
+    if (active) {
+        const int64_t col = pidx ? pidx[j] : j;
+        const float *m = mu + col * sp;
+        int64_t lag_min = lag[0];
+#pragma unroll
+        for (int g = 1; g < kG; ++g) lag_min = lag[g] < lag_min ? lag[g] : lag_min;
+        const int64_t s_end = injection ? (lag_min < nsave ? 1 : 0) : (nsave - lag_min);
+        for (int64_t s = 0; s < s_end; ++s) {
+            const float a = m[s * ss];
+            if (!isfinite(a)) continue;
+            const float v = bin_abs ? fabsf(a) : a;
+            if (!(v >= cur_lo && v < cur_hi)) {
+                if (cur_bin >= 0) {
+#pragma unroll
+                    for (int g = 0; g < kG; ++g)
+                        if (rc[g]) {
+                            atomicAdd(&b_cnt[g * nbins + cur_bin], static_cast<unsigned long long>(rc[g]));
+                            atomicAdd(&b_s1[g * nbins + cur_bin], r1[g]);
+                            atomicAdd(&b_s2[g * nbins + cur_bin], r2[g]);
+                            rc[g] = 0;
+                            r1[g] = r2[g] = 0.0;
+                        }
+                }
+                if (v < lo[0]) {
+                    cur_bin = -1;
+                    cur_lo = -INFINITY;
+                    cur_hi = lo[0];
+                } else if (v >= hi_last) {
+                    cur_bin = -1;
+                    cur_lo = hi_last;
+                    cur_hi = INFINITY;
+                } else {
+                    int b0 = 0, b1 = nbins - 1;  // largest b with lo[b] <= v
+                    while (b0 < b1) {
+                        const int mid = (b0 + b1 + 1) >> 1;
+                        if (lo[mid] <= v)
+                            b0 = mid;
+                        else
+                            b1 = mid - 1;
+                    }
+                    cur_bin = b0;
+                    cur_lo = lo[b0];
+                    cur_hi = b0 + 1 < nbins ? lo[b0 + 1] : hi_last;
+                }
+            }
+            const double ad = static_cast<double>(a);
+#pragma unroll
+            for (int g = 0; g < kG; ++g) {
+                if (s + lag[g] < nsave) {
+                    const float b = m[(s + lag[g]) * ss];
+                    if (isfinite(b)) {
+                        const double d = static_cast<double>(b) - ad;
+                        const double d2 = d * d;
+                        ps[g] += d2;
+                        pc[g] += 1;
+                        if (cur_bin >= 0) {
+                            rc[g] += 1;
+                            r1[g] += d;
+                            r2[g] += d2;
+                        }
+                    }
+                }
+            }
+        }
+        if (cur_bin >= 0) {
+#pragma unroll
+            for (int g = 0; g < kG; ++g)
+                if (rc[g]) {
+                    atomicAdd(&b_cnt[g * nbins + cur_bin], static_cast<unsigned long long>(rc[g]));
+                    atomicAdd(&b_s1[g * nbins + cur_bin], r1[g]);
+                    atomicAdd(&b_s2[g * nbins + cur_bin], r2[g]);
+                }
+        }
+    }
+
+    const int nbx = gridDim.x;
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+        const int li = gy * kG + g;
+        Welford w{0, 0, 0};
+        if (active && pc[g] > 0) {
+            w.n = 1.0;
+            w.mean = ps[g] / static_cast<double>(pc[g]);
+        }
+        const Welford bw = block_welford(w, sh_w);
+        const double pss = block_sum_d(pc[g] > 0 ? ps[g] : 0.0, sh_d);
+        const long long pcs = block_sum_i(static_cast<long long>(pc[g]), sh_i);
+        if (threadIdx.x == 0 && li < nlags) {
+            const int64_t o = static_cast<int64_t>(li) * nbx + blockIdx.x;
+            part_d[o * 3 + 0] = bw.mean;
+            part_d[o * 3 + 1] = bw.m2;
+            part_d[o * 3 + 2] = pss;
+            part_i[o * 2 + 0] = static_cast<long long>(bw.n);
+            part_i[o * 2 + 1] = pcs;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kG * nbins; i += kTPB) {
+        const int g = i / nbins, b = i % nbins;
+        const int li = gy * kG + g;
+        if (li < nlags && b_cnt[i]) {
+            atomicAdd(&g_cnt[static_cast<int64_t>(li) * nbins + b], b_cnt[i]);
+            atomicAdd(&g_s1[static_cast<int64_t>(li) * nbins + b], b_s1[i]);
+            atomicAdd(&g_s2[static_cast<int64_t>(li) * nbins + b], b_s2[i]);
+        }
+    }
+}
+
+// Merge per-workgroup partials of each lag in workgroup order.
+__global__ void k_merge_partials(const double *__restrict__ part_d, const long long *__restrict__ part_i,
+                                 int nlags, int nbx, double *__restrict__ out_d,
+                                 long long *__restrict__ out_i) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nlags) return;
+    Welford w{0, 0, 0};
+    double pss = 0.0;
+    long long pcs = 0;
+    for (int b = 0; b < nbx; ++b) {
+        const int64_t o = static_cast<int64_t>(li) * nbx + b;
+        Welford x{static_cast<double>(part_i[o * 2]), part_d[o * 3], part_d[o * 3 + 1]};
+        w = chan(w, x);
+        pss += part_d[o * 3 + 2];
+        pcs += part_i[o * 2 + 1];
+    }
+    out_d[li * 3 + 0] = w.mean;
+    out_d[li * 3 + 1] = w.m2;
+    out_d[li * 3 + 2] = pss;
+    out_i[li * 2 + 0] = static_cast<long long>(w.n);
+    out_i[li * 2 + 1] = pcs;
+}
+
+// ------------------------------------------------------------------ D_pp ---
+
+__device__ __forceinline__ double pmag(const float *__restrict__ mom, int64_t s, int64_t total,
+                                       int64_t col) {
+    const float *q = mom + s * 3 * total + col;
+    const double x = q[0], y = q[total], z = q[2 * total];
+    return sqrt(x * x + y * y + z * z);
+}
+
+__global__ void __launch_bounds__(kTPB)
+k_dpp(const float *__restrict__ mom, int64_t nsave, int64_t total, int64_t n_sel,
+      const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags, double p0,
+      double *__restrict__ part_d, long long *__restrict__ part_i) {
+    __shared__ double sh_d[kTPB / 64];
+    __shared__ long long sh_i[kTPB / 64];
+    const int gy = blockIdx.y;
+    int64_t lag[kG];
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+        const int li = gy * kG + g;
+        lag[g] = li < nlags ? lags[li] : INT64_MAX / 4;
+    }
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * kTPB + threadIdx.x;
+    unsigned cnt[kG];
+    double s1[kG], s2[kG], s3[kG], s4[kG];
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+        cnt[g] = 0;
+        s1[g] = s2[g] = s3[g] = s4[g] = 0.0;
+    }
+    if (j < n_sel) {
+        const int64_t col = pidx ? pidx[j] : j;
+        int64_t lag_min = lag[0];
+#pragma unroll
+        for (int g = 1; g < kG; ++g) lag_min = lag[g] < lag_min ? lag[g] : lag_min;
+        for (int64_t s = 0; s + lag_min < nsave; ++s) {
+            const double pst = pmag(mom, s, total, col);
+            if (!isfinite(pst)) continue;
+#pragma unroll
+            for (int g = 0; g < kG; ++g) {
+                if (s + lag[g] < nsave) {
+                    const double pe = pmag(mom, s + lag[g], total, col);
+                    if (isfinite(pe)) {
+                        const double d = pe - pst;
+                        const double dn = d / p0;
+                        cnt[g] += 1;
+                        s1[g] += d;
+                        s2[g] += d * d;
+                        s3[g] += dn;
+                        s4[g] += dn * dn;
+                    }
+                }
+            }
+        }
+    }
+    const int nbx = gridDim.x;
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+        const int li = gy * kG + g;
+        const long long c = block_sum_i(static_cast<long long>(cnt[g]), sh_i);
+        const double a = block_sum_d(s1[g], sh_d);
+        const double b = block_sum_d(s2[g], sh_d);
+        const double cc = block_sum_d(s3[g], sh_d);
+        const double dd = block_sum_d(s4[g], sh_d);
+        if (threadIdx.x == 0 && li < nlags) {
+            const int64_t o = static_cast<int64_t>(li) * nbx + blockIdx.x;
+            part_i[o] = c;
+            part_d[o * 4 + 0] = a;
+            part_d[o * 4 + 1] = b;
+            part_d[o * 4 + 2] = cc;
+            part_d[o * 4 + 3] = dd;
+        }
+    }
+}
+
+__global__ void k_merge_dpp(const double *__restrict__ part_d, const long long *__restrict__ part_i,
+                            int nlags, int nbx, double *__restrict__ out_d, long long *__restrict__ out_i) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nlags) return;
+    double a = 0, b = 0, c = 0, d = 0;
+    long long n = 0;
+    for (int k = 0; k < nbx; ++k) {
+        const int64_t o = static_cast<int64_t>(li) * nbx + k;
+        n += part_i[o];
+        a += part_d[o * 4 + 0];
+        b += part_d[o * 4 + 1];
+        c += part_d[o * 4 + 2];
+        d += part_d[o * 4 + 3];
+    }
+    out_i[li] = n;
+    out_d[li * 4 + 0] = a;
+    out_d[li * 4 + 1] = b;
+    out_d[li * 4 + 2] = c;
+    out_d[li * 4 + 3] = d;
+}
+
+// kinetic_energy_gev_from_pmag (compute_dpp_full.jl:236-240) at snapshot saves.
+__global__ void k_energy(const float *__restrict__ mom, int64_t total, int64_t n_sel,
+                         const int64_t *__restrict__ pidx, const int64_t *__restrict__ snap, int64_t k,
+                         double *__restrict__ out) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n_sel * k) return;
+    const int64_t jj = t / k, q = t % k;
+    const int64_t col = pidx ? pidx[jj] : jj;
+    const double pm = pmag(mom, snap[q], total, col);
+    double e = NAN;
+    if (isfinite(pm)) {
+        const double r = pm / (kMp * kC);
+        const double gamma = sqrt(1.0 + r * r);
+        e = (gamma - 1.0) * kMp * kC * kC / kGeVtoJ;
+    }
+    out[q + k * jj] = e;
+}
+
+// ----------------------------------------------------- mu from phase space -
+
+template <class T>
+struct PlanarGeom {
+    T lo[3], hi[3], d[3];
+    int64_t n[3];
+};
+
+template <class T>
+__device__ T trilinear_t(const T *__restrict__ arr, T x, T y, T z, const PlanarGeom<T> &g) {
+    const T fx = (x - g.lo[0]) / g.d[0] + T(1);
+    const T fy = (y - g.lo[1]) / g.d[1] + T(1);
+    const T fz = (z - g.lo[2]) / g.d[2] + T(1);
+    const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    auto cf = [](T f, int64_t hi) -> int64_t {
+        T fl = floor(f);
+        if (fl < T(1)) return 1;
+        if (fl > static_cast<T>(hi)) return hi;
+        return static_cast<int64_t>(fl);
+    };
+    const int64_t i0 = cf(fx, nx - 1), j0 = cf(fy, ny - 1), k0 = cf(fz, nz - 1);
+    const T tx = fx - static_cast<T>(i0), ty = fy - static_cast<T>(j0), tz = fz - static_cast<T>(k0);
+    const int64_t base = (i0 - 1) + nx * ((j0 - 1) + ny * (k0 - 1));
+    const int64_t sy = nx, sz = nx * ny;
+    const T c00 = arr[base] * (1 - tx) + arr[base + 1] * tx;
+    const T c10 = arr[base + sy] * (1 - tx) + arr[base + 1 + sy] * tx;
+    const T c01 = arr[base + sz] * (1 - tx) + arr[base + 1 + sz] * tx;
+    const T c11 = arr[base + sy + sz] * (1 - tx) + arr[base + 1 + sy + sz] * tx;
+    const T c0 = c00 * (1 - ty) + c10 * ty;
+    const T c1 = c01 * (1 - ty) + c11 * ty;
+    return c0 * (1 - tz) + c1 * tz;
+}
+
+template <class T, class S, class O>
+__global__ void k_mu_from_phase(const S *__restrict__ pos, const S *__restrict__ mom, int64_t chunk,
+                                int64_t nsave, const T *__restrict__ F, PlanarGeom<T> g, O *__restrict__ mu) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= chunk * nsave) return;
+    const int64_t s = t / chunk, p = t % chunk;
+    const int64_t nn = g.n[0] * g.n[1] * g.n[2];
+    const T x = static_cast<T>(pos[(s * 3 + 0) * chunk + p]);
+    const T y = static_cast<T>(pos[(s * 3 + 1) * chunk + p]);
+    const T z = static_cast<T>(pos[(s * 3 + 2) * chunk + p]);
+    const T px = static_cast<T>(mom[(s * 3 + 0) * chunk + p]);
+    const T py = static_cast<T>(mom[(s * 3 + 1) * chunk + p]);
+    const T pz = static_cast<T>(mom[(s * 3 + 2) * chunk + p]);
+    T value = T(NAN);
+    if (!(isnan(x) || isnan(y) || isnan(z) || isnan(px) || isnan(py) || isnan(pz))) {
+        if (g.lo[0] <= x && x <= g.hi[0] && g.lo[1] <= y && y <= g.hi[1] && g.lo[2] <= z && z <= g.hi[2]) {
+            const T bx = trilinear_t(F, x, y, z, g);
+            const T by = trilinear_t(F + nn, x, y, z, g);
+            const T bz = trilinear_t(F + 2 * nn, x, y, z, g);
+            // pitch_mu_from_pB, compute_delta_mu2_curve.jl:101-110
+            const T pn = sqrt(px * px + py * py + pz * pz);
+            const T bn = sqrt(bx * bx + by * by + bz * bz);
+            if (isfinite(pn) && isfinite(bn) && pn != T(0) && bn != T(0)) {
+                T m = (px * bx + py * by + pz * bz) / (pn * bn);
+                value = m < T(-1) ? T(-1) : (m > T(1) ? T(1) : m);
+            }
+        }
+    }
+    mu[s + nsave * p] = static_cast<O>(value);
+}
+
+template <class F>
+int guarded(F &&fn) {
+    try {
+        fn();
+        return PT_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (...) {
+        set_error("unknown C++ exception");
+        return PT_ERR_STATE;
+    }
+}
+
+// Bring a host or device array into device memory (owned copy in `hold`).
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void *alloc(size_t bytes) {
+        PT_HIP_CHECK(hipMalloc(&p, bytes ? bytes : 1));
+        return p;
+    }
+};
+
+template <class T>
+const T *to_device(const T *src, int on_device, size_t count, DevBuf &hold, hipStream_t s) {
+    if (on_device || count == 0) return src;
+    void *d = hold.alloc(count * sizeof(T));
+    PT_HIP_CHECK(hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return static_cast<const T *>(d);
+}
+
+float ceil_to_float(double e) {
+    float f = static_cast<float>(e);
+    if (static_cast<double>(f) < e) f = std::nextafter(f, INFINITY);
+    return f;
+}
+float floor_to_float(double e) {
+    float f = static_cast<float>(e);
+    if (static_cast<double>(f) > e) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+
+void combine_welford_host(int64_t *counts, double *means, double *m2s, int64_t li, int64_t lc,
+                          double lmean, double lm2) {
+    // combine_welford_stats!, compute_delta_mu2_dmumu_full.jl:439-455
+    if (lc <= 0) return;
+    const int64_t gc = counts[li];
+    if (gc == 0) {
+        counts[li] = lc;
+        means[li] = lmean;
+        m2s[li] = lm2;
+        return;
+    }
+    const int64_t cc = gc + lc;
+    const double delta = lmean - means[li];
+    means[li] += delta * static_cast<double>(lc) / static_cast<double>(cc);
+    m2s[li] += lm2 + delta * delta * static_cast<double>(gc) * static_cast<double>(lc) / static_cast<double>(cc);
+    counts[li] = cc;
+}
+
+}  // namespace
+}  // namespace pt
+
+using namespace pt;
+
+extern "C" {
+
+int pt_mu_from_phase(pt_ctx *ctx, const pt_field *field, int64_t chunk, int64_t nsave,
+                     int src_dtype, int src_on_device, const void *positions,
+                     const void *momenta, int out_dtype, int out_on_device, void *mu_out) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && field && positions && momenta && mu_out, "pt_mu_from_phase: null argument");
+        PT_REQUIRE(chunk >= 1 && nsave >= 1, "pt_mu_from_phase: empty chunk");
+        PT_REQUIRE(field->storage == PT_STORE_F32_PLANAR || field->storage == PT_STORE_F64_PLANAR,
+                   "pt_mu_from_phase: needs a planar field (compute_precision Float32 or Float64)");
+        PT_REQUIRE(src_dtype == PT_F32 || src_dtype == PT_F64, "pt_mu_from_phase: bad src dtype");
+        PT_REQUIRE(out_dtype == PT_F32 || out_dtype == PT_F64, "pt_mu_from_phase: bad out dtype");
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const size_t cnt = static_cast<size_t>(chunk * 3 * nsave);
+        DevBuf hp, hm, ho;
+        const size_t esz = src_dtype == PT_F64 ? 8 : 4;
+        const void *dp = positions, *dm = momenta;
+        if (!src_on_device) {
+            dp = hp.alloc(cnt * esz);
+            dm = hm.alloc(cnt * esz);
+            PT_HIP_CHECK(hipMemcpyAsync(const_cast<void *>(dp), positions, cnt * esz, hipMemcpyHostToDevice, st));
+            PT_HIP_CHECK(hipMemcpyAsync(const_cast<void *>(dm), momenta, cnt * esz, hipMemcpyHostToDevice, st));
+        }
+        const size_t osz = out_dtype == PT_F64 ? 8 : 4;
+        void *dout = out_on_device ? mu_out : ho.alloc(static_cast<size_t>(chunk * nsave) * osz);
+        const unsigned blocks = static_cast<unsigned>((chunk * nsave + 255) / 256);
+        auto run = [&](auto tag_t, auto tag_s, auto tag_o) {
+            using T = decltype(tag_t);
+            using S = decltype(tag_s);
+            using O = decltype(tag_o);
+            PlanarGeom<T> g;
+            // The reference builds the grid in compute precision T
+            // (compute_delta_mu2_curve.jl:55-60, :193-198).
+            const T L = static_cast<T>(field->g.L);
+            for (int c = 0; c < 3; ++c) {
+                g.lo[c] = T(0);
+                g.hi[c] = L;
+                g.d[c] = static_cast<T>(static_cast<double>(L) / static_cast<double>(field->g.n[c] - 1));
+                g.n[c] = field->g.n[c];
+            }
+            k_mu_from_phase<T, S, O><<<blocks, 256, 0, st>>>(static_cast<const S *>(dp), static_cast<const S *>(dm),
+                                                          chunk, nsave, static_cast<const T *>(field->data), g,
+                                                          static_cast<O *>(dout));
+        };
+        const bool f64 = field->storage == PT_STORE_F64_PLANAR;
+        if (f64) {
+            if (src_dtype == PT_F64) {
+                if (out_dtype == PT_F64) run(double(), double(), double()); else run(double(), double(), float());
+            } else {
+                if (out_dtype == PT_F64) run(double(), float(), double()); else run(double(), float(), float());
+            }
+        } else {
+            if (src_dtype == PT_F64) {
+                if (out_dtype == PT_F64) run(float(), double(), double()); else run(float(), double(), float());
+            } else {
+                if (out_dtype == PT_F64) run(float(), float(), double()); else run(float(), float(), float());
+            }
+        }
+        PT_HIP_CHECK(hipGetLastError());
+        if (!out_on_device)
+            PT_HIP_CHECK(hipMemcpyAsync(mu_out, dout, static_cast<size_t>(chunk * nsave) * osz, hipMemcpyDeviceToHost, st));
+        PT_HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t nsave,
+                        int64_t n_sel, int64_t stride_s, int64_t stride_p,
+                        const int64_t *particle_index, const int64_t *lag_steps, int64_t nlags,
+                        const double *mu_edges, int64_t n_bins, int start_mode, int bin_abs,
+                        int64_t *particle_counts, double *particle_means, double *particle_m2s,
+                        double *pair_sum_squares, int64_t *pair_counts, int64_t *dmumu_counts,
+                        double *dmumu_sum_delta, double *dmumu_sum_delta2) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && mu && lag_steps && mu_edges, "pt_dmumu_accumulate: null argument");
+        PT_REQUIRE(particle_counts && particle_means && particle_m2s && pair_sum_squares && pair_counts &&
+                       dmumu_counts && dmumu_sum_delta && dmumu_sum_delta2,
+                   "pt_dmumu_accumulate: null accumulator");
+        PT_REQUIRE(nsave >= 1 && n_sel >= 0 && nlags >= 1, "pt_dmumu_accumulate: empty input");
+        PT_REQUIRE(n_bins >= 1 && n_bins <= kMaxBins, "pt_dmumu_accumulate: n_bins must be in [1, 256]");
+        PT_REQUIRE(start_mode == 0 || start_mode == 1, "Unknown dmumu_start_mode");
+        PT_REQUIRE(nlags <= (1 << 20), "pt_dmumu_accumulate: too many lags");
+        std::vector<int64_t> lags(lag_steps, lag_steps + nlags);
+        for (int64_t l : lags) PT_REQUIRE(l >= 1, "pt_dmumu_accumulate: lag steps must be >= 1");
+        for (int64_t b = 0; b < n_bins; ++b)
+            PT_REQUIRE(mu_edges[b] < mu_edges[b + 1], "pt_dmumu_accumulate: edges must increase");
+        if (n_sel == 0) return;
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+
+        // float thresholds equivalent to the FP64 edge comparisons
+        std::vector<float> lo(static_cast<size_t>(n_bins));
+        for (int64_t b = 0; b < n_bins; ++b) lo[b] = ceil_to_float(mu_edges[b]);
+        const float hi_last = std::nextafter(floor_to_float(mu_edges[n_bins]), INFINITY);
+
+        // Extent of the mu array the kernel may touch, for host staging.
+        int64_t max_col = n_sel - 1;
+        if (particle_index) {
+            max_col = 0;
+            for (int64_t j = 0; j < n_sel; ++j) {
+                PT_REQUIRE(particle_index[j] >= 0, "pt_dmumu_accumulate: negative particle index");
+                max_col = std::max(max_col, particle_index[j]);
+            }
+        }
+        const size_t mu_count = static_cast<size_t>((nsave - 1) * stride_s + max_col * stride_p + 1);
+        DevBuf hmu, hidx, hlag, hlo, hout;
+        const float *dmu = to_device(mu, mu_on_device, mu_count, hmu, st);
+        const int64_t *didx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
+        const int64_t *dlag = to_device(lags.data(), 0, lags.size(), hlag, st);
+        const float *dlo = to_device(lo.data(), 0, lo.size(), hlo, st);
+
+        const int nbx = static_cast<int>((n_sel + kTPB - 1) / kTPB);
+        const int ngy = static_cast<int>((nlags + kG - 1) / kG);
+        const size_t nb = static_cast<size_t>(n_bins), nl = static_cast<size_t>(nlags);
+        const size_t parts = nl * static_cast<size_t>(nbx);
+        // device layout: counts u64 [nlags][nbins], s1, s2, part_d, part_i, out_d, out_i
+        const size_t bytes = nl * nb * 24 + parts * (24 + 16) + nl * (24 + 16);
+        char *base = static_cast<char *>(hout.alloc(bytes));
+        auto *g_cnt = reinterpret_cast<unsigned long long *>(base);
+        auto *g_s1 = reinterpret_cast<double *>(g_cnt + nl * nb);
+        auto *g_s2 = g_s1 + nl * nb;
+        auto *part_d = g_s2 + nl * nb;
+        auto *part_i = reinterpret_cast<long long *>(part_d + parts * 3);
+        auto *out_d = reinterpret_cast<double *>(part_i + parts * 2);
+        auto *out_i = reinterpret_cast<long long *>(out_d + nl * 3);
+        PT_HIP_CHECK(hipMemsetAsync(base, 0, nl * nb * 24, st));
+
+        const size_t shm = static_cast<size_t>(kG) * nb * 24 + nb * 4;
+        dim3 grid(static_cast<unsigned>(nbx), static_cast<unsigned>(ngy));
+        k_dmumu<<<grid, kTPB, shm, st>>>(dmu, nsave, n_sel, stride_s, stride_p, didx, dlag, static_cast<int>(nlags),
+                                         dlo, static_cast<int>(n_bins), hi_last, bin_abs ? 1 : 0, start_mode,
+                                         g_cnt, g_s1, g_s2, part_d, part_i);
+        PT_HIP_CHECK(hipGetLastError());
+        k_merge_partials<<<static_cast<unsigned>((nlags + 63) / 64), 64, 0, st>>>(part_d, part_i, static_cast<int>(nlags),
+                                                                                  nbx, out_d, out_i);
+        PT_HIP_CHECK(hipGetLastError());
+        std::vector<unsigned long long> hc(nl * nb);
+        std::vector<double> h1(nl * nb), h2(nl * nb), hd(nl * 3);
+        std::vector<long long> hi(nl * 2);
+        PT_HIP_CHECK(hipMemcpyAsync(hc.data(), g_cnt, nl * nb * 8, hipMemcpyDeviceToHost, st));
+        PT_HIP_CHECK(hipMemcpyAsync(h1.data(), g_s1, nl * nb * 8, hipMemcpyDeviceToHost, st));
+        PT_HIP_CHECK(hipMemcpyAsync(h2.data(), g_s2, nl * nb * 8, hipMemcpyDeviceToHost, st));
+        PT_HIP_CHECK(hipMemcpyAsync(hd.data(), out_d, nl * 3 * 8, hipMemcpyDeviceToHost, st));
+        PT_HIP_CHECK(hipMemcpyAsync(hi.data(), out_i, nl * 2 * 8, hipMemcpyDeviceToHost, st));
+        PT_HIP_CHECK(hipStreamSynchronize(st));
+        for (size_t li = 0; li < nl; ++li) {
+            if (lags[li] >= nsave) continue;  // the reference skips lags beyond the series
+            for (size_t b = 0; b < nb; ++b) {
+                dmumu_counts[b + nb * li] += static_cast<int64_t>(hc[li * nb + b]);
+                dmumu_sum_delta[b + nb * li] += h1[li * nb + b];
+                dmumu_sum_delta2[b + nb * li] += h2[li * nb + b];
+            }
+            combine_welford_host(particle_counts, particle_means, particle_m2s, static_cast<int64_t>(li), hi[li * 2],
+                                 hd[li * 3], hd[li * 3 + 1]);
+            pair_sum_squares[li] += hd[li * 3 + 2];
+            pair_counts[li] += hi[li * 2 + 1];
+        }
+    });
+}
+
+int pt_dpp_accumulate(pt_ctx *ctx, const float *momenta, int on_device, int64_t nsave,
+                      int64_t total, int64_t n_sel, const int64_t *particle_index,
+                      const int64_t *lag_steps, int64_t nlags, double p0, int64_t *counts,
+                      double *sum_delta_p, double *sum_delta_p2, double *sum_delta_p_norm,
+                      double *sum_delta_p_norm2) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && momenta && lag_steps && counts && sum_delta_p && sum_delta_p2 && sum_delta_p_norm &&
+                       sum_delta_p_norm2,
+                   "pt_dpp_accumulate: null argument");
+        PT_REQUIRE(nsave >= 1 && total >= 1 && n_sel >= 0 && nlags >= 1, "pt_dpp_accumulate: empty input");
+        PT_REQUIRE(particle_index || n_sel <= total, "pt_dpp_accumulate: n_sel exceeds total");
+        std::vector<int64_t> lags(lag_steps, lag_steps + nlags);
+        for (int64_t l : lags) PT_REQUIRE(l >= 1, "pt_dpp_accumulate: lag steps must be >= 1");
+        if (particle_index)
+            for (int64_t j = 0; j < n_sel; ++j)
+                PT_REQUIRE(particle_index[j] >= 0 && particle_index[j] < total, "pt_dpp_accumulate: particle index out of range");
+        if (n_sel == 0) return;
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        DevBuf hm, hidx, hlag, hout;
+        const float *dm = to_device(momenta, on_device, static_cast<size_t>(nsave * 3 * total), hm, st);
+        const int64_t *didx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
+        const int64_t *dlag = to_device(lags.data(), 0, lags.size(), hlag, st);
+        const int nbx = static_cast<int>((n_sel + kTPB - 1) / kTPB);
+        const int ngy = static_cast<int>((nlags + kG - 1) / kG);
+        const size_t nl = static_cast<size_t>(nlags), parts = nl * static_cast<size_t>(nbx);
+        char *base = static_cast<char *>(hout.alloc(parts * (32 + 8) + nl * (32 + 8)));
+        auto *part_d = reinterpret_cast<double *>(base);
+        auto *part_i = reinterpret_cast<long long *>(part_d + parts * 4);
+        auto *out_d = reinterpret_cast<double *>(part_i + parts);
+        auto *out_i = reinterpret_cast<long long *>(out_d + nl * 4);
+        dim3 grid(static_cast<unsigned>(nbx), static_cast<unsigned>(ngy));
+        k_dpp<<<grid, kTPB, 0, st>>>(dm, nsave, total, n_sel, didx, dlag, static_cast<int>(nlags), p0, part_d, part_i);
+        PT_HIP_CHECK(hipGetLastError());
+        k_merge_dpp<<<static_cast<unsigned>((nlags + 63) / 64), 64, 0, st>>>(part_d, part_i, static_cast<int>(nlags), nbx,
+                                                                             out_d, out_i);
+        PT_HIP_CHECK(hipGetLastError());
+        std::vector<double> hd(nl * 4);
+        std::vector<long long> hi(nl);
+        PT_HIP_CHECK(hipMemcpyAsync(hd.data(), out_d, nl * 32, hipMemcpyDeviceToHost, st));
+        PT_HIP_CHECK(hipMemcpyAsync(hi.data(), out_i, nl * 8, hipMemcpyDeviceToHost, st));
+        PT_HIP_CHECK(hipStreamSynchronize(st));
+        for (size_t li = 0; li < nl; ++li) {
+            if (lags[li] >= nsave) continue;
+            counts[li] += hi[li];
+            sum_delta_p[li] += hd[li * 4 + 0];
+            sum_delta_p2[li] += hd[li * 4 + 1];
+            sum_delta_p_norm[li] += hd[li * 4 + 2];
+            sum_delta_p_norm2[li] += hd[li * 4 + 3];
+        }
+    });
+}
+
+int pt_energy_snapshots(pt_ctx *ctx, const float *momenta, int on_device, int64_t nsave,
+                        int64_t total, int64_t n_sel, const int64_t *particle_index,
+                        const int64_t *snapshot_idx, int64_t k, double *out, int64_t col0) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && momenta && snapshot_idx && out, "pt_energy_snapshots: null argument");
+        PT_REQUIRE(k >= 1 && n_sel >= 0 && col0 >= 0, "pt_energy_snapshots: bad sizes");
+        for (int64_t q = 0; q < k; ++q)
+            PT_REQUIRE(snapshot_idx[q] >= 0 && snapshot_idx[q] < nsave, "pt_energy_snapshots: snapshot index out of range");
+        if (particle_index)
+            for (int64_t j = 0; j < n_sel; ++j)
+                PT_REQUIRE(particle_index[j] >= 0 && particle_index[j] < total, "pt_energy_snapshots: particle index out of range");
+        else
+            PT_REQUIRE(n_sel <= total, "pt_energy_snapshots: n_sel exceeds total");
+        if (n_sel == 0) return;
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        DevBuf hm, hidx, hs, ho;
+        const float *dm = to_device(momenta, on_device, static_cast<size_t>(nsave * 3 * total), hm, st);
+        const int64_t *didx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
+        const int64_t *dsn = to_device(snapshot_idx, 0, static_cast<size_t>(k), hs, st);
+        double *dout = static_cast<double *>(ho.alloc(static_cast<size_t>(n_sel * k) * 8));
+        const unsigned blocks = static_cast<unsigned>((n_sel * k + 255) / 256);
+        k_energy<<<blocks, 256, 0, st>>>(dm, total, n_sel, didx, dsn, k, dout);
+        PT_HIP_CHECK(hipGetLastError());
+        PT_HIP_CHECK(hipMemcpyAsync(out + k * col0, dout, static_cast<size_t>(n_sel * k) * 8, hipMemcpyDeviceToHost, st));
+        PT_HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+}  // extern "C"

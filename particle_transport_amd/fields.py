@@ -1,0 +1,225 @@
+"""Field inputs: the reference's loader and grid, and a synthetic turbulence generator.
+
+``load_static_fields`` / ``build_uniform_coords`` / ``estimate_min_dx`` restate
+phase_space_gpu_runner.jl:95-134.  The reference reads HDF5 datasets named by
+``B_paths``/``v_paths``; h5py is not part of this build, so the loader reads
+the same dataset names from a ``.npz`` archive (or takes arrays directly).
+
+``generate_turbulence`` writes the synthetic inputs the benchmark configs
+describe (the reference ships no generator): B = B0 z + dB and u = du, each
+fluctuation an isotropic Gaussian random field whose shell spectrum is
+|dB(k)|^2 ~ k^(-5/3) (3-D power ~ k^(-11/3)) for integer wavenumbers
+|k| L / 2pi in [kmin, kmax], made divergence-free by dB(k) <- dB - k (k.dB)/k^2
+and rescaled to the requested vector rms.  Fields are generated periodic on
+(n-1)^3 points and the first plane is repeated as the last node plane, so the
+node grid range(0, L, length=n) is seamless across the periodic wrap.
+Values are in the reference's input units: B in microgauss, u in km/s.
+
+``mode_decompose=True`` also returns Alfven/fast/slow parts.  Per k-mode the
+solenoidal fluctuation is split on the orthonormal pair e_A = k x z / |k x z|
+and e_P = k x e_A (the k-z plane): Alfven = (f.e_A) e_A; the e_P part is
+shared as slow = cos^2(theta) (f.e_P) e_P and fast = sin^2(theta) (f.e_P) e_P,
+theta the angle between k and z.  Each part is divergence-free and the parts
+sum to the total fluctuation; each mode field is B0 z + its own part.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+UGAUSS_TO_T = 1e-10
+KMPS_TO_MPS = 1e3
+PC_TO_M = 3.085677581e16
+
+B_PATHS = ("i_mag_field", "j_mag_field", "k_mag_field")
+V_PATHS = ("i_velocity", "j_velocity", "k_velocity")
+
+
+def box_length_m(box_length_pc: float) -> float:
+    """L = box_length_pc * PC_TO_M in Float64 (phase_space_gpu_runner.jl:126)."""
+    return float(box_length_pc) * PC_TO_M
+
+
+def build_uniform_coords(box_length_pc: float, nx: int, ny: int, nz: int):
+    """Grid extents of collect(range(0, L, length=n)) (phase_space_gpu_runner.jl:125-130):
+    xmin = 0, xmax = L, and x[2]-x[1] = L/(n-1) rounded once."""
+    L = box_length_m(box_length_pc)
+    return [dict(min=0.0, max=L, d=L / (n - 1), n=n) for n in (nx, ny, nz)]
+
+
+def estimate_min_dx(axes) -> float:
+    """phase_space_gpu_runner.jl:132-134 (uniform grid: the node spacing)."""
+    return min(a["d"] for a in axes)
+
+
+def load_static_fields(cfg, dtype=np.float64, arrays=None):
+    """phase_space_gpu_runner.jl:95-123.  Returns (Bx, By, Bz, vx, vy, vz) as
+    Fortran-ordered (nx,ny,nz) arrays in SI units (tesla, m/s).
+
+    ``arrays`` may hold the six raw datasets (microgauss, velocity units);
+    otherwise ``cfg["file"]`` names a .npz archive with the B_paths/v_paths
+    keys, each stored as a C-order (nz,ny,nx) array (the HDF5 on-disk order of
+    a Julia (nx,ny,nz) dataset)."""
+    subset = cfg.get("field_subset")
+    bp = tuple(cfg.get("B_paths", B_PATHS))
+    vp = tuple(cfg.get("v_paths", V_PATHS))
+    if arrays is None:
+        with np.load(cfg["file"], allow_pickle=False) as z:
+            raw = [np.asarray(z[k]).transpose(2, 1, 0) for k in (*bp, *vp)]
+    else:
+        raw = list(arrays)
+    out = []
+    for i, a in enumerate(raw):
+        a = np.asarray(a)
+        if subset:
+            nx, ny, nz = subset
+            a = a[:nx, :ny, :nz]
+        a = np.asfortranarray(a, dtype=dtype).copy(order="F")
+        scale = UGAUSS_TO_T if i < 3 else float(cfg.get("velocity_unit_in_m_per_s", KMPS_TO_MPS))
+        a *= dtype(scale)
+        out.append(a)
+    return tuple(out)
+
+
+def _wavenumbers(M: int, xp, device=None):
+    """Integer wavenumber grids for an rfftn over a C-order (M,M,M) array
+    indexed [kz, ky, kx] (x fastest)."""
+    if xp is np:
+        k = np.fft.fftfreq(M, d=1.0 / M)
+        kr = np.fft.rfftfreq(M, d=1.0 / M)
+        KZ, KY, KX = np.meshgrid(k, k, kr, indexing="ij")
+        return KX, KY, KZ
+    import torch
+    k = torch.fft.fftfreq(M, d=1.0 / M, device=device, dtype=torch.float64)
+    kr = torch.fft.rfftfreq(M, d=1.0 / M, device=device, dtype=torch.float64)
+    KZ, KY, KX = torch.meshgrid(k, k, kr, indexing="ij")
+    return KX, KY, KZ
+
+
+def _solenoidal_spectrum(M, kmin, kmax, rng, xp, device):
+    """Three rfft components of a divergence-free Gaussian field with 3-D power
+    ~ k^(-11/3) on the shell band [kmin, kmax]."""
+    if xp is np:
+        noise = rng.standard_normal((3, M, M, M))
+        F = np.fft.rfftn(noise, axes=(1, 2, 3))
+    else:
+        import torch
+        noise = torch.randn((3, M, M, M), generator=rng, device=device, dtype=torch.float32)
+        F = torch.fft.rfftn(noise.double() if device == "cpu" else noise, dim=(1, 2, 3))
+        del noise
+    KX, KY, KZ = _wavenumbers(M, xp, device)
+    k2 = KX * KX + KY * KY + KZ * KZ
+    kmag = xp.sqrt(k2)
+    band = (kmag >= kmin) & (kmag <= kmax)
+    safe = xp.where(k2 > 0, k2, xp.ones_like(k2))
+    amp = xp.where(band, safe ** (-11.0 / 12.0), xp.zeros_like(k2))
+    F = F * amp
+    kdotF = (KX * F[0] + KY * F[1] + KZ * F[2]) / safe
+    F = xp.stack([F[0] - KX * kdotF, F[1] - KY * kdotF, F[2] - KZ * kdotF])
+    return F, (KX, KY, KZ, k2)
+
+
+def _mode_split(F, K, xp):
+    KX, KY, KZ, k2 = K
+    kperp = xp.sqrt(KX * KX + KY * KY)
+    kmag = xp.sqrt(k2)
+    one = xp.ones_like(kperp)
+    kp_safe = xp.where(kperp > 0, kperp, one)
+    km_safe = xp.where(kmag > 0, kmag, one)
+    # e_A = k x z / |k x z| = (ky, -kx, 0)/kperp ; k parallel to z -> e_A = x
+    eAx = xp.where(kperp > 0, KY / kp_safe, one)
+    eAy = xp.where(kperp > 0, -KX / kp_safe, xp.zeros_like(kperp))
+    eAz = xp.zeros_like(kperp)
+    khx, khy, khz = KX / km_safe, KY / km_safe, KZ / km_safe
+    # e_P = khat x e_A
+    ePx = khy * eAz - khz * eAy
+    ePy = khz * eAx - khx * eAz
+    ePz = khx * eAy - khy * eAx
+    a = F[0] * eAx + F[1] * eAy + F[2] * eAz
+    p = F[0] * ePx + F[1] * ePy + F[2] * ePz
+    cos2 = xp.where(kmag > 0, (KZ / km_safe) ** 2, one)
+    alf = xp.stack([a * eAx, a * eAy, a * eAz])
+    slow = xp.stack([cos2 * p * ePx, cos2 * p * ePy, cos2 * p * ePz])
+    fast = xp.stack([(1 - cos2) * p * ePx, (1 - cos2) * p * ePy, (1 - cos2) * p * ePz])
+    return {"alfven": alf, "fast": fast, "slow": slow}
+
+
+def _to_real(F, M, xp):
+    if xp is np:
+        return np.fft.irfftn(F, s=(M, M, M), axes=(1, 2, 3))
+    import torch
+    return torch.fft.irfftn(F, s=(M, M, M), dim=(1, 2, 3))
+
+
+def _with_seam(r, xp):
+    """(3,M,M,M) periodic -> (3,M+1,M+1,M+1) node grid with the first plane repeated."""
+    if xp is np:
+        return np.pad(r, ((0, 0), (0, 1), (0, 1), (0, 1)), mode="wrap")
+    import torch
+    r = torch.cat([r, r[:, :1]], dim=1)
+    r = torch.cat([r, r[:, :, :1]], dim=2)
+    return torch.cat([r, r[:, :, :, :1]], dim=3)
+
+
+def generate_turbulence(n_nodes: int, *, B0_uG: float = 5.0, dB_over_B0: float = 0.5, u_rms_kms: float = 20.0,
+                        kmin: float = 1.0, kmax: float = 16.0, seed: int = 1234, device=None,
+                        dtype="float64", mode_decompose: bool = False):
+    """Synthetic B (microgauss) and u (km/s) on an n_nodes^3 node grid.
+
+    Returns a dict with key "total" (and "alfven"/"fast"/"slow" when
+    mode_decompose) mapping to a 6-tuple (Bx, By, Bz, ux, uy, uz), each shaped
+    (nx, ny, nz) with x fastest in memory (the Julia layout).  device=None
+    uses numpy; a torch device string ("cuda", "cpu") uses torch.fft there.
+    """
+    M = int(n_nodes) - 1
+    if M < 2:
+        raise ValueError("need at least 3 nodes per axis")
+    if device is None:
+        xp, rng = np, np.random.default_rng(seed)
+    else:
+        import torch
+        xp = torch
+        rng = torch.Generator(device=device)
+        rng.manual_seed(int(seed))
+    FB, K = _solenoidal_spectrum(M, kmin, kmax, rng, xp, device)
+    FU, _ = _solenoidal_spectrum(M, kmin, kmax, rng, xp, device)
+
+    def realise(Fc):
+        return _to_real(Fc, M, xp)
+
+    rB = realise(FB)
+    rU = realise(FU)
+    sB = float(dB_over_B0 * B0_uG) / float(xp.sqrt((rB * rB).sum(0).mean()))
+    sU = float(u_rms_kms) / float(xp.sqrt((rU * rU).sum(0).mean()))
+
+    def pack(rb, ru):
+        b = _with_seam(rb * sB, xp)
+        u = _with_seam(ru * sU, xp)
+        comps = [b[0], b[1], b[2] + B0_uG, u[0], u[1], u[2]]
+        out = []
+        for c in comps:
+            if xp is np:
+                c = np.ascontiguousarray(c, dtype=np.dtype(dtype))
+                out.append(c.transpose(2, 1, 0))   # (nx,ny,nz) view, x fastest
+            else:
+                c = c.to(getattr(xp, dtype)).contiguous()
+                out.append(c.permute(2, 1, 0))
+        return tuple(out)
+
+    result = {"total": pack(rB, rU)}
+    if mode_decompose:
+        mb = _mode_split(FB, K, xp)
+        mu = _mode_split(FU, K, xp)
+        for name in ("alfven", "fast", "slow"):
+            result[name] = pack(realise(mb[name]), realise(mu[name]))
+    return result
+
+
+def to_si(fields, velocity_unit_in_m_per_s=KMPS_TO_MPS):
+    """Scale (B in microgauss, u in velocity units) to (tesla, m/s), as
+    load_static_fields does (phase_space_gpu_runner.jl:114-119)."""
+    out = []
+    for i, a in enumerate(fields):
+        out.append(a * (UGAUSS_TO_T if i < 3 else velocity_unit_in_m_per_s))
+    return tuple(out)
