@@ -1,0 +1,254 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-steps/s of the BF16-field FP32 pusher on one MI355X.
+
+Workload (the precision-form configuration): a 512^3-cell periodic synthetic
+turbulence field (513^3 nodes, B0 = 5 uG z plus a k^-5/3 Gaussian field with
+dB/B0 = 1.0 over k = 2pi/L [1, 64], u rms 20 km/s, L = 10 pc) stored as BF16
+node records with per-component offset/scale; 4,194,304 protons at 100 GeV,
+uniform positions, isotropic momenta; 200 steps per reference gyroperiod T_g0;
+mu cached every 20 steps into device memory.  One bench "step" advances every
+particle by one T_g0 (200 Boris pushes, mu written 10 times); inputs are
+resident in HBM before timing starts.
+
+Beside the headline number the JSON line carries:
+  plain_f32     the reference-structured FP32 path (one launch per push, six
+                planar FP32 arrays, 48 corner gathers per push) on the same
+                particles, and the speed-up of the production path over it;
+  dmumu_check   D_mumu(|mu|) of the BF16 pusher against the FP64 pusher (the
+                oracle-pinned reference-structured kernel) on a 65,536-particle
+                subset;
+  roofline      HBM roofline of the production kernel (algorithmic bytes per
+                particle-step 48*2 + 36/K + 4/20, K pushes per launch) from
+                HIP-event launch durations on the library's stream, plus the
+                FP32 VALU fraction;
+  cpu_baseline  the CPU oracle (C restatement of the reference pusher, FP64,
+                one core) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FLOPS_PER_PUSH = 250.0  # FP32 operations of one production push (interp 42 FMA + Boris + move), see DESIGN.md
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_field(torch, n_nodes, dB, kmax, seed):
+    from particle_transport_amd import fields
+    raw = fields.generate_turbulence(n_nodes, dB_over_B0=dB, kmax=kmax, kmin=1.0, seed=seed, device="cuda",
+                                     dtype="float32")["total"]
+    return tuple(a * (fields.UGAUSS_TO_T if i < 3 else fields.KMPS_TO_MPS) for i, a in enumerate(raw))
+
+
+def make_particles(torch, n, L, energy_gev, seed):
+    from particle_transport_amd.injection import C_LIGHT, M_P, energy_to_gamma
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    x = torch.rand((3, n), generator=g, device="cuda", dtype=torch.float64) * L
+    d = torch.randn((3, n), generator=g, device="cuda", dtype=torch.float64)
+    d /= torch.linalg.norm(d, dim=0)
+    gamma = float(energy_to_gamma(energy_gev))
+    p = d * (gamma * M_P * C_LIGHT * math.sqrt(1 - 1 / gamma ** 2))
+    return x.contiguous(), p.contiguous(), gamma
+
+
+def traffic_from_profiles():
+    path = os.path.join(ROOT, "profiles", "traffic_push_fast_bf16_512.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
+
+
+def cpu_baseline(F32, L, x, p, dt, n_cpu, steps_cpu):
+    import numpy as np
+
+    import oracle
+    n_nodes = F32[0].shape[0]
+    F = [np.asfortranarray(c.double().cpu().numpy()) for c in F32]
+    g = oracle.make_geom(n_nodes, n_nodes, n_nodes, L)
+    xs = np.ascontiguousarray(x[:, :n_cpu].cpu().numpy())
+    ps = np.ascontiguousarray(p[:, :n_cpu].cpu().numpy())
+    t0 = time.perf_counter()
+    oracle.run(F, g, xs, ps, dt, 1, steps_cpu, steps_cpu + 1, 20, True, 2)
+    el = time.perf_counter() - t0
+    return n_cpu * steps_cpu / el, el
+
+
+def dmumu_check(torch, ctx, F32, L, x, p, dt, n_sub, steps, stride):
+    """D_mumu(|mu|, tau) of the BF16 production pusher vs the FP64 pusher."""
+    import numpy as np
+
+    from particle_transport_amd import dmumu
+    from particle_transport_amd.runner import Ensemble, Field
+    F64 = tuple(c.double() for c in F32)
+    f64 = Field(ctx, F64, L, "f64_planar")
+    del F64
+    res = {}
+    for name, pusher, field in (("fp64", "ref_f64", f64), ("bf16", "fast", None)):
+        fld = field or Field(ctx, F32, L, "bf16_brick")
+        ens = Ensemble(ctx, fld, x[:, :n_sub], p[:, :n_sub], pusher=pusher)
+        nsave = (steps - 1) // stride + 1
+        out = torch.empty((nsave, n_sub), device="cuda", dtype=torch.float32)
+        ens.advance(dt=dt, first_step=1, n_steps=steps, total_steps=steps, stride=stride, out_kind="mu", out_a=out,
+                    steps_per_launch=200)
+        ens.close()
+        edges = np.linspace(0.0, 1.0, 21)
+        lags = [1, 2, 4, 8, 16, 32, 64, 128]
+        acc = dmumu.new_accumulators(len(lags), 20)
+        dmumu.process_mu_chunk_dmumu(ctx, out, lags, edges, acc, "sliding", True, layout="particle_fastest")
+        tau = np.array(lags, dtype=np.float64)
+        D = acc.dmumu_sum_delta2 / np.maximum(acc.dmumu_counts, 1) / (2 * tau[:, None])
+        w = acc.dmumu_counts
+        res[name] = ((D * w).sum(0) / np.maximum(w.sum(0), 1), w)
+        fld.close()
+    a, b = res["bf16"][0], res["fp64"][0]
+    ok = res["fp64"][1].sum(0) >= 1000
+    rel = np.abs(a[ok] - b[ok]) / np.abs(b[ok])
+    return dict(bins=int(ok.sum()), max_rel_err=float(rel.max()), median_rel_err=float(np.median(rel)),
+                subset=n_sub, pushes=steps - 1, statistic="count-weighted tau-average of D_mumu(|mu|) over lags "
+                "1..128 cadence samples, 20 bins")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed bench steps (1 T_g0 each)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=512)
+    ap.add_argument("--particles", type=int, default=4194304)
+    ap.add_argument("--energy-gev", type=float, default=100.0)
+    ap.add_argument("--steps-per-gp", type=int, default=200)
+    ap.add_argument("--plain-steps", type=int, default=40)
+    ap.add_argument("--cpu-particles", type=int, default=32768)
+    ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the pusher on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from particle_transport_amd.fields import box_length_m
+    from particle_transport_amd.injection import M_P, Q_E
+    from particle_transport_amd.runner import Context, Ensemble, Field
+
+    L = box_length_m(10.0)
+    n = args.particles
+    n_nodes = args.cells + 1
+    stride = 20
+    S = args.steps_per_gp
+    log(f"[bench] generating {n_nodes}^3-node field on the GPU")
+    F32 = make_field(torch, n_nodes, 1.0, 64.0, seed=2024)
+    torch.cuda.synchronize()
+    ctx = Context(0)
+    log("[bench] packing BF16 and FP32 fields")
+    fb = Field(ctx, F32, L, "bf16_brick")
+    B0 = fb.B0
+    x, p, gamma = make_particles(torch, n, L, args.energy_gev, seed=99)
+    Omega0 = Q_E * B0 / (gamma * M_P)
+    dt = (2 * math.pi / S) / Omega0
+    total = 500 * S + 1
+
+    ens = Ensemble(ctx, fb, x, p, pusher="fast")
+    nsave = S // stride
+    mu_buf = torch.empty((nsave, n), device="cuda", dtype=torch.float32)
+    si = 1
+
+    def bench_step():
+        nonlocal si
+        r = ens.advance(dt=dt, first_step=si, n_steps=S, total_steps=total, stride=stride, out_kind="mu",
+                        out_a=mu_buf, steps_per_launch=S)
+        si += S
+        return r
+
+    log(f"[bench] warmup {args.warmup} x {S} pushes of {n} particles")
+    for _ in range(args.warmup):
+        bench_step()
+    torch.cuda.synchronize()
+    kms, launches = 0.0, 0
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = bench_step()
+        kms += r.kernel_ms
+        launches += r.launches
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ens.close()
+    value = n * S * args.steps / wall
+    avg_launch_s = kms / 1e3 / launches
+    K = S
+    b_alg = 48 * 2 + 36.0 / K + 4.0 / stride
+    achieved = b_alg * n * K / avg_launch_s / 1e9
+    kernel_rate = n * K / avg_launch_s
+    log(f"[bench] fast bf16: {value:.4g} particle-steps/s (kernel {kernel_rate:.4g}/s)")
+
+    # plain FP32 path, same particles
+    fp = Field(ctx, F32, L, "f32_planar")
+    ep = Ensemble(ctx, fp, x, p, pusher="plain_f32")
+    ep.advance(dt=dt, first_step=1, n_steps=2, total_steps=total, stride=stride)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rp = ep.advance(dt=dt, first_step=3, n_steps=args.plain_steps, total_steps=total, stride=stride)
+    plain_wall = time.perf_counter() - t0
+    ep.close()
+    fp.close()
+    plain_rate = n * args.plain_steps / plain_wall
+    plain_kernel = n * args.plain_steps / (rp.kernel_ms / 1e3)
+    log(f"[bench] plain f32: {plain_rate:.4g} particle-steps/s")
+
+    check = None
+    if not args.no_check:
+        log("[bench] D_mumu check on 65,536 particles")
+        check = dmumu_check(torch, ctx, F32, L, x, p, dt, 65536, 4001, stride)
+    fb.close()
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU oracle on {args.cpu_particles} particles x {args.cpu_steps} steps")
+        rate, el = cpu_baseline(F32, L, x, p, dt, args.cpu_particles, args.cpu_steps)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_particles} particles x {args.cpu_steps} pushes of the same field and dt, "
+                         f"FP64 oracle (C restatement of advance_particles_kernel!), mu every 20 steps, {el:.1f} s"}
+    ctx.close()
+
+    out = {
+        "metric": "particle-steps/s (FP32 vs BF16 field) at D_mumu err<=1%; p99 campaign latency",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"precision form: {args.cells}^3 cells BF16 field (dB/B0=1, k=[1,64]), {n} protons "
+                               f"at {args.energy_gev:g} GeV, {S} steps/T_g0, mu every {stride} steps; 1 step = 1 T_g0",
+                   "global_batch": n, "seq_len": S, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "bytes_per_particle_step": b_alg, "kernel_particle_steps_per_s": kernel_rate,
+                     "avg_launch_ms": avg_launch_s * 1e3,
+                     "fp32_valu_frac": kernel_rate * FLOPS_PER_PUSH / (FP32_PEAK_TFLOPS * 1e12)},
+        "plain_f32": {"value": plain_rate, "kernel_particle_steps_per_s": plain_kernel,
+                      "speedup_of_bf16_path": value / plain_rate},
+        "dmumu_check": check,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

