@@ -208,12 +208,83 @@ int pt_dpp_accumulate(pt_ctx *ctx, const float *momenta, int on_device, int64_t 
                       double *sum_delta_p, double *sum_delta_p2, double *sum_delta_p_norm,
                       double *sum_delta_p_norm2);
 
+/* process_momenta_chunk_dpp_cpu! (compute_dpp_full.jl:247-288) over a cache of
+ * |p| (kg m/s, FP64) laid out [save][total], as a serving request with
+ * PT_OUT_PMAG writes it; same accumulators and selection as pt_dpp_accumulate. */
+int pt_dpp_accumulate_pmag(pt_ctx *ctx, const double *pmag, int on_device, int64_t nsave,
+                           int64_t total, int64_t n_sel, const int64_t *particle_index,
+                           const int64_t *lag_steps, int64_t nlags, double p0, int64_t *counts,
+                           double *sum_delta_p, double *sum_delta_p2, double *sum_delta_p_norm,
+                           double *sum_delta_p_norm2);
+
 /* Replaces fill_energy_snapshots! (compute_dpp_full.jl:290-299): kinetic energy
  * in GeV of each selected particle at each snapshot save index (0-based),
  * written into out, Julia (k, out_cols) column-major, at column col0 + j. */
 int pt_energy_snapshots(pt_ctx *ctx, const float *momenta, int on_device, int64_t nsave,
                         int64_t total, int64_t n_sel, const int64_t *particle_index,
                         const int64_t *snapshot_idx, int64_t k, double *out, int64_t col0);
+
+/* ---- serving: continuous batching of transport campaigns ---------------- */
+
+/* No counterpart in the reference, which runs one (campaign, energy) job at a
+ * time from run_mode_campaigns (multimode_multienergy_cache_pipeline.jl:2924)
+ * through run_energy_pipeline! (:1857).  A server keeps fields resident,
+ * admits requests against an HBM budget (FIFO), and advances every admitted
+ * request in one fused multi-campaign push launch per tick, each with its own
+ * field, dt, charge, mass, duration and cadence.  Requests that arrive while
+ * others run are admitted at the next tick that has memory for them.  All
+ * fields of one server share one BRICK storage kind. */
+
+typedef struct pt_serve pt_serve;
+
+enum pt_serve_state { PT_REQ_QUEUED = 0, PT_REQ_RUNNING = 1, PT_REQ_DONE = 2, PT_REQ_RELEASED = 3 };
+
+/* Extra cadence output for serving: |p| (kg m/s, FP64) for D_pp. */
+#define PT_OUT_PMAG 3
+
+typedef struct {
+    int32_t field_id;        /* from pt_serve_add_field */
+    int32_t out_kind;        /* PT_OUT_MU (float32 cache) or PT_OUT_PMAG (float64 cache) */
+    int64_t n;               /* particles */
+    double dt_s, charge_C, mass_kg;
+    int64_t total_steps;     /* n_integration_steps: steps si = 1..total_steps */
+    int64_t save_stride;     /* cadence (trajectory_time_stride) */
+    int32_t periodic;
+    int32_t inject_fixed_mu; /* 1: momenta from mu0 about local B (fixed_mu_unit_vector,
+                                phase_space_gpu_runner.jl:170-194) on the device */
+    double mu0;              /* injection_mu0 for inject_fixed_mu */
+    double p_magnitude;      /* |p| (kg m/s) for inject_fixed_mu */
+    const double *x[3];      /* host positions (m) */
+    const double *p[3];      /* host momenta (kg m/s); unused when inject_fixed_mu */
+    const double *phase;     /* host gyrophases 2*pi*rand for inject_fixed_mu */
+} pt_serve_request;
+
+int pt_serve_create(pt_ctx *ctx, int64_t hbm_budget_bytes, int32_t steps_per_tick,
+                    int32_t max_active, pt_serve **out);
+int pt_serve_destroy(pt_serve *srv);
+int pt_serve_add_field(pt_serve *srv, const pt_field *field, int32_t *field_id);
+/* Copies the initial state; the request waits in the FIFO until admitted. */
+int pt_serve_submit(pt_serve *srv, const pt_serve_request *req, int64_t *request_id);
+/* Completes the previous tick (requests that finished become DONE), admits
+ * queued requests that fit, and enqueues the next tick without waiting for
+ * it.  Counts are after admission. */
+int pt_serve_step(pt_serve *srv, int64_t *n_running, int64_t *n_queued, int64_t *n_done);
+/* Waits for the enqueued tick, if any, and completes it. */
+int pt_serve_drain(pt_serve *srv);
+int pt_serve_poll(pt_serve *srv, int64_t request_id, int32_t *state);
+/* For a DONE request: the device cadence cache [save][n] (float32 mu or
+ * float64 |p|), its shape and the number of particles alive at the end. */
+int pt_serve_fetch(pt_serve *srv, int64_t request_id, const void **cache_device, int64_t *nsave,
+                   int64_t *n, int64_t *alive_final);
+/* Copies a DONE request's cadence cache to host memory (bytes must equal
+ * nsave * n * element size). */
+int pt_serve_copy_cache(pt_serve *srv, int64_t request_id, void *host_dst, int64_t bytes);
+/* Frees a DONE request's device memory (or drops a QUEUED one). */
+int pt_serve_release(pt_serve *srv, int64_t request_id);
+/* Budget bytes held by admitted requests, particle-steps pushed so far and
+ * the summed kernel time of all ticks (HIP events on the server's stream). */
+int pt_serve_stats(pt_serve *srv, int64_t *bytes_reserved, int64_t *particle_steps,
+                   double *kernel_ms);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,7 @@ SOURCES = {
     "pt_push_ref.hip": ["-ffp-contract=off"],
     "pt_push_fast.hip": [],
     "pt_stats.hip": ["-munsafe-fp-atomics"],
+    "pt_serve.hip": [],
 }
 
 

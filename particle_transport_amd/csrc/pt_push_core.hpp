@@ -1,0 +1,275 @@
+// Device core shared by the FP32 split-precision pushers (pt_push_fast.hip)
+// and the multi-campaign serving kernel (pt_serve.hip): field views, the
+// Boris step in folded units, split-position moves, corner loads and the
+// per-cell trilinear polynomial.  See pt_push_fast.hip for the math.
+#pragma once
+
+#include "pt_internal.hpp"
+
+namespace pt {
+namespace core {
+
+struct FieldView {
+    const void *data;
+    int64_t n[3];
+    int64_t nb[3];
+    float scale[6];
+    float offset[6];
+};
+
+struct SplitGeom {
+    int32_t nc[3];   // cells per axis (nodes - 1)
+    float k[3];      // c dt / dx per axis
+    double d[3];     // node spacing (m), for phase-space output
+};
+
+struct Phys {
+    float fac[6];    // B: q dt / 2m ; u: 1/c
+    float qsign;     // sign of the charge (mu is defined against B, not B~)
+    float mr;        // M_P / m : state momentum (units M_P c) -> units m c
+    double pout;     // m c : momentum in units m c -> kg m/s
+};
+
+// ------------------------------------------------------- shared math -------
+
+__device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, float bx, float by,
+                                            float bz, float ux, float uy, float uz, float &inv_gn) {
+    // E~ = -(u~ x B~)
+    const float ex = -(uy * bz - uz * by);
+    const float ey = -(uz * bx - ux * bz);
+    const float ez = -(ux * by - uy * bx);
+    const float pmx = Ux + ex, pmy = Uy + ey, pmz = Uz + ez;
+    const float gm = sqrtf(1.0f + (pmx * pmx + pmy * pmy + pmz * pmz));
+    const float r = 1.0f / gm;
+    const float tx = bx * r, ty = by * r, tz = bz * r;
+    const float t2 = tx * tx + ty * ty + tz * tz;
+    const float f = 2.0f / (1.0f + t2);
+    const float sx = tx * f, sy = ty * f, sz = tz * f;
+    const float ppx = pmx + (pmy * tz - pmz * ty);
+    const float ppy = pmy + (pmz * tx - pmx * tz);
+    const float ppz = pmz + (pmx * ty - pmy * tx);
+    Ux = pmx + (ppy * sz - ppz * sy) + ex;
+    Uy = pmy + (ppz * sx - ppx * sz) + ey;
+    Uz = pmz + (ppx * sy - ppy * sx) + ez;
+    inv_gn = 1.0f / sqrtf(1.0f + (Ux * Ux + Uy * Uy + Uz * Uz));
+}
+
+// Advance one axis of the split position by du cells; returns false if an
+// open boundary kills the particle.
+__device__ __forceinline__ void move_axis(int32_t &cell, float &off, float du, int32_t nc,
+                                          bool periodic) {
+    off += du;
+    if (!(off >= 0.0f && off < 1.0f)) {
+        const float fl = floorf(off);
+        cell += static_cast<int32_t>(fl);
+        off -= fl;
+        if (off >= 1.0f) {  // -tiny + 1 rounds to 1: stay put at offset 0
+            off -= 1.0f;
+            cell += 1;
+        }
+        if (periodic && (cell < 0 || cell >= nc)) {
+            cell %= nc;
+            if (cell < 0) cell += nc;
+        }
+    }
+}
+
+// x > L or x < 0 kills (phase_space_gpu_runner.jl:385-387); x == L is cell nc, offset 0.
+__device__ __forceinline__ bool outside(int32_t cell, float off, int32_t nc) {
+    return cell < 0 || cell > nc || (cell == nc && off > 0.0f);
+}
+
+// Gather cell and fractional coordinate: x == L gathers the last cell at t = 1
+// (the clamp of i0 to nx-1 in trilinear_sample).
+__device__ __forceinline__ void gather_coord(int32_t cell, float off, int32_t nc, int32_t &g,
+                                             float &t) {
+    g = cell;
+    t = off;
+    if (g > nc - 1) {
+        t += static_cast<float>(g - (nc - 1));
+        g = nc - 1;
+    }
+    // A live particle always has 0 <= cell <= nc; the clamp keeps every field
+    // access in bounds even for a state that is not (e.g. non-finite input).
+    g = g < 0 ? 0 : g;
+}
+
+template <class T>
+__device__ __forceinline__ void put(void *out, int64_t idx, double v) {
+    static_cast<T *>(out)[idx] = static_cast<T>(v);
+}
+
+__device__ __forceinline__ float mu_f32(float px, float py, float pz, float bx, float by, float bz,
+                                        float qsign) {
+    const float pn = sqrtf(px * px + py * py + pz * pz);
+    const float bn = sqrtf(bx * bx + by * by + bz * bz);
+    if (!isfinite(pn) || !isfinite(bn) || pn == 0.0f || bn == 0.0f) return NAN;
+    const float mu = qsign * (px * bx + py * by + pz * bz) / (pn * bn);
+    return fminf(fmaxf(mu, -1.0f), 1.0f);
+}
+
+// Save-step output of one particle: state before the push of step si, as
+// the reference's *_step copies (:351-356) or mu (pipeline :1349-1374).
+__device__ __forceinline__ void write_save(int out_kind, int out_f64, void *out_a, void *out_b,
+                                           int64_t slot, int64_t n, int64_t p, bool alive,
+                                           const int32_t cell[3], const float off[3],
+                                           const float U[3], const float bt[3],
+                                           const SplitGeom &g, const Phys &ph) {
+    if (out_kind == PT_OUT_PHASE) {
+        for (int c = 0; c < 3; ++c) {
+            const double xv = alive ? (static_cast<double>(cell[c]) + static_cast<double>(off[c])) * g.d[c] : NAN;
+            const double pv = alive ? static_cast<double>(U[c]) * ph.pout : NAN;
+            const int64_t idx = (slot * 3 + c) * n + p;
+            if (out_f64) {
+                put<double>(out_a, idx, xv);
+                put<double>(out_b, idx, pv);
+            } else {
+                put<float>(out_a, idx, xv);
+                put<float>(out_b, idx, pv);
+            }
+        }
+    } else {
+        const float mu = alive ? mu_f32(U[0], U[1], U[2], bt[0], bt[1], bt[2], ph.qsign) : NAN;
+        if (out_f64)
+            put<double>(out_a, slot * n + p, mu);
+        else
+            put<float>(out_a, slot * n + p, mu);
+    }
+}
+
+// Map hardware workgroup ids (dispatched round-robin over the 8 XCDs) so that
+// each XCD processes one contiguous range of particles, keeping a spatially
+// sorted ensemble's field records in that XCD's L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblk) {
+    const int64_t xcd = b & 7, q = nblk >> 3, r = nblk & 7;
+    return xcd * q + (xcd < r ? xcd : r) + (b >> 3);
+}
+
+// Corner values of the six components at the 8 nodes of cell (gi,gj,gk),
+// corner index ci = dx + 2 dy + 4 dz, in storage units.
+template <int STORE>
+__device__ __forceinline__ void load_corners(const FieldView &fv, int32_t gi, int32_t gj, int32_t gk,
+                                             float s[6][8]) {
+#pragma unroll
+    for (int ci = 0; ci < 8; ++ci) {
+        const int64_t i = gi + (ci & 1), j = gj + ((ci >> 1) & 1), k = gk + (ci >> 2);
+        if constexpr (STORE == PT_STORE_F32_PLANAR) {
+            const int64_t nn = fv.n[0] * fv.n[1] * fv.n[2];
+            const int64_t idx = i + fv.n[0] * (j + fv.n[1] * k);
+            const float *F = static_cast<const float *>(fv.data);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) s[c][ci] = F[c * nn + idx];
+        } else {
+            const int64_t rec = brick_node_index(fv.nb, i, j, k);
+            if constexpr (STORE == PT_STORE_F32_BRICK) {
+                const float4 *R = static_cast<const float4 *>(fv.data) + rec * 2;
+                const float4 a = R[0];
+                const float4 b = R[1];
+                s[0][ci] = a.x;
+                s[1][ci] = a.y;
+                s[2][ci] = a.z;
+                s[3][ci] = a.w;
+                s[4][ci] = b.x;
+                s[5][ci] = b.y;
+            } else {
+                const uint4 w = static_cast<const uint4 *>(fv.data)[rec];
+                if constexpr (STORE == PT_STORE_BF16_BRICK) {
+                    s[0][ci] = __uint_as_float(w.x << 16);
+                    s[1][ci] = __uint_as_float(w.x & 0xffff0000u);
+                    s[2][ci] = __uint_as_float(w.y << 16);
+                    s[3][ci] = __uint_as_float(w.y & 0xffff0000u);
+                    s[4][ci] = __uint_as_float(w.z << 16);
+                    s[5][ci] = __uint_as_float(w.z & 0xffff0000u);
+                } else {
+                    s[0][ci] = f16_bits_to_float(static_cast<uint16_t>(w.x & 0xffffu));
+                    s[1][ci] = f16_bits_to_float(static_cast<uint16_t>(w.x >> 16));
+                    s[2][ci] = f16_bits_to_float(static_cast<uint16_t>(w.y & 0xffffu));
+                    s[3][ci] = f16_bits_to_float(static_cast<uint16_t>(w.y >> 16));
+                    s[4][ci] = f16_bits_to_float(static_cast<uint16_t>(w.z & 0xffffu));
+                    s[5][ci] = f16_bits_to_float(static_cast<uint16_t>(w.z >> 16));
+                }
+            }
+        }
+    }
+}
+
+// Trilinear polynomial coefficients of one component over a cell:
+// f(x,y,z) = a0 + a1 x + a2 y + a3 z + a4 xy + a5 xz + a6 yz + a7 xyz,
+// decoded (stored*scale + offset) and multiplied by the physical factor.
+__device__ __forceinline__ void coeffs(const float s[8], float scale, float offset, float a[8]) {
+    const float dx00 = s[1] - s[0], dx10 = s[3] - s[2], dx01 = s[5] - s[4], dx11 = s[7] - s[6];
+    const float dy0 = s[2] - s[0], dy1 = s[6] - s[4];
+    a[0] = s[0] * scale + offset;
+    a[1] = dx00 * scale;
+    a[2] = dy0 * scale;
+    a[3] = (s[4] - s[0]) * scale;
+    a[4] = (dx10 - dx00) * scale;
+    a[5] = (dx01 - dx00) * scale;
+    a[6] = (dy1 - dy0) * scale;
+    a[7] = ((dx11 - dx01) - (dx10 - dx00)) * scale;
+}
+
+__device__ __forceinline__ float eval_poly(const float a[8], float x, float y, float z) {
+    const float z0 = fmaf(a[3], z, a[0]);
+    const float z1 = fmaf(a[5], z, a[1]);
+    const float z2 = fmaf(a[6], z, a[2]);
+    const float z3 = fmaf(a[7], z, a[4]);
+    return fmaf(fmaf(z3, x, z2), y, fmaf(z1, x, z0));
+}
+
+
+// The production step loop of one lane: K steps from si0 with state and the
+// current cell's coefficients in registers.  save(si, alive, cell, off, U, v)
+// is called at every cadence step (before the push, as the reference's
+// *_step copies); die(si) when an open boundary removes the particle.
+template <int STORE, class SaveFn, class DieFn>
+__device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[3], bool &alive,
+                                         const FieldView &fv, const SplitGeom &g, const Phys &ph,
+                                         int64_t si0, int K, int64_t total_steps, int64_t stride,
+                                         bool want_save, int periodic, SaveFn save, DieFn die) {
+    int32_t cg0 = -1, cg1 = -1, cg2 = -1;
+    float a[6][8];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a[c][q] = 0.0f;
+    for (int k = 0; k < K; ++k) {
+        const int64_t si = si0 + k;
+        const bool do_save = want_save && ((si - 1) % stride) == 0;
+        const bool push = alive && si < total_steps;
+        if (!do_save && !push) continue;
+        float v[6] = {0, 0, 0, 0, 0, 0};
+        if (alive) {
+            int32_t gi, gj, gk;
+            float tx, ty, tz;
+            gather_coord(cell[0], off[0], g.nc[0], gi, tx);
+            gather_coord(cell[1], off[1], g.nc[1], gj, ty);
+            gather_coord(cell[2], off[2], g.nc[2], gk, tz);
+            if (gi != cg0 || gj != cg1 || gk != cg2) {
+                float sv[6][8];
+                load_corners<STORE>(fv, gi, gj, gk, sv);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) coeffs(sv[c], fv.scale[c] * ph.fac[c], fv.offset[c] * ph.fac[c], a[c]);
+                cg0 = gi;
+                cg1 = gj;
+                cg2 = gk;
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) v[c] = eval_poly(a[c], tx, ty, tz);
+        }
+        if (do_save) save(si, alive, cell, off, U, v);
+        if (!push) continue;
+        float inv_gn;
+        boris_split(U[0], U[1], U[2], v[0], v[1], v[2], v[3], v[4], v[5], inv_gn);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) move_axis(cell[c], off[c], U[c] * (g.k[c] * inv_gn), g.nc[c], periodic);
+        if (!periodic && (outside(cell[0], off[0], g.nc[0]) || outside(cell[1], off[1], g.nc[1]) ||
+                          outside(cell[2], off[2], g.nc[2]))) {
+            alive = false;
+            die(si);
+        }
+    }
+}
+
+}  // namespace core
+}  // namespace pt

@@ -1,0 +1,617 @@
+// Continuous-batching campaign server.
+//
+// Requests (field x energy x injection x duration, uneven particle counts)
+// are admitted FIFO against an HBM budget for their particle state plus
+// cadence cache, and every admitted request is advanced by one fused launch
+// per tick: the grid is the concatenation of every request's 256-particle
+// blocks, each block finds its request in a device table (binary search on
+// the block prefix) and runs the production step loop (core::run_lane) with
+// that request's own field, dt, charge, mass, duration and cadence.  A tick
+// is enqueued without waiting for it; the next pt_serve_step completes it,
+// retires finished requests and admits queued ones, so requests arriving
+// while others run join at the next tick and share the device.
+//
+// The tick launch is an ordinary kernel rather than a spin-waiting resident
+// kernel: the host changes the request mix between ticks, and no kernel ever
+// waits on host memory.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "pt_push_core.hpp"
+
+namespace pt {
+namespace {
+
+using namespace core;
+
+struct DevReq {
+    int32_t *cell[3];
+    float *off[3];
+    float *U[3];            // momentum / (m c) of this request's species
+    uint8_t *alive;
+    unsigned long long *deaths;
+    int *bad;               // set by the injection kernel on zero/non-finite local B
+    void *out;              // cadence cache [save][n]
+    int64_t n, si0, K, total, stride, block0;
+    int32_t out_kind, periodic;
+    FieldView fv;
+    SplitGeom g;
+    Phys ph;
+};
+
+template <int STORE>
+__global__ void __launch_bounds__(256) k_serve(const DevReq *__restrict__ reqs, int nreq) {
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = nreq - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].block0 <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const DevReq &R = reqs[lo];
+    const int64_t p = (b - R.block0) * blockDim.x + threadIdx.x;
+    const int64_t n = R.n;
+    if (p >= n) return;
+    bool alive = R.alive[p] != 0;
+    int32_t cell[3];
+    float off[3], U[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        cell[c] = R.cell[c][p];
+        off[c] = R.off[c][p];
+        U[c] = R.U[c][p];
+    }
+    const int kind = R.out_kind;
+    void *out = R.out;
+    const int64_t stride = R.stride;
+    const float qsign = R.ph.qsign;
+    const double pout = R.ph.pout;
+    run_lane<STORE>(
+        cell, off, U, alive, R.fv, R.g, R.ph, R.si0, static_cast<int>(R.K), R.total, stride, kind != PT_OUT_NONE,
+        R.periodic,
+        [&](int64_t si, bool alv, const int32_t *, const float *, const float *u, const float *v) {
+            const int64_t slot = (si - 1) / stride;
+            if (kind == PT_OUT_MU) {
+                static_cast<float *>(out)[slot * n + p] = alv ? mu_f32(u[0], u[1], u[2], v[0], v[1], v[2], qsign) : NAN;
+            } else {
+                const double ux = u[0], uy = u[1], uz = u[2];
+                static_cast<double *>(out)[slot * n + p] = alv ? sqrt(ux * ux + uy * uy + uz * uz) * pout : NAN;
+            }
+        },
+        [&](int64_t) { atomicAdd(R.deaths, 1ull); });
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        R.cell[c][p] = cell[c];
+        R.off[c][p] = off[c];
+        R.U[c][p] = U[c];
+    }
+    R.alive[p] = alive ? 1 : 0;
+}
+
+// Initial state of one request: split position from metres; momentum either
+// given (isotropic or any host-built state) or built on the device for a
+// fixed pitch cosine mu0 about the local field (fixed_mu_unit_vector,
+// phase_space_gpu_runner.jl:170-194) from the request's gyrophase draws.
+template <int STORE>
+__global__ void k_serve_init(DevReq R, const double *__restrict__ init, int fixed_mu, float mu0, double pnorm_inv,
+                             double p_magnitude) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t n = R.n;
+    if (i >= n) return;
+    for (int c = 0; c < 3; ++c) {
+        const double u = init[c * n + i] / R.g.d[c];
+        const double fl = floor(u);
+        float off = static_cast<float>(u - fl);
+        int32_t cell = static_cast<int32_t>(fl);
+        if (off >= 1.0f) {
+            off = 0.0f;
+            cell += 1;
+        }
+        R.cell[c][i] = cell;
+        R.off[c][i] = off;
+    }
+    if (!fixed_mu) {
+        for (int c = 0; c < 3; ++c) R.U[c][i] = static_cast<float>(init[(3 + c) * n + i] * pnorm_inv);
+        return;
+    }
+    int32_t g[3];
+    float t[3];
+    for (int c = 0; c < 3; ++c) gather_coord(R.cell[c][i], R.off[c][i], R.g.nc[c], g[c], t[c]);
+    float sv[6][8], a[8], b[3];
+    load_corners<STORE>(R.fv, g[0], g[1], g[2], sv);
+    for (int c = 0; c < 3; ++c) {
+        coeffs(sv[c], R.fv.scale[c], R.fv.offset[c], a);
+        b[c] = eval_poly(a, t[0], t[1], t[2]);
+    }
+    const float bn = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    if (!isfinite(bn) || bn == 0.0f) {
+        atomicExch(R.bad, 1);
+        for (int c = 0; c < 3; ++c) R.U[c][i] = 0.0f;
+        return;
+    }
+    const float bhx = b[0] / bn, bhy = b[1] / bn, bhz = b[2] / bn;
+    float e1x, e1y, e1z;
+    if (fabsf(bhx) < 0.9f) {
+        e1x = 0.0f;
+        e1y = bhz;
+        e1z = -bhy;
+    } else {
+        e1x = -bhz;
+        e1y = 0.0f;
+        e1z = bhx;
+    }
+    const float e1n = sqrtf(e1x * e1x + e1y * e1y + e1z * e1z);
+    e1x /= e1n;
+    e1y /= e1n;
+    e1z /= e1n;
+    const float e2x = bhy * e1z - bhz * e1y;
+    const float e2y = bhz * e1x - bhx * e1z;
+    const float e2z = bhx * e1y - bhy * e1x;
+    const float phi = static_cast<float>(init[3 * n + i]);
+    const float perp = sqrtf(fmaxf(0.0f, 1.0f - mu0 * mu0));
+    const float cp = cosf(phi), sp = sinf(phi);
+    const float scale = static_cast<float>(p_magnitude * pnorm_inv);
+    R.U[0][i] = scale * (mu0 * bhx + perp * (cp * e1x + sp * e2x));
+    R.U[1][i] = scale * (mu0 * bhy + perp * (cp * e1y + sp * e2y));
+    R.U[2][i] = scale * (mu0 * bhz + perp * (cp * e1z + sp * e2z));
+}
+
+template <class F>
+int guarded(F &&fn) {
+    try {
+        fn();
+        return PT_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return PT_ERR_MEMORY;
+    } catch (...) {
+        set_error("unknown C++ exception");
+        return PT_ERR_STATE;
+    }
+}
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+struct SReq {
+    int64_t id = 0;
+    int state = PT_REQ_QUEUED;
+    pt_serve_request d{};
+    std::vector<double> init;
+    char *mem = nullptr;
+    size_t bytes = 0;
+    int64_t si = 1;
+    int64_t nsave = 0;
+    int64_t alive_final = 0;
+    bool failed = false;
+    DevReq dev{};
+};
+
+}  // namespace
+}  // namespace pt
+
+struct pt_serve {
+    pt_ctx *ctx = nullptr;
+    int64_t budget = 0;
+    int32_t steps_per_tick = 200;
+    int32_t max_active = 1024;
+    std::vector<const pt_field *> fields;
+    int storage = -1;
+    std::map<int64_t, pt::SReq> reqs;
+    std::deque<int64_t> queue;
+    std::vector<int64_t> active;
+    int64_t reserved = 0, next_id = 1, n_done = 0;
+    pt::DevReq *d_table = nullptr, *h_table = nullptr;
+    int table_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool pending = false;
+    std::vector<std::pair<int64_t, int64_t>> tick_work;  // (request id, steps)
+    int64_t particle_steps = 0;
+    double kernel_ms = 0.0;
+};
+
+namespace pt {
+namespace {
+
+size_t request_bytes(const pt_serve_request &d, int64_t nsave) {
+    const size_t n = static_cast<size_t>(d.n);
+    const size_t esz = d.out_kind == PT_OUT_PMAG ? 8 : 4;
+    size_t b = 9 * align_up(n * 4) + align_up(n) + kAlign;
+    if (d.out_kind != PT_OUT_NONE) b += align_up(static_cast<size_t>(nsave) * n * esz);
+    return b;
+}
+
+void fill_dev(pt_serve *srv, SReq &r) {
+    const pt_field *f = srv->fields[static_cast<size_t>(r.d.field_id)];
+    const size_t n = static_cast<size_t>(r.d.n);
+    char *m = r.mem;
+    DevReq &D = r.dev;
+    for (int c = 0; c < 3; ++c) {
+        D.cell[c] = reinterpret_cast<int32_t *>(m);
+        m += align_up(n * 4);
+    }
+    for (int c = 0; c < 3; ++c) {
+        D.off[c] = reinterpret_cast<float *>(m);
+        m += align_up(n * 4);
+    }
+    for (int c = 0; c < 3; ++c) {
+        D.U[c] = reinterpret_cast<float *>(m);
+        m += align_up(n * 4);
+    }
+    D.alive = reinterpret_cast<uint8_t *>(m);
+    m += align_up(n);
+    D.deaths = reinterpret_cast<unsigned long long *>(m);
+    D.bad = reinterpret_cast<int *>(m + 8);
+    m += kAlign;
+    D.out = r.d.out_kind != PT_OUT_NONE ? m : nullptr;
+    D.n = r.d.n;
+    D.total = r.d.total_steps;
+    D.stride = r.d.save_stride;
+    D.out_kind = r.d.out_kind;
+    D.periodic = r.d.periodic ? 1 : 0;
+    D.fv.data = f->data;
+    for (int c = 0; c < 3; ++c) {
+        D.fv.n[c] = f->g.n[c];
+        D.fv.nb[c] = f->g.nb[c];
+        D.g.nc[c] = static_cast<int32_t>(f->g.n[c] - 1);
+        D.g.k[c] = static_cast<float>(kC * r.d.dt_s / f->g.d[c]);
+        D.g.d[c] = f->g.d[c];
+    }
+    const double bfac = r.d.charge_C * r.d.dt_s / (2.0 * r.d.mass_kg);
+    for (int c = 0; c < 6; ++c) {
+        D.fv.scale[c] = f->sc.scale[c];
+        D.fv.offset[c] = f->sc.offset[c];
+        D.ph.fac[c] = static_cast<float>(c < 3 ? bfac : 1.0 / kC);
+    }
+    D.ph.qsign = r.d.charge_C > 0 ? 1.0f : -1.0f;
+    D.ph.mr = 1.0f;  // state is already in units of this species' m c
+    D.ph.pout = r.d.mass_kg * kC;
+}
+
+template <int STORE>
+void launch_init(pt_serve *srv, SReq &r, const double *dinit) {
+    const unsigned blocks = static_cast<unsigned>((r.d.n + 255) / 256);
+    k_serve_init<STORE><<<blocks, 256, 0, srv->ctx->stream>>>(r.dev, dinit, r.d.inject_fixed_mu,
+                                                               static_cast<float>(r.d.mu0),
+                                                               1.0 / (r.d.mass_kg * kC), r.d.p_magnitude);
+    PT_HIP_CHECK(hipGetLastError());
+}
+
+void admit(pt_serve *srv, SReq &r) {
+    hipStream_t st = srv->ctx->stream;
+    void *mem = nullptr;
+    hipError_t e = hipMallocAsync(&mem, r.bytes, st);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{PT_ERR_MEMORY, std::string("hipMallocAsync failed: ") + hipGetErrorString(e)};
+    }
+    r.mem = static_cast<char *>(mem);
+    srv->ctx->bytes_in_use += static_cast<int64_t>(r.bytes);
+    fill_dev(srv, r);
+    double *dinit = static_cast<double *>(ctx_scratch(srv->ctx, r.init.size() * sizeof(double)));
+    PT_HIP_CHECK(hipMemcpyAsync(dinit, r.init.data(), r.init.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    PT_HIP_CHECK(hipMemsetAsync(r.dev.alive, 1, static_cast<size_t>(r.d.n), st));
+    PT_HIP_CHECK(hipMemsetAsync(r.dev.deaths, 0, kAlign, st));
+    switch (srv->storage) {
+        case PT_STORE_F32_BRICK: launch_init<PT_STORE_F32_BRICK>(srv, r, dinit); break;
+        case PT_STORE_BF16_BRICK: launch_init<PT_STORE_BF16_BRICK>(srv, r, dinit); break;
+        case PT_STORE_F16_BRICK: launch_init<PT_STORE_F16_BRICK>(srv, r, dinit); break;
+        default: throw Error{PT_ERR_STATE, "server storage not set"};
+    }
+    if (r.d.inject_fixed_mu) {
+        int bad = 0;
+        PT_HIP_CHECK(hipMemcpyAsync(&bad, r.dev.bad, sizeof(int), hipMemcpyDeviceToHost, st));
+        PT_HIP_CHECK(hipStreamSynchronize(st));
+        if (bad) r.failed = true;
+    }
+    std::vector<double>().swap(r.init);
+    srv->reserved += static_cast<int64_t>(r.bytes);
+    r.state = PT_REQ_RUNNING;
+}
+
+void complete_tick(pt_serve *srv) {
+    if (!srv->pending) return;
+    PT_HIP_CHECK(hipEventSynchronize(srv->ev1));
+    float ms = 0.f;
+    PT_HIP_CHECK(hipEventElapsedTime(&ms, srv->ev0, srv->ev1));
+    srv->kernel_ms += ms;
+    srv->pending = false;
+    std::vector<int64_t> still;
+    for (auto &w : srv->tick_work) {
+        SReq &r = srv->reqs.at(w.first);
+        const int64_t pushes = std::max<int64_t>(0, std::min(w.second, r.d.total_steps - r.si));
+        srv->particle_steps += pushes * r.d.n;
+        r.si += w.second;
+    }
+    for (int64_t id : srv->active) {
+        SReq &r = srv->reqs.at(id);
+        if (r.si > r.d.total_steps) {
+            unsigned long long deaths = 0;
+            PT_HIP_CHECK(hipMemcpy(&deaths, r.dev.deaths, 8, hipMemcpyDeviceToHost));
+            r.alive_final = r.d.n - static_cast<int64_t>(deaths);
+            r.state = PT_REQ_DONE;
+            srv->n_done += 1;
+        } else {
+            still.push_back(id);
+        }
+    }
+    srv->active.swap(still);
+    srv->tick_work.clear();
+}
+
+template <int STORE>
+void launch_tick(pt_serve *srv, int nreq, int64_t blocks) {
+    k_serve<STORE><<<static_cast<unsigned>(blocks), 256, 0, srv->ctx->stream>>>(srv->d_table, nreq);
+    PT_HIP_CHECK(hipGetLastError());
+}
+
+void enqueue_tick(pt_serve *srv) {
+    const int nreq = static_cast<int>(srv->active.size());
+    if (nreq == 0) return;
+    if (srv->table_cap < nreq) {
+        PT_HIP_CHECK(hipStreamSynchronize(srv->ctx->stream));
+        if (srv->d_table) (void)hipFree(srv->d_table);
+        if (srv->h_table) (void)hipHostFree(srv->h_table);
+        srv->d_table = nullptr;
+        srv->h_table = nullptr;
+        const int cap = std::max(nreq, 2 * srv->table_cap);
+        PT_HIP_CHECK(hipMalloc(&srv->d_table, sizeof(DevReq) * cap));
+        PT_HIP_CHECK(hipHostMalloc(&srv->h_table, sizeof(DevReq) * cap, hipHostMallocDefault));
+        srv->table_cap = cap;
+    }
+    int64_t blocks = 0;
+    srv->tick_work.clear();
+    for (int i = 0; i < nreq; ++i) {
+        SReq &r = srv->reqs.at(srv->active[static_cast<size_t>(i)]);
+        const int64_t K = std::min<int64_t>(srv->steps_per_tick, r.d.total_steps - r.si + 1);
+        r.dev.si0 = r.si;
+        r.dev.K = K;
+        r.dev.block0 = blocks;
+        blocks += (r.d.n + 255) / 256;
+        srv->h_table[i] = r.dev;
+        srv->tick_work.emplace_back(r.id, K);
+    }
+    PT_REQUIRE(blocks < (1LL << 31), "pt_serve_step: tick grid too large");
+    hipStream_t st = srv->ctx->stream;
+    PT_HIP_CHECK(hipEventRecord(srv->ev0, st));
+    PT_HIP_CHECK(hipMemcpyAsync(srv->d_table, srv->h_table, sizeof(DevReq) * nreq, hipMemcpyHostToDevice, st));
+    switch (srv->storage) {
+        case PT_STORE_F32_BRICK: launch_tick<PT_STORE_F32_BRICK>(srv, nreq, blocks); break;
+        case PT_STORE_BF16_BRICK: launch_tick<PT_STORE_BF16_BRICK>(srv, nreq, blocks); break;
+        case PT_STORE_F16_BRICK: launch_tick<PT_STORE_F16_BRICK>(srv, nreq, blocks); break;
+        default: throw Error{PT_ERR_STATE, "server storage not set"};
+    }
+    PT_HIP_CHECK(hipEventRecord(srv->ev1, st));
+    srv->pending = true;
+}
+
+}  // namespace
+}  // namespace pt
+
+using namespace pt;
+
+extern "C" {
+
+int pt_serve_create(pt_ctx *ctx, int64_t hbm_budget_bytes, int32_t steps_per_tick, int32_t max_active,
+                    pt_serve **out) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && out, "pt_serve_create: null argument");
+        PT_REQUIRE(hbm_budget_bytes > 0, "pt_serve_create: budget must be positive");
+        PT_REQUIRE(steps_per_tick >= 1, "pt_serve_create: steps_per_tick must be >= 1");
+        PT_REQUIRE(max_active >= 1, "pt_serve_create: max_active must be >= 1");
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        auto *s = new pt_serve();
+        s->ctx = ctx;
+        s->budget = hbm_budget_bytes;
+        s->steps_per_tick = steps_per_tick;
+        s->max_active = max_active;
+        PT_HIP_CHECK(hipEventCreate(&s->ev0));
+        PT_HIP_CHECK(hipEventCreate(&s->ev1));
+        *out = s;
+    });
+}
+
+int pt_serve_destroy(pt_serve *srv) {
+    return guarded([&] {
+        if (!srv) return;
+        try {
+            complete_tick(srv);
+        } catch (...) {
+        }
+        hipStream_t st = srv->ctx->stream;
+        for (auto &kv : srv->reqs)
+            if (kv.second.mem) {
+                (void)hipFreeAsync(kv.second.mem, st);
+                srv->ctx->bytes_in_use -= static_cast<int64_t>(kv.second.bytes);
+            }
+        (void)hipStreamSynchronize(st);
+        if (srv->d_table) (void)hipFree(srv->d_table);
+        if (srv->h_table) (void)hipHostFree(srv->h_table);
+        (void)hipEventDestroy(srv->ev0);
+        (void)hipEventDestroy(srv->ev1);
+        delete srv;
+    });
+}
+
+int pt_serve_add_field(pt_serve *srv, const pt_field *field, int32_t *field_id) {
+    return guarded([&] {
+        PT_REQUIRE(srv && field && field_id, "pt_serve_add_field: null argument");
+        PT_REQUIRE(field->storage == PT_STORE_BF16_BRICK || field->storage == PT_STORE_F16_BRICK ||
+                       field->storage == PT_STORE_F32_BRICK,
+                   "pt_serve_add_field: serving needs a BRICK field");
+        PT_REQUIRE(srv->storage < 0 || srv->storage == field->storage,
+                   "pt_serve_add_field: all fields of a server share one storage kind");
+        PT_REQUIRE(field->ctx == srv->ctx, "pt_serve_add_field: field belongs to another context");
+        srv->storage = field->storage;
+        srv->fields.push_back(field);
+        *field_id = static_cast<int32_t>(srv->fields.size() - 1);
+    });
+}
+
+int pt_serve_submit(pt_serve *srv, const pt_serve_request *req, int64_t *request_id) {
+    return guarded([&] {
+        PT_REQUIRE(srv && req && request_id, "pt_serve_submit: null argument");
+        const pt_serve_request &d = *req;
+        PT_REQUIRE(d.field_id >= 0 && d.field_id < static_cast<int32_t>(srv->fields.size()), "pt_serve_submit: unknown field id");
+        PT_REQUIRE(d.n >= 1 && d.n < (1LL << 31), "pt_serve_submit: particle count out of range");
+        PT_REQUIRE(std::isfinite(d.dt_s) && d.dt_s > 0, "pt_serve_submit: dt must be finite and positive");
+        PT_REQUIRE(std::isfinite(d.mass_kg) && d.mass_kg > 0, "pt_serve_submit: mass must be positive");
+        PT_REQUIRE(std::isfinite(d.charge_C) && d.charge_C != 0, "pt_serve_submit: charge must be non-zero");
+        PT_REQUIRE(d.total_steps >= 1 && d.save_stride >= 1, "pt_serve_submit: total_steps and save_stride must be >= 1");
+        PT_REQUIRE(d.out_kind == PT_OUT_NONE || d.out_kind == PT_OUT_MU || d.out_kind == PT_OUT_PMAG,
+                   "pt_serve_submit: out_kind must be PT_OUT_NONE, PT_OUT_MU or PT_OUT_PMAG");
+        for (int c = 0; c < 3; ++c) PT_REQUIRE(d.x[c], "pt_serve_submit: null position array");
+        if (d.inject_fixed_mu) {
+            PT_REQUIRE(d.phase, "pt_serve_submit: fixed-mu injection needs gyrophases");
+            PT_REQUIRE(d.mu0 >= -1.0 && d.mu0 <= 1.0, "injection_mu0 must be in [-1, 1]");
+            PT_REQUIRE(std::isfinite(d.p_magnitude) && d.p_magnitude > 0, "pt_serve_submit: p_magnitude must be positive");
+        } else {
+            for (int c = 0; c < 3; ++c) PT_REQUIRE(d.p[c], "pt_serve_submit: null momentum array");
+        }
+        const pt_field *f = srv->fields[static_cast<size_t>(d.field_id)];
+        const size_t n = static_cast<size_t>(d.n);
+        SReq r;
+        r.id = srv->next_id++;
+        r.d = d;
+        r.nsave = d.out_kind != PT_OUT_NONE ? (d.total_steps - 1) / d.save_stride + 1 : 0;
+        r.bytes = request_bytes(d, r.nsave);
+        PT_REQUIRE(static_cast<int64_t>(r.bytes) <= srv->budget,
+                   "pt_serve_submit: request needs " + std::to_string(r.bytes) + " bytes, more than the whole budget");
+        r.init.resize(d.inject_fixed_mu ? 4 * n : 6 * n);
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < n; ++i) {
+                const double v = d.x[c][i];
+                PT_REQUIRE(std::isfinite(v) && v >= f->g.lo[c] && v <= f->g.hi[c],
+                           "pt_serve_submit: initial position outside the simulation box");
+                r.init[c * n + i] = v;
+            }
+        if (d.inject_fixed_mu) {
+            std::memcpy(r.init.data() + 3 * n, d.phase, n * sizeof(double));
+        } else {
+            for (int c = 0; c < 3; ++c)
+                for (size_t i = 0; i < n; ++i) {
+                    PT_REQUIRE(std::isfinite(d.p[c][i]), "pt_serve_submit: non-finite momentum");
+                    r.init[(3 + c) * n + i] = d.p[c][i];
+                }
+        }
+        for (int c = 0; c < 3; ++c) {
+            r.d.x[c] = nullptr;
+            r.d.p[c] = nullptr;
+        }
+        r.d.phase = nullptr;
+        *request_id = r.id;
+        srv->queue.push_back(r.id);
+        srv->reqs.emplace(r.id, std::move(r));
+    });
+}
+
+int pt_serve_step(pt_serve *srv, int64_t *n_running, int64_t *n_queued, int64_t *n_done) {
+    return guarded([&] {
+        PT_REQUIRE(srv, "pt_serve_step: null server");
+        PT_HIP_CHECK(hipSetDevice(srv->ctx->device));
+        complete_tick(srv);
+        while (!srv->queue.empty() && static_cast<int32_t>(srv->active.size()) < srv->max_active) {
+            SReq &r = srv->reqs.at(srv->queue.front());
+            if (srv->reserved + static_cast<int64_t>(r.bytes) > srv->budget) break;  // FIFO: wait for memory
+            srv->queue.pop_front();
+            admit(srv, r);
+            if (r.failed) {
+                r.state = PT_REQ_DONE;
+                srv->n_done += 1;
+            } else {
+                srv->active.push_back(r.id);
+            }
+        }
+        enqueue_tick(srv);
+        if (n_running) *n_running = static_cast<int64_t>(srv->active.size());
+        if (n_queued) *n_queued = static_cast<int64_t>(srv->queue.size());
+        if (n_done) *n_done = srv->n_done;
+    });
+}
+
+int pt_serve_drain(pt_serve *srv) {
+    return guarded([&] {
+        PT_REQUIRE(srv, "pt_serve_drain: null server");
+        complete_tick(srv);
+    });
+}
+
+int pt_serve_poll(pt_serve *srv, int64_t request_id, int32_t *state) {
+    return guarded([&] {
+        PT_REQUIRE(srv && state, "pt_serve_poll: null argument");
+        auto it = srv->reqs.find(request_id);
+        PT_REQUIRE(it != srv->reqs.end(), "pt_serve_poll: unknown request id");
+        *state = it->second.state;
+    });
+}
+
+int pt_serve_fetch(pt_serve *srv, int64_t request_id, const void **cache_device, int64_t *nsave, int64_t *n,
+                   int64_t *alive_final) {
+    return guarded([&] {
+        PT_REQUIRE(srv, "pt_serve_fetch: null server");
+        auto it = srv->reqs.find(request_id);
+        PT_REQUIRE(it != srv->reqs.end(), "pt_serve_fetch: unknown request id");
+        SReq &r = it->second;
+        if (r.state != PT_REQ_DONE) throw Error{PT_ERR_STATE, "pt_serve_fetch: request is not done"};
+        if (r.failed) throw Error{PT_ERR_INVALID, "cannot use fixed-mu injection where local B is zero or non-finite"};
+        if (cache_device) *cache_device = r.dev.out;
+        if (nsave) *nsave = r.nsave;
+        if (n) *n = r.d.n;
+        if (alive_final) *alive_final = r.alive_final;
+    });
+}
+
+int pt_serve_copy_cache(pt_serve *srv, int64_t request_id, void *host_dst, int64_t bytes) {
+    return guarded([&] {
+        PT_REQUIRE(srv && host_dst, "pt_serve_copy_cache: null argument");
+        auto it = srv->reqs.find(request_id);
+        PT_REQUIRE(it != srv->reqs.end(), "pt_serve_copy_cache: unknown request id");
+        SReq &r = it->second;
+        if (r.state != PT_REQ_DONE || r.failed || !r.dev.out)
+            throw Error{PT_ERR_STATE, "pt_serve_copy_cache: no completed cache for this request"};
+        const int64_t esz = r.d.out_kind == PT_OUT_PMAG ? 8 : 4;
+        PT_REQUIRE(bytes == r.nsave * r.d.n * esz, "pt_serve_copy_cache: size mismatch");
+        PT_HIP_CHECK(hipMemcpy(host_dst, r.dev.out, static_cast<size_t>(bytes), hipMemcpyDeviceToHost));
+    });
+}
+
+int pt_serve_release(pt_serve *srv, int64_t request_id) {
+    return guarded([&] {
+        PT_REQUIRE(srv, "pt_serve_release: null server");
+        auto it = srv->reqs.find(request_id);
+        PT_REQUIRE(it != srv->reqs.end(), "pt_serve_release: unknown request id");
+        SReq &r = it->second;
+        if (r.state == PT_REQ_RUNNING) throw Error{PT_ERR_STATE, "pt_serve_release: request is still running"};
+        if (r.state == PT_REQ_QUEUED) {
+            srv->queue.erase(std::remove(srv->queue.begin(), srv->queue.end(), request_id), srv->queue.end());
+        }
+        if (r.mem) {
+            PT_HIP_CHECK(hipFreeAsync(r.mem, srv->ctx->stream));
+            srv->ctx->bytes_in_use -= static_cast<int64_t>(r.bytes);
+            srv->reserved -= static_cast<int64_t>(r.bytes);
+            r.mem = nullptr;
+        }
+        srv->reqs.erase(it);
+    });
+}
+
+int pt_serve_stats(pt_serve *srv, int64_t *bytes_reserved, int64_t *particle_steps, double *kernel_ms) {
+    return guarded([&] {
+        PT_REQUIRE(srv, "pt_serve_stats: null server");
+        if (bytes_reserved) *bytes_reserved = srv->reserved;
+        if (particle_steps) *particle_steps = srv->particle_steps;
+        if (kernel_ms) *kernel_ms = srv->kernel_ms;
+    });
+}
+
+}  // extern "C"

@@ -274,9 +274,14 @@ __device__ __forceinline__ double pmag(const float *__restrict__ mom, int64_t s,
     const double x = q[0], y = q[total], z = q[2 * total];
     return sqrt(x * x + y * y + z * z);
 }
+// |p| read from a precomputed FP64 cache [save][total].
+__device__ __forceinline__ double pmag(const double *__restrict__ pm, int64_t s, int64_t total, int64_t col) {
+    return pm[s * total + col];
+}
 
+template <class Src>
 __global__ void __launch_bounds__(kTPB)
-k_dpp(const float *__restrict__ mom, int64_t nsave, int64_t total, int64_t n_sel,
+k_dpp(const Src *__restrict__ mom, int64_t nsave, int64_t total, int64_t n_sel,
       const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags, double p0,
       double *__restrict__ part_d, long long *__restrict__ part_i) {
     __shared__ double sh_d[kTPB / 64];
@@ -672,11 +677,15 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
     });
 }
 
-int pt_dpp_accumulate(pt_ctx *ctx, const float *momenta, int on_device, int64_t nsave,
-                      int64_t total, int64_t n_sel, const int64_t *particle_index,
-                      const int64_t *lag_steps, int64_t nlags, double p0, int64_t *counts,
-                      double *sum_delta_p, double *sum_delta_p2, double *sum_delta_p_norm,
-                      double *sum_delta_p_norm2) {
+}  // extern "C"
+
+namespace pt {
+namespace {
+template <class Src>
+int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave, int64_t total, int64_t n_sel,
+                   const int64_t *particle_index, const int64_t *lag_steps, int64_t nlags, double p0,
+                   int64_t *counts, double *sum_delta_p, double *sum_delta_p2, double *sum_delta_p_norm,
+                   double *sum_delta_p_norm2, int64_t comps) {
     return guarded([&] {
         PT_REQUIRE(ctx && momenta && lag_steps && counts && sum_delta_p && sum_delta_p2 && sum_delta_p_norm &&
                        sum_delta_p_norm2,
@@ -692,7 +701,7 @@ int pt_dpp_accumulate(pt_ctx *ctx, const float *momenta, int on_device, int64_t 
         PT_HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t st = ctx->stream;
         DevBuf hm, hidx, hlag, hout;
-        const float *dm = to_device(momenta, on_device, static_cast<size_t>(nsave * 3 * total), hm, st);
+        const Src *dm = to_device(momenta, on_device, static_cast<size_t>(nsave * comps * total), hm, st);
         const int64_t *didx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
         const int64_t *dlag = to_device(lags.data(), 0, lags.size(), hlag, st);
         const int nbx = static_cast<int>((n_sel + kTPB - 1) / kTPB);
@@ -723,6 +732,29 @@ int pt_dpp_accumulate(pt_ctx *ctx, const float *momenta, int on_device, int64_t 
             sum_delta_p_norm2[li] += hd[li * 4 + 3];
         }
     });
+}
+
+}  // namespace
+}  // namespace pt
+
+extern "C" {
+
+int pt_dpp_accumulate(pt_ctx *ctx, const float *momenta, int on_device, int64_t nsave,
+                      int64_t total, int64_t n_sel, const int64_t *particle_index,
+                      const int64_t *lag_steps, int64_t nlags, double p0, int64_t *counts,
+                      double *sum_delta_p, double *sum_delta_p2, double *sum_delta_p_norm,
+                      double *sum_delta_p_norm2) {
+    return pt::dpp_accumulate(ctx, momenta, on_device, nsave, total, n_sel, particle_index, lag_steps, nlags, p0,
+                              counts, sum_delta_p, sum_delta_p2, sum_delta_p_norm, sum_delta_p_norm2, 3);
+}
+
+int pt_dpp_accumulate_pmag(pt_ctx *ctx, const double *pmag, int on_device, int64_t nsave,
+                           int64_t total, int64_t n_sel, const int64_t *particle_index,
+                           const int64_t *lag_steps, int64_t nlags, double p0, int64_t *counts,
+                           double *sum_delta_p, double *sum_delta_p2, double *sum_delta_p_norm,
+                           double *sum_delta_p_norm2) {
+    return pt::dpp_accumulate(ctx, pmag, on_device, nsave, total, n_sel, particle_index, lag_steps, nlags, p0,
+                              counts, sum_delta_p, sum_delta_p2, sum_delta_p_norm, sum_delta_p_norm2, 1);
 }
 
 int pt_energy_snapshots(pt_ctx *ctx, const float *momenta, int on_device, int64_t nsave,

@@ -1,0 +1,141 @@
+"""GPU tests of the continuous-batching server: a request's results are the
+same alone and in a mix (mu cache bit-identical, D_mumu sums within 1e-12),
+equal the single-campaign production pusher, admission respects the HBM
+budget, and fixed-mu injection starts at mu0."""
+import math
+
+import numpy as np
+import pytest
+
+from particle_transport_amd import fields, serving
+from particle_transport_amd.injection import C_LIGHT, M_P, Q_E, energy_to_gamma
+from particle_transport_amd.runner import Context, Ensemble, Field
+
+pytestmark = pytest.mark.gpu
+
+L_BOX = fields.box_length_m(10.0)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    ctx = Context(0)
+    stats = Context(0)
+    raw = fields.generate_turbulence(33, seed=21, kmax=8, dB_over_B0=1.0)["total"]
+    F = [np.asfortranarray(a) for a in fields.to_si(raw)]
+    fld = Field(ctx, F, L_BOX, "bf16_brick")
+    yield ctx, stats, fld, F
+    fld.close()
+    stats.close()
+    ctx.close()
+
+
+def req(i, n, E, dur_steps, kind, B0, mu0=None, seed=0):
+    gamma = float(energy_to_gamma(E))
+    dt = (2 * math.pi / 100) / (Q_E * B0 / (gamma * M_P))
+    from types import SimpleNamespace
+    return SimpleNamespace(i=i, arrival=0.0, field_id=0, energy_gev=E, n=n, mu0=mu0, out_kind=kind, dt=dt,
+                           total_steps=dur_steps, stride=10, gamma=gamma, seed=seed)
+
+
+def run_alone(ctx, fld, r, steps_per_tick=64):
+    srv = serving.Server(ctx, 1 << 32, steps_per_tick)
+    srv.add_field(fld)
+    rid = serving.submit_request(srv, r, L_BOX)
+    while srv.poll(rid) != serving.DONE:
+        srv.step()
+    out = srv.copy_cache(rid, np.float32 if r.out_kind == serving.OUT_MU else np.float64)
+    _, _, _, alive = srv.fetch(rid)
+    srv.release(rid)
+    srv.close()
+    return out, alive
+
+
+def test_alone_equals_mix_and_single_campaign_pusher(setup):
+    ctx, stats, fld, F = setup
+    B0 = fld.B0
+    A = req(0, 3000, 10.0, 401, serving.OUT_MU, B0, seed=1)
+    B = req(1, 777, 1e4, 251, serving.OUT_PMAG, B0, seed=2)
+    C = req(2, 5000, 3.0, 333, serving.OUT_MU, B0, mu0=0.4, seed=3)
+    alone = {r.i: run_alone(ctx, fld, r) for r in (A, B, C)}
+    srv = serving.Server(ctx, 1 << 32, 64)
+    srv.add_field(fld)
+    ids = {serving.submit_request(srv, A, L_BOX): A}
+    srv.step()
+    srv.step()  # B and C join while A is running
+    ids[serving.submit_request(srv, B, L_BOX)] = B
+    srv.step()
+    ids[serving.submit_request(srv, C, L_BOX)] = C
+    mixed = {}
+    while len(mixed) < 3:
+        srv.step()
+        for rid, r in ids.items():
+            if r.i not in mixed and srv.poll(rid) == serving.DONE:
+                mixed[r.i] = (srv.copy_cache(rid, np.float32 if r.out_kind == serving.OUT_MU else np.float64),
+                              srv.fetch(rid)[3])
+                res_mix = serving.finish_request(srv, stats, rid, r)
+                if r.i == 0:
+                    mix_acc = res_mix
+    srv.close()
+    for i in range(3):
+        a, b = alone[i][0], mixed[i][0]
+        assert a.shape == b.shape
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        assert np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)]), f"request {i} differs alone vs mixed"
+        assert alone[i][1] == mixed[i][1]
+    # D_mumu of request A from its cache: alone vs mixed
+    lags = serving.log_lag_steps(alone[0][0].shape[0], 64)
+    import torch
+    acc_alone = serving.dmumu_on_device(stats, torch.from_numpy(alone[0][0]).cuda().data_ptr(), alone[0][0].shape[0],
+                                        3000, lags, np.linspace(0, 1, 21))
+    assert np.array_equal(acc_alone.dmumu_counts, mix_acc.dmumu_counts)
+    assert np.allclose(acc_alone.dmumu_sum_delta2, mix_acc.dmumu_sum_delta2, rtol=1e-12, atol=0)
+    # the same request through the single-campaign production pusher
+    x, p, _, _ = serving.request_state(A, L_BOX)
+    ens = Ensemble(ctx, fld, x, p, pusher="fast")
+    r = ens.advance(dt=A.dt, first_step=1, n_steps=A.total_steps, total_steps=A.total_steps, stride=A.stride,
+                    out_kind="mu", steps_per_launch=100)
+    ens.close()
+    assert np.array_equal(r.out_a, alone[0][0])
+
+
+def test_budget_admission_fifo(setup):
+    ctx, stats, fld, F = setup
+    B0 = fld.B0
+    A = req(0, 20000, 10.0, 301, serving.OUT_MU, B0, seed=4)
+    B = req(1, 20000, 10.0, 301, serving.OUT_MU, B0, seed=5)
+    one = 9 * 4 * 20000 + 20000 + 31 * 20000 * 4 + 8192  # state + alive + cache (+ alignment slack)
+    srv = serving.Server(ctx, int(1.5 * one), 50)
+    srv.add_field(fld)
+    a = serving.submit_request(srv, A, L_BOX)
+    b = serving.submit_request(srv, B, L_BOX)
+    running, queued, _ = srv.step()
+    assert (running, queued) == (1, 1) and srv.poll(b) == serving.QUEUED
+    while srv.poll(a) != serving.DONE:
+        srv.step()
+    assert srv.poll(b) == serving.QUEUED  # still waiting: A holds the memory until released
+    srv.release(a)
+    running, queued, _ = srv.step()
+    assert (running, queued) == (1, 0) and srv.poll(b) == serving.RUNNING
+    while srv.poll(b) != serving.DONE:
+        srv.step()
+    srv.release(b)
+    assert srv.stats().bytes_reserved == 0
+    srv.close()
+
+
+def test_fixed_mu_injection_starts_at_mu0(setup):
+    ctx, stats, fld, F = setup
+    r = req(0, 4096, 100.0, 11, serving.OUT_MU, fld.B0, mu0=-0.6, seed=9)
+    mu, _ = run_alone(ctx, fld, r)
+    assert np.max(np.abs(mu[0] + 0.6)) <= 2e-5
+
+
+def test_oversized_request_rejected(setup):
+    ctx, stats, fld, F = setup
+    srv = serving.Server(ctx, 1 << 20, 50)
+    srv.add_field(fld)
+    r = req(0, 100000, 10.0, 1001, serving.OUT_MU, fld.B0)
+    from particle_transport_amd._lib import PTError
+    with pytest.raises(PTError):
+        serving.submit_request(srv, r, L_BOX)
+    srv.close()
