@@ -36,6 +36,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from particle_transport_amd.injection import M_P as M_P_, Q_E as Q_E_  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_PUSH = 250.0  # FP32 operations of one production push (interp 42 FMA + Boris + move), see DESIGN.md
@@ -45,23 +47,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_field(torch, n_nodes, dB, kmax, seed):
+def make_fields(n_nodes, dB, kmax, seed, modes=False):
+    """Generate synthetic fields in a child process (torch FFTs on the GPU when
+    torch sees one, numpy otherwise) and memory-map the result.  The child has
+    its own HIP runtime; this process only ever loads libpt_hip.so's."""
+    import subprocess
+    import tempfile
+
     from particle_transport_amd import fields
-    raw = fields.generate_turbulence(n_nodes, dB_over_B0=dB, kmax=kmax, kmin=1.0, seed=seed, device="cuda",
-                                     dtype="float32")["total"]
-    return tuple(a * (fields.UGAUSS_TO_T if i < 3 else fields.KMPS_TO_MPS) for i, a in enumerate(raw))
+    fd, path = tempfile.mkstemp(suffix=".npy", prefix="pt_fields_")
+    os.close(fd)
+    cmd = [sys.executable, "-m", "particle_transport_amd.fields", "--out", path, "--nodes", str(n_nodes),
+           "--seed", str(seed), "--dB", str(dB), "--kmax", str(kmax)] + (["--modes"] if modes else [])
+    subprocess.run(cmd, check=True, cwd=ROOT)
+    out = fields.load_generated(path)
+    os.unlink(path)  # the memory map keeps the data until it is dropped
+    return out
 
 
-def make_particles(torch, n, L, energy_gev, seed):
+def make_particles(n, L, energy_gev, seed):
+    import numpy as np
+
     from particle_transport_amd.injection import C_LIGHT, M_P, energy_to_gamma
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    x = torch.rand((3, n), generator=g, device="cuda", dtype=torch.float64) * L
-    d = torch.randn((3, n), generator=g, device="cuda", dtype=torch.float64)
-    d /= torch.linalg.norm(d, dim=0)
+    rng = np.random.default_rng(seed)
+    x = rng.random((3, n)) * L
+    d = rng.standard_normal((3, n))
+    d /= np.linalg.norm(d, axis=0)
     gamma = float(energy_to_gamma(energy_gev))
     p = d * (gamma * M_P * C_LIGHT * math.sqrt(1 - 1 / gamma ** 2))
-    return x.contiguous(), p.contiguous(), gamma
+    return x, p, gamma
 
 
 def traffic_from_profiles():
@@ -77,23 +91,24 @@ def cpu_baseline(F32, L, x, p, dt, n_cpu, steps_cpu):
 
     import oracle
     n_nodes = F32[0].shape[0]
-    F = [np.asfortranarray(c.double().cpu().numpy()) for c in F32]
+    F = [np.asfortranarray(c, dtype=np.float64) for c in F32]
     g = oracle.make_geom(n_nodes, n_nodes, n_nodes, L)
-    xs = np.ascontiguousarray(x[:, :n_cpu].cpu().numpy())
-    ps = np.ascontiguousarray(p[:, :n_cpu].cpu().numpy())
+    xs = np.ascontiguousarray(x[:, :n_cpu])
+    ps = np.ascontiguousarray(p[:, :n_cpu])
     t0 = time.perf_counter()
     oracle.run(F, g, xs, ps, dt, 1, steps_cpu, steps_cpu + 1, 20, True, 2)
     el = time.perf_counter() - t0
     return n_cpu * steps_cpu / el, el
 
 
-def dmumu_check(torch, ctx, F32, L, x, p, dt, n_sub, steps, stride):
+def dmumu_check(ctx, F32, L, x, p, dt, n_sub, steps, stride):
     """D_mumu(|mu|, tau) of the BF16 production pusher vs the FP64 pusher."""
     import numpy as np
 
     from particle_transport_amd import dmumu
+    from particle_transport_amd._lib import DeviceBuffer
     from particle_transport_amd.runner import Ensemble, Field
-    F64 = tuple(c.double() for c in F32)
+    F64 = tuple(np.asfortranarray(c, dtype=np.float64) for c in F32)
     f64 = Field(ctx, F64, L, "f64_planar")
     del F64
     res = {}
@@ -101,7 +116,7 @@ def dmumu_check(torch, ctx, F32, L, x, p, dt, n_sub, steps, stride):
         fld = field or Field(ctx, F32, L, "bf16_brick")
         ens = Ensemble(ctx, fld, x[:, :n_sub], p[:, :n_sub], pusher=pusher)
         nsave = (steps - 1) // stride + 1
-        out = torch.empty((nsave, n_sub), device="cuda", dtype=torch.float32)
+        out = DeviceBuffer(ctx, (nsave, n_sub), np.float32)
         ens.advance(dt=dt, first_step=1, n_steps=steps, total_steps=steps, stride=stride, out_kind="mu", out_a=out,
                     steps_per_launch=200)
         ens.close()
@@ -113,6 +128,7 @@ def dmumu_check(torch, ctx, F32, L, x, p, dt, n_sub, steps, stride):
         D = acc.dmumu_sum_delta2 / np.maximum(acc.dmumu_counts, 1) / (2 * tau[:, None])
         w = acc.dmumu_counts
         res[name] = ((D * w).sum(0) / np.maximum(w.sum(0), 1), w)
+        out.free()
         fld.close()
     a, b = res["bf16"][0], res["fp64"][0]
     ok = res["fp64"][1].sum(0) >= 1000
@@ -120,6 +136,78 @@ def dmumu_check(torch, ctx, F32, L, x, p, dt, n_sub, steps, stride):
     return dict(bins=int(ok.sum()), max_rel_err=float(rel.max()), median_rel_err=float(np.median(rel)),
                 subset=n_sub, pushes=steps - 1, statistic="count-weighted tau-average of D_mumu(|mu|) over lags "
                 "1..128 cadence samples, 20 bins")
+
+
+def serving_run(args, log_fn):
+    """The serving stream on four resident 512^3 BF16 fields: Poisson arrivals
+    at 20/s, per-request field/energy/count/duration/injection/output drawn as
+    the serving configuration specifies, 200 GB budget.  Runs the first
+    args.serving_requests requests of the stream in real time."""
+    from particle_transport_amd import serving
+    from particle_transport_amd.fields import box_length_m
+    from particle_transport_amd.runner import Context, Field
+    L = box_length_m(10.0)
+    push_ctx, stats_ctx = Context(0), Context(0)
+    flds = []
+    for k in range(4):
+        log_fn(f"[serve] field {k}")
+        F = make_fields(args.cells + 1, 1.0, 64.0, seed=3000 + k)[0]
+        flds.append(Field(push_ctx, F, L, "bf16_brick"))
+        del F
+    B0s = [f.B0 for f in flds]
+    reqs = serving.make_stream(args.serving_requests, 4, B0s, rate_hz=20.0, seed=77)
+    res = serving.run_stream(push_ctx, stats_ctx, flds, reqs, budget_bytes=200 * 10 ** 9, steps_per_tick=200,
+                             box_length_m=L, log=log_fn)
+    for f in flds:
+        f.close()
+    stats_ctx.close()
+    push_ctx.close()
+    return {"requests": res.requests, "latency_p50_s": res.latency_p50_s, "latency_p99_s": res.latency_p99_s,
+            "particle_steps_per_s": res.particle_steps_per_s, "particle_steps": res.particle_steps,
+            "wall_s": res.wall_s, "stream": "Poisson 20/s, 4 resident BF16 fields, 200 GB budget, "
+            "energy log-U[1 GeV,100 TeV], n log-U[1e4,1e6], duration U[10,500] T_g0, mu+D_mumu(64x20) or D_pp"}
+
+
+def modes_run(args, log_fn):
+    """Mode-decomposed batch: total + Alfven/fast/slow 256^3 fields (BF16),
+    clock from the total field, 6 energies log-spaced 1 GeV..100 TeV, fixed
+    mu0 in {0.1,0.5,0.9} plus isotropic, uneven counts 2^14..2^20 per
+    campaign: 96 campaigns submitted together and packed into the same ticks."""
+    import numpy as np
+
+    from particle_transport_amd import fields, serving
+    from particle_transport_amd.injection import energy_to_gamma
+    from particle_transport_amd.runner import Context, Field
+    L = fields.box_length_m(10.0)
+    ctx, sctx = Context(0), Context(0)
+    gen = make_fields(args.mode_cells + 1, 0.5, 16.0, seed=555, modes=True)
+    flds = [Field(ctx, F, L, "bf16_brick") for F in gen]
+    del gen
+    B0 = flds[0].B0  # shared total-field clock
+    rng = np.random.default_rng(96)
+    from types import SimpleNamespace
+    reqs = []
+    energies = np.geomspace(1.0, 1e5, 6)
+    for fid in range(4):
+        for E in energies:
+            for mu0 in (0.1, 0.5, 0.9, None):
+                gamma = float(energy_to_gamma(E))
+                dt = (2 * math.pi / 200) / (Q_E_ * B0 / (gamma * M_P_))
+                n = int(2 ** rng.uniform(14, 20))
+                reqs.append(SimpleNamespace(i=len(reqs), arrival=0.0, field_id=fid, energy_gev=float(E), n=n,
+                                            mu0=mu0, out_kind=serving.OUT_MU, dt=dt,
+                                            total_steps=args.mode_gp * 200 + 1, stride=20, gamma=gamma,
+                                            seed=int(rng.integers(1 << 31))))
+    res = serving.run_stream(ctx, sctx, flds, reqs, budget_bytes=200 * 10 ** 9, steps_per_tick=200, box_length_m=L,
+                             log=log_fn)
+    for f in flds:
+        f.close()
+    sctx.close()
+    ctx.close()
+    return {"campaigns": res.requests, "particle_steps_per_s": res.particle_steps_per_s, "wall_s": res.wall_s,
+            "latency_p50_s": res.latency_p50_s, "latency_p99_s": res.latency_p99_s,
+            "config": f"4 x {args.mode_cells}^3 BF16 fields (total/alfven/fast/slow), 6 energies x 4 injections, "
+                      f"{args.mode_gp} T_g0 each"}
 
 
 def main():
@@ -136,6 +224,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=100,
+                    help="requests of the serving stream to replay (0 = skip; 2000 = the full stream)")
+    ap.add_argument("--modes", action="store_true", help="also run the 96-campaign mode-decomposed batch")
+    ap.add_argument("--mode-cells", type=int, default=256)
+    ap.add_argument("--mode-gp", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the pusher on a single MI355X "
@@ -143,8 +236,8 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
+    from particle_transport_amd._lib import DeviceBuffer
     from particle_transport_amd.fields import box_length_m
     from particle_transport_amd.injection import M_P, Q_E
     from particle_transport_amd.runner import Context, Ensemble, Field
@@ -154,21 +247,20 @@ def main():
     n_nodes = args.cells + 1
     stride = 20
     S = args.steps_per_gp
-    log(f"[bench] generating {n_nodes}^3-node field on the GPU")
-    F32 = make_field(torch, n_nodes, 1.0, 64.0, seed=2024)
-    torch.cuda.synchronize()
+    log(f"[bench] generating the {n_nodes}^3-node field")
+    F32 = make_fields(n_nodes, 1.0, 64.0, seed=2024)[0]
     ctx = Context(0)
     log("[bench] packing BF16 and FP32 fields")
     fb = Field(ctx, F32, L, "bf16_brick")
     B0 = fb.B0
-    x, p, gamma = make_particles(torch, n, L, args.energy_gev, seed=99)
+    x, p, gamma = make_particles(n, L, args.energy_gev, seed=99)
     Omega0 = Q_E * B0 / (gamma * M_P)
     dt = (2 * math.pi / S) / Omega0
     total = 500 * S + 1
 
     ens = Ensemble(ctx, fb, x, p, pusher="fast")
     nsave = S // stride
-    mu_buf = torch.empty((nsave, n), device="cuda", dtype=torch.float32)
+    mu_buf = DeviceBuffer(ctx, (nsave, n), np.float32)
     si = 1
 
     def bench_step():
@@ -181,7 +273,6 @@ def main():
     log(f"[bench] warmup {args.warmup} x {S} pushes of {n} particles")
     for _ in range(args.warmup):
         bench_step()
-    torch.cuda.synchronize()
     kms, launches = 0.0, 0
     log(f"[bench] timing {args.steps} steps")
     t0 = time.perf_counter()
@@ -189,9 +280,9 @@ def main():
         r = bench_step()
         kms += r.kernel_ms
         launches += r.launches
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    wall = time.perf_counter() - t0  # each advance returns after its stream has drained
     ens.close()
+    mu_buf.free()
     value = n * S * args.steps / wall
     avg_launch_s = kms / 1e3 / launches
     K = S
@@ -204,7 +295,6 @@ def main():
     fp = Field(ctx, F32, L, "f32_planar")
     ep = Ensemble(ctx, fp, x, p, pusher="plain_f32")
     ep.advance(dt=dt, first_step=1, n_steps=2, total_steps=total, stride=stride)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     rp = ep.advance(dt=dt, first_step=3, n_steps=args.plain_steps, total_steps=total, stride=stride)
     plain_wall = time.perf_counter() - t0
@@ -217,7 +307,7 @@ def main():
     check = None
     if not args.no_check:
         log("[bench] D_mumu check on 65,536 particles")
-        check = dmumu_check(torch, ctx, F32, L, x, p, dt, 65536, 4001, stride)
+        check = dmumu_check(ctx, F32, L, x, p, dt, 65536, 4001, stride)
     fb.close()
 
     cpu = None
@@ -228,6 +318,10 @@ def main():
                "sample": f"{args.cpu_particles} particles x {args.cpu_steps} pushes of the same field and dt, "
                          f"FP64 oracle (C restatement of advance_particles_kernel!), mu every 20 steps, {el:.1f} s"}
     ctx.close()
+    del F32
+
+    serve = serving_run(args, log) if args.serving_requests > 0 else None
+    modes = modes_run(args, log) if args.modes else None
 
     out = {
         "metric": "particle-steps/s (FP32 vs BF16 field) at D_mumu err<=1%; p99 campaign latency",
@@ -245,6 +339,8 @@ def main():
         "plain_f32": {"value": plain_rate, "kernel_particle_steps_per_s": plain_kernel,
                       "speedup_of_bf16_path": value / plain_rate},
         "dmumu_check": check,
+        "serving": serve,
+        "modes_batch": modes,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

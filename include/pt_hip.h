@@ -87,6 +87,14 @@ int pt_ctx_destroy(pt_ctx *ctx);
 /* Device bytes currently held by the context's fields and ensembles. */
 int pt_ctx_bytes_in_use(const pt_ctx *ctx, int64_t *bytes);
 
+/* Device buffers owned by the context (cadence caches that stay in HBM and
+ * feed the statistics entry points with on_device = 1).  No reference
+ * counterpart: the reference holds these as CuArrays. direction: 0 host to
+ * device, 1 device to host, 2 device to device. */
+int pt_buffer_alloc(pt_ctx *ctx, int64_t bytes, void **dev_ptr);
+int pt_buffer_free(pt_ctx *ctx, void *dev_ptr, int64_t bytes);
+int pt_memcpy(pt_ctx *ctx, void *dst, const void *src, int64_t bytes, int direction);
+
 /* ---- fields ------------------------------------------------------------- */
 
 /* Replaces the CuArray uploads of the six field components

@@ -25,7 +25,11 @@ EXPORTS = (
     "pt_abi_version", "pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_ctx_bytes_in_use",
     "pt_field_create", "pt_field_destroy", "pt_field_mean_abs_b", "pt_mean_abs_b",
     "pt_ens_create", "pt_ens_destroy", "pt_ens_advance", "pt_ens_get_state", "pt_ens_last_timing",
-    "pt_mu_from_phase", "pt_dmumu_accumulate", "pt_dpp_accumulate", "pt_energy_snapshots",
+    "pt_mu_from_phase", "pt_dmumu_accumulate", "pt_dpp_accumulate", "pt_dpp_accumulate_pmag",
+    "pt_energy_snapshots", "pt_buffer_alloc", "pt_buffer_free", "pt_memcpy",
+    "pt_serve_create", "pt_serve_destroy", "pt_serve_add_field", "pt_serve_submit", "pt_serve_step",
+    "pt_serve_drain", "pt_serve_poll", "pt_serve_fetch", "pt_serve_copy_cache", "pt_serve_release",
+    "pt_serve_stats",
 )
 
 
@@ -85,6 +89,9 @@ def load():
         "pt_dpp_accumulate": (_int, [_vp, _fp, _int, _i64, _i64, _i64, _i64p, _i64p, _i64, ctypes.c_double,
                                      _i64p, _dp, _dp, _dp, _dp]),
         "pt_energy_snapshots": (_int, [_vp, _fp, _int, _i64, _i64, _i64, _i64p, _i64p, _i64, _dp, _i64]),
+        "pt_buffer_alloc": (_int, [_vp, _i64, ctypes.POINTER(_vp)]),
+        "pt_buffer_free": (_int, [_vp, _vp, _i64]),
+        "pt_memcpy": (_int, [_vp, _vp, _vp, _i64, _int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -106,6 +113,47 @@ def ptr(a: np.ndarray, t=_vp):
     if t is _vp:
         return ctypes.c_void_p(a.ctypes.data)
     return a.ctypes.data_as(t)
+
+
+class DeviceBuffer:
+    """A device array owned by a context: shape, dtype and a device pointer.
+    Accepted wherever the bindings take device data (on_device = 1)."""
+
+    def __init__(self, ctx, shape, dtype=np.float32):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(load().pt_buffer_alloc(ctx.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(ctx, arr.shape, arr.dtype)
+        b.upload(arr)
+        return b
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes
+        check(load().pt_memcpy(self.ctx.h, ctypes.c_void_p(self.ptr), ptr(arr), self.nbytes, 0))
+
+    def to_host(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(load().pt_memcpy(self.ctx.h, ptr(out), ctypes.c_void_p(self.ptr), self.nbytes, 1))
+        return out
+
+    def row(self, i):
+        """Pointer to row i of a C-ordered buffer (for per-component arrays)."""
+        stride = int(np.prod(self.shape[1:])) * self.dtype.itemsize
+        return ctypes.c_void_p(self.ptr + i * stride)
+
+    def free(self):
+        if self.ptr:
+            check(load().pt_buffer_free(self.ctx.h, ctypes.c_void_p(self.ptr), self.nbytes))
+            self.ptr = None
 
 
 def exported_symbols():

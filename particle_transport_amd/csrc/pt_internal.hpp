@@ -94,12 +94,15 @@ struct pt_ens {
 namespace pt {
 
 // Split-precision particle state (structure of arrays, n elements each):
-// int32 cell[3], float off[3] in [0,1) cell units, float P[3] = p / (M_P c).
+// double P2 = |p|^2 / (M_P c)^2, int32 cell[3], float off[3] in [0,1) cell
+// units, float P[3] = p / (M_P c).  44 bytes per particle.
 struct SplitState {
+    double *P2;
     int32_t *cell[3];
     float *off[3];
     float *P[3];
 };
+constexpr size_t kSplitStateBytes = 44;
 // FP64 reference-structured state: x in metres, p in kg m/s.
 struct RefState {
     double *x[3];
@@ -110,6 +113,8 @@ inline SplitState split_state(pt_ens *e) {
     SplitState s;
     char *b = static_cast<char *>(e->state);
     size_t n = static_cast<size_t>(e->n);
+    s.P2 = reinterpret_cast<double *>(b);
+    b += n * 8;
     for (int c = 0; c < 3; ++c) s.cell[c] = reinterpret_cast<int32_t *>(b + c * n * 4);
     for (int c = 0; c < 3; ++c) s.off[c] = reinterpret_cast<float *>(b + (3 + c) * n * 4);
     for (int c = 0; c < 3; ++c) s.P[c] = reinterpret_cast<float *>(b + (6 + c) * n * 4);

@@ -32,26 +32,50 @@ struct Phys {
 
 // ------------------------------------------------------- shared math -------
 
-__device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, float bx, float by,
+// One Boris push in folded units with the momentum magnitude carried in FP64.
+// U (FP32 vector, units m c) and P2 = |U|^2 (FP64): the two half E-kicks
+// are the only steps that change |p|, so P2 is advanced through them exactly
+// in FP64 (|U+e|^2 = |U|^2 + 2 U.e + e.e), the rotation is applied to the FP32
+// vector, and the result is rescaled to |U| = sqrt(P2).  Without this the FP32
+// rotation's rounding drifts |p| by ~5e-8 per push, systematically.
+__device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, double &P2, float bx, float by,
                                             float bz, float ux, float uy, float uz, float &inv_gn) {
     // E~ = -(u~ x B~)
     const float ex = -(uy * bz - uz * by);
     const float ey = -(uz * bx - ux * bz);
     const float ez = -(ux * by - uy * bx);
+    const double ee = static_cast<double>(ex * ex + ey * ey + ez * ez);
+    const double P2m = P2 + 2.0 * static_cast<double>(Ux * ex + Uy * ey + Uz * ez) + ee;
     const float pmx = Ux + ex, pmy = Uy + ey, pmz = Uz + ez;
-    const float gm = sqrtf(1.0f + (pmx * pmx + pmy * pmy + pmz * pmz));
-    const float r = 1.0f / gm;
+    // 1/gamma- from the hardware reciprocal square root: an error of an ulp
+    // changes the rotation angle by that relative amount and leaves |p| alone.
+    const float r = __builtin_amdgcn_rsqf(static_cast<float>(1.0 + P2m));
     const float tx = bx * r, ty = by * r, tz = bz * r;
     const float t2 = tx * tx + ty * ty + tz * tz;
-    const float f = 2.0f / (1.0f + t2);
+    // s = 2t/(1+t^2): one Newton step on the hardware reciprocal (~0.5 ulp).
+    const float d = 1.0f + t2;
+    const float r0 = __builtin_amdgcn_rcpf(d);
+    const float f = 2.0f * fmaf(r0, fmaf(-d, r0, 1.0f), r0);
     const float sx = tx * f, sy = ty * f, sz = tz * f;
     const float ppx = pmx + (pmy * tz - pmz * ty);
     const float ppy = pmy + (pmz * tx - pmx * tz);
     const float ppz = pmz + (pmx * ty - pmy * tx);
-    Ux = pmx + (ppy * sz - ppz * sy) + ex;
-    Uy = pmy + (ppz * sx - ppx * sz) + ey;
-    Uz = pmz + (ppx * sy - ppy * sx) + ez;
-    inv_gn = 1.0f / sqrtf(1.0f + (Ux * Ux + Uy * Uy + Uz * Uz));
+    const float qx = pmx + (ppy * sz - ppz * sy);
+    const float qy = pmy + (ppz * sx - ppx * sz);
+    const float qz = pmz + (ppx * sy - ppy * sx);
+    const double P2n = P2m + 2.0 * static_cast<double>(qx * ex + qy * ey + qz * ez) + ee;
+    Ux = qx + ex;
+    Uy = qy + ey;
+    Uz = qz + ez;
+    // rescale to sqrt(P2n): the ratio is 1 + O(1e-7), so sqrt(1+d) = 1 + d/2 - d^2/8
+    const float u2 = Ux * Ux + Uy * Uy + Uz * Uz;
+    const float dd = static_cast<float>(P2n - static_cast<double>(u2)) * __builtin_amdgcn_rcpf(u2);
+    const float sc = fmaf(dd, fmaf(-0.125f, dd, 0.5f), 1.0f);
+    Ux *= sc;
+    Uy *= sc;
+    Uz *= sc;
+    P2 = P2n;
+    inv_gn = __builtin_amdgcn_rsqf(static_cast<float>(1.0 + P2n));
 }
 
 // Advance one axis of the split position by du cells; returns false if an
@@ -218,25 +242,52 @@ __device__ __forceinline__ float eval_poly(const float a[8], float x, float y, f
 }
 
 
-// The production step loop of one lane: K steps from si0 with state and the
-// current cell's coefficients in registers.  save(si, alive, cell, off, U, v)
-// is called at every cadence step (before the push, as the reference's
-// *_step copies); die(si) when an open boundary removes the particle.
+// Cadence of one launch: steps si0 .. si0+K-1; a save happens where
+// (si-1) % stride == 0.  phase0 = (si0-1) % stride and slot0 = the index of
+// the first save at or after si0 are computed once on the host, so the loop
+// keeps 32-bit counters instead of dividing 64-bit step indices.
+struct Cadence {
+    int64_t si0, total_steps;
+    int32_t K, stride, phase0, want_save;
+    int64_t slot0;
+};
+
+__host__ inline Cadence make_cadence(int64_t si0, int K, int64_t total_steps, int64_t stride, bool want_save,
+                                     int64_t save_base) {
+    Cadence c;
+    c.si0 = si0;
+    c.total_steps = total_steps;
+    c.K = K;
+    c.stride = static_cast<int32_t>(stride);
+    c.phase0 = static_cast<int32_t>((si0 - 1) % stride);
+    c.want_save = want_save ? 1 : 0;
+    c.slot0 = (si0 - 1 + stride - 1) / stride - save_base;
+    return c;
+}
+
+// The production step loop of one lane: K steps with state and the current
+// cell's coefficients in registers.  save(slot, alive, cell, off, U, v) is
+// called at every cadence step (before the push, as the reference's *_step
+// copies); die(si) when an open boundary removes the particle.
 template <int STORE, class SaveFn, class DieFn>
-__device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[3], bool &alive,
+__device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[3], double &P2, bool &alive,
                                          const FieldView &fv, const SplitGeom &g, const Phys &ph,
-                                         int64_t si0, int K, int64_t total_steps, int64_t stride,
-                                         bool want_save, int periodic, SaveFn save, DieFn die) {
+                                         const Cadence &cad, int periodic, SaveFn save, DieFn die) {
     int32_t cg0 = -1, cg1 = -1, cg2 = -1;
     float a[6][8];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
         for (int q = 0; q < 8; ++q) a[c][q] = 0.0f;
-    for (int k = 0; k < K; ++k) {
-        const int64_t si = si0 + k;
-        const bool do_save = want_save && ((si - 1) % stride) == 0;
-        const bool push = alive && si < total_steps;
+    int32_t phase = cad.phase0;
+    int64_t slot = cad.slot0;
+    // pushes happen while si < total_steps: at most n_push of the K steps
+    const int64_t room = cad.total_steps - cad.si0;
+    const int32_t n_push = room <= 0 ? 0 : (room < cad.K ? static_cast<int32_t>(room) : cad.K);
+    for (int32_t k = 0; k < cad.K; ++k) {
+        const bool do_save = cad.want_save && phase == 0;
+        phase = phase + 1 == cad.stride ? 0 : phase + 1;
+        const bool push = alive && k < n_push;
         if (!do_save && !push) continue;
         float v[6] = {0, 0, 0, 0, 0, 0};
         if (alive) {
@@ -257,16 +308,19 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
 #pragma unroll
             for (int c = 0; c < 6; ++c) v[c] = eval_poly(a[c], tx, ty, tz);
         }
-        if (do_save) save(si, alive, cell, off, U, v);
+        if (do_save) {
+            save(slot, alive, cell, off, U, P2, v);
+            ++slot;
+        }
         if (!push) continue;
         float inv_gn;
-        boris_split(U[0], U[1], U[2], v[0], v[1], v[2], v[3], v[4], v[5], inv_gn);
+        boris_split(U[0], U[1], U[2], P2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn);
 #pragma unroll
         for (int c = 0; c < 3; ++c) move_axis(cell[c], off[c], U[c] * (g.k[c] * inv_gn), g.nc[c], periodic);
         if (!periodic && (outside(cell[0], off[0], g.nc[0]) || outside(cell[1], off[1], g.nc[1]) ||
                           outside(cell[2], off[2], g.nc[2]))) {
             alive = false;
-            die(si);
+            die(cad.si0 + k);
         }
     }
 }

@@ -51,6 +51,7 @@ k_push_plain(SplitState s, uint8_t *__restrict__ alive_arr, const float *__restr
         off[c] = s.off[c][p];
         U[c] = s.P[c][p] * ph.mr;
     }
+    double P2 = s.P2[p] * static_cast<double>(ph.mr) * static_cast<double>(ph.mr);
     float v[6] = {0, 0, 0, 0, 0, 0};
     if (alive) {
         int32_t gi, gj, gk;
@@ -83,7 +84,7 @@ k_push_plain(SplitState s, uint8_t *__restrict__ alive_arr, const float *__restr
     }
     if (!push) return;
     float inv_gn;
-    boris_split(U[0], U[1], U[2], v[0], v[1], v[2], v[3], v[4], v[5], inv_gn);
+    boris_split(U[0], U[1], U[2], P2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn);
     for (int c = 0; c < 3; ++c) move_axis(cell[c], off[c], U[c] * (g.k[c] * inv_gn), g.nc[c], periodic);
     if (!periodic && (outside(cell[0], off[0], g.nc[0]) || outside(cell[1], off[1], g.nc[1]) ||
                       outside(cell[2], off[2], g.nc[2]))) {
@@ -96,6 +97,7 @@ k_push_plain(SplitState s, uint8_t *__restrict__ alive_arr, const float *__restr
         s.off[c][p] = off[c];
         s.P[c][p] = U[c] / ph.mr;
     }
+    s.P2[p] = P2 / (static_cast<double>(ph.mr) * static_cast<double>(ph.mr));
 }
 
 // ------------------------------------------------------------ fast ---------
@@ -103,9 +105,8 @@ k_push_plain(SplitState s, uint8_t *__restrict__ alive_arr, const float *__restr
 template <int STORE>
 __global__ void __launch_bounds__(256)
 k_push_fast(SplitState s, uint8_t *__restrict__ alive_arr, FieldView fv, SplitGeom g, Phys ph,
-            int64_t n, int64_t si0, int K, int64_t total_steps, int64_t stride, int64_t save_base,
-            int out_kind, int out_f64, void *out_a, void *out_b, int periodic, int64_t first_step,
-            unsigned long long *deaths) {
+            int64_t n, Cadence cad, int out_kind, int out_f64, void *out_a, void *out_b, int periodic,
+            int64_t first_step, unsigned long long *deaths) {
     const int64_t nblk = gridDim.x;
     const int64_t p = xcd_block(blockIdx.x, nblk) * blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -118,10 +119,11 @@ k_push_fast(SplitState s, uint8_t *__restrict__ alive_arr, FieldView fv, SplitGe
         off[c] = s.off[c][p];
         U[c] = s.P[c][p] * ph.mr;
     }
+    const double mr2 = static_cast<double>(ph.mr) * static_cast<double>(ph.mr);
+    double P2 = s.P2[p] * mr2;
     run_lane<STORE>(
-        cell, off, U, alive, fv, g, ph, si0, K, total_steps, stride, out_kind != PT_OUT_NONE, periodic,
-        [&](int64_t si, bool alv, const int32_t *ce, const float *of, const float *u, const float *v) {
-            const int64_t slot = (si - 1) / stride - save_base;
+        cell, off, U, P2, alive, fv, g, ph, cad, periodic,
+        [&](int64_t slot, bool alv, const int32_t *ce, const float *of, const float *u, double, const float *v) {
             write_save(out_kind, out_f64, out_a, out_b, slot, n, p, alv, ce, of, u, v, g, ph);
         },
         [&](int64_t si) { atomicAdd(&deaths[si - first_step], 1ull); });
@@ -131,6 +133,7 @@ k_push_fast(SplitState s, uint8_t *__restrict__ alive_arr, FieldView fv, SplitGe
         s.off[c][p] = off[c];
         s.P[c][p] = U[c] / ph.mr;
     }
+    s.P2[p] = P2 / mr2;
     alive_arr[p] = alive ? 1 : 0;
 }
 
@@ -190,26 +193,27 @@ int64_t launch_push_fast(pt_ens *e, const pt_field *f, const PushArgs &a) {
         const int k = static_cast<int>(std::min<int64_t>(K, a.n_steps - done));
         auto *dl = reinterpret_cast<unsigned long long *>(e->deaths);
         const int64_t si0 = a.first_step + done;
+        const Cadence cad = make_cadence(si0, k, a.total_steps, a.stride, a.out_kind != PT_OUT_NONE, a.save_base);
         switch (f->storage) {
             case PT_STORE_F32_PLANAR:
                 k_push_fast<PT_STORE_F32_PLANAR><<<blocks, 256, 0, e->ctx->stream>>>(
-                    split_state(e), e->alive, fv, g, ph, e->n, si0, k, a.total_steps, a.stride, a.save_base,
-                    a.out_kind, a.out_f64, a.out_a, a.out_b, e->periodic, a.first_step, dl);
+                    split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
+                    e->periodic, a.first_step, dl);
                 break;
             case PT_STORE_F32_BRICK:
                 k_push_fast<PT_STORE_F32_BRICK><<<blocks, 256, 0, e->ctx->stream>>>(
-                    split_state(e), e->alive, fv, g, ph, e->n, si0, k, a.total_steps, a.stride, a.save_base,
-                    a.out_kind, a.out_f64, a.out_a, a.out_b, e->periodic, a.first_step, dl);
+                    split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
+                    e->periodic, a.first_step, dl);
                 break;
             case PT_STORE_BF16_BRICK:
                 k_push_fast<PT_STORE_BF16_BRICK><<<blocks, 256, 0, e->ctx->stream>>>(
-                    split_state(e), e->alive, fv, g, ph, e->n, si0, k, a.total_steps, a.stride, a.save_base,
-                    a.out_kind, a.out_f64, a.out_a, a.out_b, e->periodic, a.first_step, dl);
+                    split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
+                    e->periodic, a.first_step, dl);
                 break;
             case PT_STORE_F16_BRICK:
                 k_push_fast<PT_STORE_F16_BRICK><<<blocks, 256, 0, e->ctx->stream>>>(
-                    split_state(e), e->alive, fv, g, ph, e->n, si0, k, a.total_steps, a.stride, a.save_base,
-                    a.out_kind, a.out_f64, a.out_a, a.out_b, e->periodic, a.first_step, dl);
+                    split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
+                    e->periodic, a.first_step, dl);
                 break;
             default:
                 throw Error{PT_ERR_INVALID, "PT_PUSH_FAST: unsupported field storage"};

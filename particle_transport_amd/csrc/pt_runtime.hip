@@ -173,6 +173,12 @@ __global__ void k_split_from_si(const double *__restrict__ x, const double *__re
     int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double d[3] = {d0, d1, d2};
+    double p2 = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        const double pc = p[c * n + i] / pnorm;
+        p2 += pc * pc;
+    }
+    s.P2[i] = p2;
     for (int c = 0; c < 3; ++c) {
         double u = x[c * n + i] / d[c];
         double fl = floor(u);
@@ -391,6 +397,35 @@ int pt_ctx_bytes_in_use(const pt_ctx *ctx, int64_t *bytes) {
     });
 }
 
+int pt_buffer_alloc(pt_ctx *ctx, int64_t bytes, void **dev_ptr) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && dev_ptr, "pt_buffer_alloc: null argument");
+        PT_REQUIRE(bytes >= 0, "pt_buffer_alloc: negative size");
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        *dev_ptr = dev_alloc(ctx, static_cast<size_t>(bytes));
+    });
+}
+
+int pt_buffer_free(pt_ctx *ctx, void *dev_ptr, int64_t bytes) {
+    return guarded([&] {
+        PT_REQUIRE(ctx, "pt_buffer_free: null context");
+        PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        dev_free(ctx, dev_ptr, static_cast<size_t>(bytes));
+    });
+}
+
+int pt_memcpy(pt_ctx *ctx, void *dst, const void *src, int64_t bytes, int direction) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && dst && src, "pt_memcpy: null argument");
+        PT_REQUIRE(bytes >= 0, "pt_memcpy: negative size");
+        PT_REQUIRE(direction >= 0 && direction <= 2, "pt_memcpy: direction must be 0, 1 or 2");
+        const hipMemcpyKind k = direction == 0 ? hipMemcpyHostToDevice
+                                : direction == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        PT_HIP_CHECK(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), k, ctx->stream));
+        PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 int pt_field_create(pt_ctx *ctx, int64_t nx, int64_t ny, int64_t nz, double box_length_m,
                     int storage, int src_dtype, int src_on_device, const void *const comps[6],
                     pt_field **out) {
@@ -560,7 +595,7 @@ int pt_ens_create(pt_ctx *ctx, const pt_field *field, int pusher, int64_t n, int
         e->periodic = periodic ? 1 : 0;
         e->n_alive = n;
         try {
-            e->state_bytes = pusher == PT_PUSH_REF_F64 ? 6 * N * 8 : 9 * N * 4;
+            e->state_bytes = pusher == PT_PUSH_REF_F64 ? 6 * N * 8 : kSplitStateBytes * N;
             e->state = dev_alloc(ctx, e->state_bytes);
             e->alive = static_cast<uint8_t *>(dev_alloc(ctx, N));
             PT_HIP_CHECK(hipMemsetAsync(e->alive, 1, N, ctx->stream));

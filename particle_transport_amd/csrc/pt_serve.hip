@@ -33,11 +33,13 @@ struct DevReq {
     int32_t *cell[3];
     float *off[3];
     float *U[3];            // momentum / (m c) of this request's species
+    double *P2;             // |U|^2 in FP64
     uint8_t *alive;
     unsigned long long *deaths;
     int *bad;               // set by the injection kernel on zero/non-finite local B
     void *out;              // cadence cache [save][n]
-    int64_t n, si0, K, total, stride, block0;
+    int64_t n, total, stride, block0;
+    Cadence cad;
     int32_t out_kind, periodic;
     FieldView fv;
     SplitGeom g;
@@ -68,21 +70,18 @@ __global__ void __launch_bounds__(256) k_serve(const DevReq *__restrict__ reqs, 
         off[c] = R.off[c][p];
         U[c] = R.U[c][p];
     }
+    double P2 = R.P2[p];
     const int kind = R.out_kind;
     void *out = R.out;
-    const int64_t stride = R.stride;
     const float qsign = R.ph.qsign;
     const double pout = R.ph.pout;
     run_lane<STORE>(
-        cell, off, U, alive, R.fv, R.g, R.ph, R.si0, static_cast<int>(R.K), R.total, stride, kind != PT_OUT_NONE,
-        R.periodic,
-        [&](int64_t si, bool alv, const int32_t *, const float *, const float *u, const float *v) {
-            const int64_t slot = (si - 1) / stride;
+        cell, off, U, P2, alive, R.fv, R.g, R.ph, R.cad, R.periodic,
+        [&](int64_t slot, bool alv, const int32_t *, const float *, const float *u, double p2, const float *v) {
             if (kind == PT_OUT_MU) {
                 static_cast<float *>(out)[slot * n + p] = alv ? mu_f32(u[0], u[1], u[2], v[0], v[1], v[2], qsign) : NAN;
             } else {
-                const double ux = u[0], uy = u[1], uz = u[2];
-                static_cast<double *>(out)[slot * n + p] = alv ? sqrt(ux * ux + uy * uy + uz * uz) * pout : NAN;
+                static_cast<double *>(out)[slot * n + p] = alv ? sqrt(p2) * pout : NAN;
             }
         },
         [&](int64_t) { atomicAdd(R.deaths, 1ull); });
@@ -92,6 +91,7 @@ __global__ void __launch_bounds__(256) k_serve(const DevReq *__restrict__ reqs, 
         R.off[c][p] = off[c];
         R.U[c][p] = U[c];
     }
+    R.P2[p] = P2;
     R.alive[p] = alive ? 1 : 0;
 }
 
@@ -118,7 +118,13 @@ __global__ void k_serve_init(DevReq R, const double *__restrict__ init, int fixe
         R.off[c][i] = off;
     }
     if (!fixed_mu) {
-        for (int c = 0; c < 3; ++c) R.U[c][i] = static_cast<float>(init[(3 + c) * n + i] * pnorm_inv);
+        double p2 = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const double pc = init[(3 + c) * n + i] * pnorm_inv;
+            p2 += pc * pc;
+            R.U[c][i] = static_cast<float>(pc);
+        }
+        R.P2[i] = p2;
         return;
     }
     int32_t g[3];
@@ -134,6 +140,7 @@ __global__ void k_serve_init(DevReq R, const double *__restrict__ init, int fixe
     if (!isfinite(bn) || bn == 0.0f) {
         atomicExch(R.bad, 1);
         for (int c = 0; c < 3; ++c) R.U[c][i] = 0.0f;
+        R.P2[i] = 0.0;
         return;
     }
     const float bhx = b[0] / bn, bhy = b[1] / bn, bhz = b[2] / bn;
@@ -157,7 +164,9 @@ __global__ void k_serve_init(DevReq R, const double *__restrict__ init, int fixe
     const float phi = static_cast<float>(init[3 * n + i]);
     const float perp = sqrtf(fmaxf(0.0f, 1.0f - mu0 * mu0));
     const float cp = cosf(phi), sp = sinf(phi);
-    const float scale = static_cast<float>(p_magnitude * pnorm_inv);
+    const double pm = p_magnitude * pnorm_inv;
+    R.P2[i] = pm * pm;
+    const float scale = static_cast<float>(pm);
     R.U[0][i] = scale * (mu0 * bhx + perp * (cp * e1x + sp * e2x));
     R.U[1][i] = scale * (mu0 * bhy + perp * (cp * e1y + sp * e2y));
     R.U[2][i] = scale * (mu0 * bhz + perp * (cp * e1z + sp * e2z));
@@ -226,7 +235,7 @@ namespace {
 size_t request_bytes(const pt_serve_request &d, int64_t nsave) {
     const size_t n = static_cast<size_t>(d.n);
     const size_t esz = d.out_kind == PT_OUT_PMAG ? 8 : 4;
-    size_t b = 9 * align_up(n * 4) + align_up(n) + kAlign;
+    size_t b = align_up(n * 8) + 9 * align_up(n * 4) + align_up(n) + kAlign;
     if (d.out_kind != PT_OUT_NONE) b += align_up(static_cast<size_t>(nsave) * n * esz);
     return b;
 }
@@ -236,6 +245,8 @@ void fill_dev(pt_serve *srv, SReq &r) {
     const size_t n = static_cast<size_t>(r.d.n);
     char *m = r.mem;
     DevReq &D = r.dev;
+    D.P2 = reinterpret_cast<double *>(m);
+    m += align_up(n * 8);
     for (int c = 0; c < 3; ++c) {
         D.cell[c] = reinterpret_cast<int32_t *>(m);
         m += align_up(n * 4);
@@ -374,8 +385,8 @@ void enqueue_tick(pt_serve *srv) {
     for (int i = 0; i < nreq; ++i) {
         SReq &r = srv->reqs.at(srv->active[static_cast<size_t>(i)]);
         const int64_t K = std::min<int64_t>(srv->steps_per_tick, r.d.total_steps - r.si + 1);
-        r.dev.si0 = r.si;
-        r.dev.K = K;
+        r.dev.cad = make_cadence(r.si, static_cast<int>(K), r.d.total_steps, r.d.save_stride,
+                                 r.d.out_kind != PT_OUT_NONE, 0);
         r.dev.block0 = blocks;
         blocks += (r.d.n + 255) / 256;
         srv->h_table[i] = r.dev;

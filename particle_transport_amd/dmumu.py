@@ -162,25 +162,20 @@ def process_mu_chunk_dmumu(ctx, mu, lag_steps, mu_edges, acc, start_mode="slidin
                            layout="time_fastest", particle_index=None, n_sel=None):
     """process_mu_chunk_dmumu_cpu! (compute_delta_mu2_dmumu_full.jl:656-704) on the GPU.
 
-    mu: float32 numpy array or torch CUDA tensor.
+    mu: float32 numpy array, or a _lib.DeviceBuffer already in HBM.
       layout "time_fastest": the reference's mu_chunk, numpy shape (chunk, nsave)
         (Julia (nsave, chunk)); column j is particle j.
       layout "particle_fastest": a mu cache (nsave, n) as the pushers write it;
         particle_index (0-based) selects columns, else the first n_sel.
     acc: from new_accumulators, updated in place."""
-    on_dev = 0
-    if type(mu).__module__.startswith("torch"):
-        import torch
-        if mu.dtype != torch.float32:
+    if isinstance(mu, _lib.DeviceBuffer):
+        if mu.dtype != np.float32:
             raise PipelineError("mu must be float32")
-        t = mu.contiguous()
-        rows, cols = t.shape
-        p = ctypes.cast(ctypes.c_void_p(t.data_ptr()), _lib._fp) if t.is_cuda else None
-        if not t.is_cuda:
-            mu = t.numpy()
-        else:
-            on_dev = 1
-    if not on_dev:
+        on_dev = 1
+        rows, cols = mu.shape
+        p = ctypes.cast(ctypes.c_void_p(mu.ptr), _lib._fp)
+    else:
+        on_dev = 0
         mu = np.ascontiguousarray(mu, dtype=np.float32)
         rows, cols = mu.shape
         p = mu.ctypes.data_as(_lib._fp)
@@ -288,8 +283,8 @@ def build_output_dataframe(lag_steps, t_s, t_norm, acc):
 
 def run_combined_from_mu_cache(cfg, ctx, mu_cache, t_s, t_norm, t_gyroperiods=None):
     """run_combined_from_mu_cache (multimode_multienergy_cache_pipeline.jl:1633-1812)
-    over a mu cache shaped (nsave, n) (Julia (n, nsave)), numpy or a torch
-    CUDA tensor already on the device.  Returns the "delta_mu2" and "dmumu"
+    over a mu cache shaped (nsave, n) (Julia (n, nsave)), numpy or a
+    DeviceBuffer already on the device.  Returns the "delta_mu2" and "dmumu"
     product groups of delta_mu2_dmumu_full.h5 as dicts."""
     edges, centers = build_mu_edges_full(cfg)
     start_mode = dmumu_start_mode(cfg)

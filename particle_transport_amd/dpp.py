@@ -49,11 +49,8 @@ def new_accumulators(n_lags: int):
 
 
 def _mom_pointer(momenta):
-    if type(momenta).__module__.startswith("torch"):
-        t = momenta.contiguous()
-        if t.is_cuda:
-            return t, ctypes.cast(ctypes.c_void_p(t.data_ptr()), _lib._fp), 1, tuple(t.shape)
-        momenta = t.numpy()
+    if isinstance(momenta, _lib.DeviceBuffer):
+        return momenta, ctypes.cast(ctypes.c_void_p(momenta.ptr), _lib._fp), 1, momenta.shape
     a = np.ascontiguousarray(momenta, dtype=np.float32)
     return a, a.ctypes.data_as(_lib._fp), 0, a.shape
 
@@ -137,7 +134,10 @@ def run_dpp_full(cfg, ctx, momenta, t_s, t_norm, t_gyroperiods=None):
     lag_steps = grid.lag_steps
     tau_s = np.array([t_s[l] - t_s[0] for l in lag_steps])
     tau_norm = grid.tau_norm
-    m0 = np.asarray(momenta[0, :, int(pidx[0]) - 1].cpu() if hasattr(momenta, "cpu") else momenta[0, :, pidx[0] - 1])
+    if isinstance(momenta, _lib.DeviceBuffer):
+        m0 = momenta.to_host()[0, :, int(pidx[0]) - 1]
+    else:
+        m0 = np.asarray(momenta[0, :, int(pidx[0]) - 1])
     p0 = momentum_magnitude(*m0)
     if not (math.isfinite(p0) and p0 > 0.0):
         raise PipelineError("Cannot compute D_pp normalization: first selected particle has invalid initial momentum.")

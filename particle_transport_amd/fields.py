@@ -223,3 +223,59 @@ def to_si(fields, velocity_unit_in_m_per_s=KMPS_TO_MPS):
     for i, a in enumerate(fields):
         out.append(a * (UGAUSS_TO_T if i < 3 else velocity_unit_in_m_per_s))
     return tuple(out)
+
+
+def generate_to_file(path, n_nodes, *, seed, dB_over_B0=1.0, kmin=1.0, kmax=64.0, B0_uG=5.0, u_rms_kms=20.0,
+                     mode_decompose=False, device="auto"):
+    """Write the synthetic fields in SI units (tesla, m/s) to ``path`` as a
+    float32 .npy array shaped (n_fields, 6, nz, ny, nx) (component c of field
+    f is the Julia (nx,ny,nz) array ``a[f, c].transpose(2, 1, 0)``).  Field 0
+    is the total field; with mode_decompose fields 1-3 are Alfven/fast/slow.
+
+    Run this in its own process (``python -m particle_transport_amd.fields``)
+    when torch is used for the FFTs: torch carries its own HIP runtime, which
+    cannot share a process with libpt_hip.so's."""
+    if device == "auto":
+        try:
+            import torch
+            device = "cuda" if torch.cuda.is_available() else None
+        except ImportError:
+            device = None
+    gen = generate_turbulence(n_nodes, B0_uG=B0_uG, dB_over_B0=dB_over_B0, u_rms_kms=u_rms_kms, kmin=kmin,
+                              kmax=kmax, seed=seed, device=device, dtype="float32", mode_decompose=mode_decompose)
+    names = ("total", "alfven", "fast", "slow") if mode_decompose else ("total",)
+    M = n_nodes
+    out = np.lib.format.open_memmap(path, mode="w+", dtype=np.float32, shape=(len(names), 6, M, M, M))
+    for f, name in enumerate(names):
+        for c, a in enumerate(gen[name]):
+            scale = UGAUSS_TO_T if c < 3 else KMPS_TO_MPS
+            if device is None:
+                out[f, c] = (np.asarray(a).transpose(2, 1, 0) * scale).astype(np.float32)
+            else:
+                out[f, c] = (a.permute(2, 1, 0) * scale).float().cpu().numpy()
+    out.flush()
+    del out
+    return path
+
+
+def load_generated(path):
+    """Memory-mapped fields written by generate_to_file: a list (one entry per
+    field) of 6-tuples of (nx,ny,nz) Fortran-ordered float32 views."""
+    a = np.load(path, mmap_mode="r")
+    return [tuple(a[f, c].transpose(2, 1, 0) for c in range(6)) for f in range(a.shape[0])]
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="write synthetic turbulence fields (SI, float32 .npy)")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--nodes", type=int, required=True)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--dB", type=float, default=1.0)
+    ap.add_argument("--kmin", type=float, default=1.0)
+    ap.add_argument("--kmax", type=float, default=64.0)
+    ap.add_argument("--modes", action="store_true")
+    ap.add_argument("--cpu", action="store_true")
+    a = ap.parse_args()
+    generate_to_file(a.out, a.nodes, seed=a.seed, dB_over_B0=a.dB, kmin=a.kmin, kmax=a.kmax, mode_decompose=a.modes,
+                     device=None if a.cpu else "auto")

@@ -31,45 +31,21 @@ from .time_units import (TIME_REFERENCE_MODE, PipelineError, campaign_time_refer
                          resolve_trajectory_time_grid, sampled_step_indices)
 
 
-def _is_torch(a):
-    return type(a).__module__.startswith("torch")
-
-
 def _julia_layout_pointer(a, keep):
-    """Pointer to an (nx,ny,nz) array whose memory has x fastest.
+    """Pointer to a host (nx,ny,nz) array whose memory has x fastest.
     Returns (pointer, dtype_code, on_device, shape)."""
-    if _is_torch(a):
-        import torch
-        t = a.permute(2, 1, 0)
-        if not t.is_contiguous():
-            t = t.contiguous()
-        if t.dtype not in (torch.float32, torch.float64):
-            t = t.to(torch.float32)
-        keep.append(t)
-        shape = tuple(a.shape)
-        if t.is_cuda:
-            return ctypes.c_void_p(t.data_ptr()), (PT_F64 if t.dtype == torch.float64 else PT_F32), 1, shape
-        arr = t.numpy()
-    else:
-        arr = np.asarray(a)
-        if arr.dtype not in (np.float32, np.float64):
-            arr = arr.astype(np.float64)
-        arr = np.asfortranarray(arr)
-        shape = arr.shape
-        arr = arr.transpose(2, 1, 0)
+    arr = np.asarray(a)
+    if arr.dtype not in (np.float32, np.float64):
+        arr = arr.astype(np.float64)
+    arr = np.asfortranarray(arr)
+    shape = arr.shape
+    arr = arr.transpose(2, 1, 0)
     keep.append(arr)
     assert arr.flags.c_contiguous
     return ptr(arr), (PT_F64 if arr.dtype == np.float64 else PT_F32), 0, shape
 
 
 def _vec_pointer(v, keep, dtype=None):
-    if _is_torch(v):
-        t = v.contiguous()
-        keep.append(t)
-        if t.is_cuda:
-            import torch
-            return ctypes.c_void_p(t.data_ptr()), (PT_F64 if t.dtype == torch.float64 else PT_F32), 1
-        v = t.numpy()
     a = np.ascontiguousarray(v, dtype=dtype or (np.float64 if np.asarray(v).dtype == np.float64 else np.float32))
     keep.append(a)
     return ptr(a), (PT_F64 if a.dtype == np.float64 else PT_F32), 0
@@ -176,7 +152,7 @@ class Ensemble:
                 out_capacity=None):
         """pt_ens_advance.  With out_a/out_b None, host arrays are allocated:
         phase space -> (nsave, 3, n) each (Julia (n,3,nsave)); mu -> (nsave, n).
-        Torch CUDA tensors may be passed as out_a/out_b (device output)."""
+        _lib.DeviceBuffer outputs keep the cadence samples in HBM."""
         field = field or self.field
         kind = _lib.OUT[out_kind]
         nsv = _saves_in_range(first_step, n_steps, stride) if kind else 0
@@ -190,10 +166,10 @@ class Ensemble:
                 cap = nsv
             else:
                 cap = out_capacity if out_capacity is not None else nsv
-            if _is_torch(out_a) and out_a.is_cuda:
+            if isinstance(out_a, _lib.DeviceBuffer):
                 on_dev = 1
-                pa = ctypes.c_void_p(out_a.data_ptr())
-                pb = ctypes.c_void_p(out_b.data_ptr()) if out_b is not None else None
+                pa = ctypes.c_void_p(out_a.ptr)
+                pb = ctypes.c_void_p(out_b.ptr) if out_b is not None else None
             else:
                 pa, pb = ptr(out_a), (ptr(out_b) if out_b is not None else None)
         else:

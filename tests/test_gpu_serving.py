@@ -84,9 +84,10 @@ def test_alone_equals_mix_and_single_campaign_pusher(setup):
         assert alone[i][1] == mixed[i][1]
     # D_mumu of request A from its cache: alone vs mixed
     lags = serving.log_lag_steps(alone[0][0].shape[0], 64)
-    import torch
-    acc_alone = serving.dmumu_on_device(stats, torch.from_numpy(alone[0][0]).cuda().data_ptr(), alone[0][0].shape[0],
-                                        3000, lags, np.linspace(0, 1, 21))
+    from particle_transport_amd._lib import DeviceBuffer
+    buf = DeviceBuffer.from_host(stats, alone[0][0])
+    acc_alone = serving.dmumu_on_device(stats, buf.ptr, alone[0][0].shape[0], 3000, lags, np.linspace(0, 1, 21))
+    buf.free()
     assert np.array_equal(acc_alone.dmumu_counts, mix_acc.dmumu_counts)
     assert np.allclose(acc_alone.dmumu_sum_delta2, mix_acc.dmumu_sum_delta2, rtol=1e-12, atol=0)
     # the same request through the single-campaign production pusher
