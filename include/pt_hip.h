@@ -141,6 +141,8 @@ typedef struct {
     int64_t save_stride;  /* cadence: save at si with (si-1) % save_stride == 0 (:393-396) */
     int32_t out_kind;     /* PT_OUT_* */
     int32_t steps_per_launch; /* FAST pusher: steps per kernel launch (0 = library default) */
+    int64_t sort_every_steps; /* split pushers: Morton re-sort before the first launch and
+                                 every this many steps (0 = never) */
 } pt_advance_params;
 
 /* Replaces the step loop of run_gpu_ensemble (phase_space_gpu_runner.jl:562-591) and
@@ -164,6 +166,13 @@ int pt_ens_advance(pt_ens *ens, const pt_field *field, const pt_advance_params *
  * final state, exact_final_state_stored=false, phase_space_gpu_runner.jl:436); used
  * by parity tests and checkpointing. */
 int pt_ens_get_state(pt_ens *ens, double *x[3], double *p[3], uint8_t *alive);
+
+/* Reorder the particles of a split-precision ensemble (PLAIN_F32, FAST) by
+ * the Morton key of their 8^3 brick so neighbouring lanes gather neighbouring
+ * field records.  Outputs, pt_ens_get_state and alive counts keep the
+ * original particle order.  No reference counterpart (the reference keeps
+ * injection order). */
+int pt_ens_sort(pt_ens *ens);
 
 /* Kernel time (ms) of the last pt_ens_advance, measured with HIP events on
  * the context's stream around the launches, and the number of launches. */

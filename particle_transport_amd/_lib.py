@@ -24,7 +24,7 @@ OUT = {"none": 0, "phase": 1, "phase-space": 1, "mu": 2}
 EXPORTS = (
     "pt_abi_version", "pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_ctx_bytes_in_use",
     "pt_field_create", "pt_field_destroy", "pt_field_mean_abs_b", "pt_mean_abs_b",
-    "pt_ens_create", "pt_ens_destroy", "pt_ens_advance", "pt_ens_get_state", "pt_ens_last_timing",
+    "pt_ens_create", "pt_ens_destroy", "pt_ens_advance", "pt_ens_get_state", "pt_ens_last_timing", "pt_ens_sort",
     "pt_mu_from_phase", "pt_dmumu_accumulate", "pt_dpp_accumulate", "pt_dpp_accumulate_pmag",
     "pt_energy_snapshots", "pt_buffer_alloc", "pt_buffer_free", "pt_memcpy",
     "pt_serve_create", "pt_serve_destroy", "pt_serve_add_field", "pt_serve_submit", "pt_serve_step",
@@ -43,7 +43,7 @@ class AdvanceParams(ctypes.Structure):
     _fields_ = [("dt_s", ctypes.c_double), ("charge_C", ctypes.c_double), ("mass_kg", ctypes.c_double),
                 ("first_step", ctypes.c_int64), ("n_steps", ctypes.c_int64), ("total_steps", ctypes.c_int64),
                 ("save_stride", ctypes.c_int64), ("out_kind", ctypes.c_int32),
-                ("steps_per_launch", ctypes.c_int32)]
+                ("steps_per_launch", ctypes.c_int32), ("sort_every_steps", ctypes.c_int64)]
 
 
 _lib = None
@@ -83,6 +83,7 @@ def load():
                                   _i64p, _i64p]),
         "pt_ens_get_state": (_int, [_vp, ctypes.POINTER(_dp), ctypes.POINTER(_dp), ctypes.POINTER(ctypes.c_uint8)]),
         "pt_ens_last_timing": (_int, [_vp, _dp, _i64p]),
+        "pt_ens_sort": (_int, [_vp]),
         "pt_mu_from_phase": (_int, [_vp, _vp, _i64, _i64, _int, _int, _vp, _vp, _int, _int, _vp]),
         "pt_dmumu_accumulate": (_int, [_vp, _fp, _int, _i64, _i64, _i64, _i64, _i64p, _i64p, _i64, _dp, _i64,
                                        _int, _int, _i64p, _dp, _dp, _dp, _i64p, _i64p, _dp, _dp]),

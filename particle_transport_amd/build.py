@@ -32,6 +32,7 @@ SOURCES = {
     "pt_push_fast.hip": [],
     "pt_stats.hip": ["-munsafe-fp-atomics"],
     "pt_serve.hip": [],
+    "pt_sort.hip": [],
 }
 
 

@@ -85,6 +85,7 @@ struct pt_ens {
     void *state = nullptr;       // see SplitState / RefState below
     size_t state_bytes = 0;
     uint8_t *alive = nullptr;
+    int32_t *perm = nullptr;     // original index of the particle at each slot (after a sort)
     int64_t *deaths = nullptr;   // device per-step death counters
     int64_t deaths_cap = 0;
     double kernel_ms = 0.0;
@@ -136,6 +137,7 @@ struct PushArgs {
     int out_kind;
     int out_f64;             // output element type: 1 double, 0 float
     void *out_a, *out_b;     // device output buffers [save][3][n] / [save][n]
+    int64_t death_base;      // step whose deaths go to deaths[0] (the call's first step)
     int64_t out_cap;         // save slots in out buffers
     double dt, q, m;
     int steps_per_launch;
@@ -153,10 +155,14 @@ void launch_mean_abs_b(pt_ctx *ctx, int64_t n, int src_f64, const void *bx, cons
 void launch_component_stats(pt_ctx *ctx, int64_t n, int src_f64, const void *src, double *mean,
                             double *maxdev);
 void launch_split_from_si(pt_ens *e, const double *dx, const double *dp);
-void launch_split_to_si(pt_ens *e, double *dx, double *dp);
+void morton_sort(pt_ens *e);  // pt_sort.hip
+void launch_split_to_si(pt_ens *e, double *dx, double *dp, uint8_t *alive_out);
 
 // Device scratch owned by the context, grown on demand (never shrinks).
 void *ctx_scratch(pt_ctx *ctx, size_t bytes);
+// Tracked device allocations (counted in pt_ctx_bytes_in_use).
+void *dev_alloc(pt_ctx *ctx, size_t bytes);
+void dev_free(pt_ctx *ctx, void *p, size_t bytes);
 
 // -------------------------------------------------------- device helpers ---
 __host__ __device__ inline int64_t brick_node_index(const int64_t nb[3], int64_t i, int64_t j,

@@ -44,8 +44,9 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
     const float ex = -(uy * bz - uz * by);
     const float ey = -(uz * bx - ux * bz);
     const float ez = -(ux * by - uy * bx);
-    const double ee = static_cast<double>(ex * ex + ey * ey + ez * ez);
-    const double P2m = P2 + 2.0 * static_cast<double>(Ux * ex + Uy * ey + Uz * ez) + ee;
+    // kick terms are ~1e-5 of |U|^2: FP32 for them, one FP64 add to accumulate
+    const float ee = ex * ex + ey * ey + ez * ez;
+    const double P2m = P2 + static_cast<double>(fmaf(2.0f, Ux * ex + Uy * ey + Uz * ez, ee));
     const float pmx = Ux + ex, pmy = Uy + ey, pmz = Uz + ez;
     // 1/gamma- from the hardware reciprocal square root: an error of an ulp
     // changes the rotation angle by that relative amount and leaves |p| alone.
@@ -63,7 +64,7 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
     const float qx = pmx + (ppy * sz - ppz * sy);
     const float qy = pmy + (ppz * sx - ppx * sz);
     const float qz = pmz + (ppx * sy - ppy * sx);
-    const double P2n = P2m + 2.0 * static_cast<double>(qx * ex + qy * ey + qz * ez) + ee;
+    const double P2n = P2m + static_cast<double>(fmaf(2.0f, qx * ex + qy * ey + qz * ez, ee));
     Ux = qx + ex;
     Uy = qy + ey;
     Uz = qz + ez;

@@ -37,9 +37,10 @@ k_push_plain(SplitState s, uint8_t *__restrict__ alive_arr, const float *__restr
              int64_t nx, int64_t ny, int64_t nz, SplitGeom g, Phys ph, int64_t n, int64_t si,
              int64_t total_steps, int64_t stride, int64_t save_base, int out_kind, int out_f64,
              void *out_a, void *out_b, int periodic, int64_t first_step,
-             unsigned long long *deaths) {
+             unsigned long long *deaths, const int32_t *__restrict__ perm) {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    const int64_t col = perm ? perm[p] : p;
     const bool save = out_kind != PT_OUT_NONE && ((si - 1) % stride) == 0;
     bool alive = alive_arr[p] != 0;
     const bool push = alive && si < total_steps;
@@ -80,7 +81,7 @@ k_push_plain(SplitState s, uint8_t *__restrict__ alive_arr, const float *__restr
     }
     if (save) {
         const int64_t slot = (si - 1) / stride - save_base;
-        write_save(out_kind, out_f64, out_a, out_b, slot, n, p, alive, cell, off, U, v, g, ph);
+        write_save(out_kind, out_f64, out_a, out_b, slot, n, col, alive, cell, off, U, v, g, ph);
     }
     if (!push) return;
     float inv_gn;
@@ -106,10 +107,11 @@ template <int STORE>
 __global__ void __launch_bounds__(256)
 k_push_fast(SplitState s, uint8_t *__restrict__ alive_arr, FieldView fv, SplitGeom g, Phys ph,
             int64_t n, Cadence cad, int out_kind, int out_f64, void *out_a, void *out_b, int periodic,
-            int64_t first_step, unsigned long long *deaths) {
+            int64_t first_step, unsigned long long *deaths, const int32_t *__restrict__ perm) {
     const int64_t nblk = gridDim.x;
     const int64_t p = xcd_block(blockIdx.x, nblk) * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    const int64_t col = perm ? perm[p] : p;
     bool alive = alive_arr[p] != 0;
     int32_t cell[3];
     float off[3], U[3];
@@ -124,7 +126,7 @@ k_push_fast(SplitState s, uint8_t *__restrict__ alive_arr, FieldView fv, SplitGe
     run_lane<STORE>(
         cell, off, U, P2, alive, fv, g, ph, cad, periodic,
         [&](int64_t slot, bool alv, const int32_t *ce, const float *of, const float *u, double, const float *v) {
-            write_save(out_kind, out_f64, out_a, out_b, slot, n, p, alv, ce, of, u, v, g, ph);
+            write_save(out_kind, out_f64, out_a, out_b, slot, n, col, alv, ce, of, u, v, g, ph);
         },
         [&](int64_t si) { atomicAdd(&deaths[si - first_step], 1ull); });
 #pragma unroll
@@ -165,7 +167,7 @@ int64_t launch_push_plain_f32(pt_ens *e, const pt_field *f, const PushArgs &a) {
         k_push_plain<<<blocks, 256, 0, e->ctx->stream>>>(
             split_state(e), e->alive, static_cast<const float *>(f->data), f->g.n[0], f->g.n[1], f->g.n[2],
             g, ph, e->n, a.first_step + k, a.total_steps, a.stride, a.save_base, a.out_kind, a.out_f64,
-            a.out_a, a.out_b, e->periodic, a.first_step, reinterpret_cast<unsigned long long *>(e->deaths));
+            a.out_a, a.out_b, e->periodic, a.death_base, reinterpret_cast<unsigned long long *>(e->deaths), e->perm);
         PT_HIP_CHECK(hipGetLastError());
         ++launches;
     }
@@ -198,22 +200,22 @@ int64_t launch_push_fast(pt_ens *e, const pt_field *f, const PushArgs &a) {
             case PT_STORE_F32_PLANAR:
                 k_push_fast<PT_STORE_F32_PLANAR><<<blocks, 256, 0, e->ctx->stream>>>(
                     split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
-                    e->periodic, a.first_step, dl);
+                    e->periodic, a.death_base, dl, e->perm);
                 break;
             case PT_STORE_F32_BRICK:
                 k_push_fast<PT_STORE_F32_BRICK><<<blocks, 256, 0, e->ctx->stream>>>(
                     split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
-                    e->periodic, a.first_step, dl);
+                    e->periodic, a.death_base, dl, e->perm);
                 break;
             case PT_STORE_BF16_BRICK:
                 k_push_fast<PT_STORE_BF16_BRICK><<<blocks, 256, 0, e->ctx->stream>>>(
                     split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
-                    e->periodic, a.first_step, dl);
+                    e->periodic, a.death_base, dl, e->perm);
                 break;
             case PT_STORE_F16_BRICK:
                 k_push_fast<PT_STORE_F16_BRICK><<<blocks, 256, 0, e->ctx->stream>>>(
                     split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
-                    e->periodic, a.first_step, dl);
+                    e->periodic, a.death_base, dl, e->perm);
                 break;
             default:
                 throw Error{PT_ERR_INVALID, "PT_PUSH_FAST: unsupported field storage"};

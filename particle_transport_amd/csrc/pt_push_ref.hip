@@ -213,7 +213,7 @@ int64_t launch_push_ref_f64(pt_ens *e, const pt_field *f, const PushArgs &a) {
         k_push_ref_f64<<<blocks, 256, 0, e->ctx->stream>>>(
             ref_state(e), e->alive, static_cast<const double *>(f->data), g, e->n, a.first_step + done, k,
             a.total_steps, a.stride, a.save_base, a.out_kind, a.out_f64, a.out_a, a.out_b, a.dt, a.q, a.m,
-            e->periodic, a.first_step, reinterpret_cast<unsigned long long *>(e->deaths));
+            e->periodic, a.death_base, reinterpret_cast<unsigned long long *>(e->deaths));
         PT_HIP_CHECK(hipGetLastError());
         ++launches;
     }

@@ -31,7 +31,7 @@ static int guarded(F &&fn) {
     }
 }
 
-static void *dev_alloc(pt_ctx *ctx, size_t bytes) {
+void *dev_alloc(pt_ctx *ctx, size_t bytes) {
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
     if (e != hipSuccess) {
@@ -42,7 +42,7 @@ static void *dev_alloc(pt_ctx *ctx, size_t bytes) {
     ctx->bytes_in_use += static_cast<int64_t>(bytes);
     return p;
 }
-static void dev_free(pt_ctx *ctx, void *p, size_t bytes) {
+void dev_free(pt_ctx *ctx, void *p, size_t bytes) {
     if (p) {
         (void)hipFree(p);
         ctx->bytes_in_use -= static_cast<int64_t>(bytes);
@@ -194,16 +194,18 @@ __global__ void k_split_from_si(const double *__restrict__ x, const double *__re
     }
 }
 
-__global__ void k_split_to_si(SplitState s, int64_t n, double *__restrict__ x,
-                              double *__restrict__ p, double d0, double d1, double d2,
-                              double pnorm) {
+__global__ void k_split_to_si(SplitState s, const int32_t *__restrict__ perm, const uint8_t *__restrict__ alive,
+                              uint8_t *__restrict__ alive_out, int64_t n, double *__restrict__ x,
+                              double *__restrict__ p, double d0, double d1, double d2, double pnorm) {
     int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int64_t o = perm ? perm[i] : i;  // original particle index of slot i
     double d[3] = {d0, d1, d2};
     for (int c = 0; c < 3; ++c) {
-        x[c * n + i] = (static_cast<double>(s.cell[c][i]) + static_cast<double>(s.off[c][i])) * d[c];
-        p[c * n + i] = static_cast<double>(s.P[c][i]) * pnorm;
+        x[c * n + o] = (static_cast<double>(s.cell[c][i]) + static_cast<double>(s.off[c][i])) * d[c];
+        p[c * n + o] = static_cast<double>(s.P[c][i]) * pnorm;
     }
+    alive_out[o] = alive[i];
 }
 
 static inline unsigned grid_for(int64_t n, int block = 256) {
@@ -321,11 +323,11 @@ void launch_split_from_si(pt_ens *e, const double *dx, const double *dp) {
     PT_HIP_CHECK(hipGetLastError());
 }
 
-void launch_split_to_si(pt_ens *e, double *dx, double *dp) {
+void launch_split_to_si(pt_ens *e, double *dx, double *dp, uint8_t *alive_out) {
     SplitState s = split_state(e);
     const Geom &g = e->field->g;
-    k_split_to_si<<<grid_for(e->n), 256, 0, e->ctx->stream>>>(s, e->n, dx, dp, g.d[0], g.d[1], g.d[2],
-                                                             kMp * kC);
+    k_split_to_si<<<grid_for(e->n), 256, 0, e->ctx->stream>>>(s, e->perm, e->alive, alive_out, e->n, dx, dp, g.d[0],
+                                                             g.d[1], g.d[2], kMp * kC);
     PT_HIP_CHECK(hipGetLastError());
 }
 
@@ -625,6 +627,7 @@ int pt_ens_destroy(pt_ens *ens) {
         dev_free(ctx, ens->state, ens->state_bytes);
         dev_free(ctx, ens->alive, static_cast<size_t>(ens->n));
         dev_free(ctx, ens->deaths, static_cast<size_t>(ens->deaths_cap) * 8);
+        if (ens->perm) dev_free(ctx, ens->perm, static_cast<size_t>(ens->n) * 4);
         delete ens;
     });
 }
@@ -704,21 +707,33 @@ int pt_ens_advance(pt_ens *ens, const pt_field *field, const pt_advance_params *
         a.q = P.charge_C;
         a.m = P.mass_kg;
         a.steps_per_launch = P.steps_per_launch;
+        a.death_base = P.first_step;
+        PT_REQUIRE(P.sort_every_steps >= 0, "pt_ens_advance: sort_every_steps must be >= 0");
+        PT_REQUIRE(P.sort_every_steps == 0 || ens->pusher != PT_PUSH_REF_F64,
+                   "pt_ens_advance: the FP64 reference pusher does not re-sort");
 
-        PT_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+        // Segments between Morton re-sorts; the timing covers the push launches only.
+        const int64_t seg = P.sort_every_steps > 0 ? P.sort_every_steps : std::max<int64_t>(P.n_steps, 1);
         int64_t launches = 0;
-        if (P.n_steps > 0) {
+        double ms_total = 0.0;
+        for (int64_t done = 0; done < P.n_steps; done += seg) {
+            if (P.sort_every_steps > 0) morton_sort(ens);
+            PushArgs s = a;
+            s.first_step = P.first_step + done;
+            s.n_steps = std::min(seg, P.n_steps - done);
+            PT_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
             switch (ens->pusher) {
-                case PT_PUSH_REF_F64: launches = launch_push_ref_f64(ens, field, a); break;
-                case PT_PUSH_PLAIN_F32: launches = launch_push_plain_f32(ens, field, a); break;
-                case PT_PUSH_FAST: launches = launch_push_fast(ens, field, a); break;
+                case PT_PUSH_REF_F64: launches += launch_push_ref_f64(ens, field, s); break;
+                case PT_PUSH_PLAIN_F32: launches += launch_push_plain_f32(ens, field, s); break;
+                case PT_PUSH_FAST: launches += launch_push_fast(ens, field, s); break;
             }
+            PT_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+            PT_HIP_CHECK(hipEventSynchronize(ctx->ev1));
+            float ms = 0.f;
+            PT_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            ms_total += ms;
         }
-        PT_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
-        PT_HIP_CHECK(hipEventSynchronize(ctx->ev1));
-        float ms = 0.f;
-        PT_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        ens->kernel_ms = ms;
+        ens->kernel_ms = ms_total;
         ens->launches = launches;
 
         std::vector<int64_t> deaths(static_cast<size_t>(std::max<int64_t>(P.n_steps, 1)));
@@ -748,17 +763,28 @@ int pt_ens_get_state(pt_ens *ens, double *x[3], double *p[3], uint8_t *alive) {
         std::vector<double> h(6 * N);
         if (ens->pusher == PT_PUSH_REF_F64) {
             PT_HIP_CHECK(hipMemcpyAsync(h.data(), ens->state, 6 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+            if (alive) PT_HIP_CHECK(hipMemcpyAsync(alive, ens->alive, N, hipMemcpyDeviceToHost, ctx->stream));
         } else {
-            double *st = static_cast<double *>(ctx_scratch(ctx, 6 * N * 8));
-            launch_split_to_si(ens, st, st + 3 * N);
-            PT_HIP_CHECK(hipMemcpyAsync(h.data(), st, 6 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+            char *st = static_cast<char *>(ctx_scratch(ctx, 6 * N * 8 + N));
+            auto *sd = reinterpret_cast<double *>(st);
+            auto *sa = reinterpret_cast<uint8_t *>(st + 6 * N * 8);
+            launch_split_to_si(ens, sd, sd + 3 * N, sa);
+            PT_HIP_CHECK(hipMemcpyAsync(h.data(), sd, 6 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+            if (alive) PT_HIP_CHECK(hipMemcpyAsync(alive, sa, N, hipMemcpyDeviceToHost, ctx->stream));
         }
-        if (alive) PT_HIP_CHECK(hipMemcpyAsync(alive, ens->alive, N, hipMemcpyDeviceToHost, ctx->stream));
         PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         for (int c = 0; c < 3; ++c) {
             if (x && x[c]) std::memcpy(x[c], h.data() + c * N, N * 8);
             if (p && p[c]) std::memcpy(p[c], h.data() + (3 + c) * N, N * 8);
         }
+    });
+}
+
+int pt_ens_sort(pt_ens *ens) {
+    return guarded([&] {
+        PT_REQUIRE(ens, "pt_ens_sort: null ensemble");
+        PT_HIP_CHECK(hipSetDevice(ens->ctx->device));
+        morton_sort(ens);
     });
 }
 

@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(256) k_serve(const DevReq *__restrict__ reqs, 
 // fixed pitch cosine mu0 about the local field (fixed_mu_unit_vector,
 // phase_space_gpu_runner.jl:170-194) from the request's gyrophase draws.
 template <int STORE>
-__global__ void k_serve_init(DevReq R, const double *__restrict__ init, int fixed_mu, float mu0, double pnorm_inv,
+__global__ void k_serve_init(DevReq R, const double *__restrict__ init, int fixed_mu, float mu0, double pnorm,
                              double p_magnitude) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t n = R.n;
@@ -120,7 +120,7 @@ __global__ void k_serve_init(DevReq R, const double *__restrict__ init, int fixe
     if (!fixed_mu) {
         double p2 = 0.0;
         for (int c = 0; c < 3; ++c) {
-            const double pc = init[(3 + c) * n + i] * pnorm_inv;
+            const double pc = init[(3 + c) * n + i] / pnorm;  // as pt_ens_create converts
             p2 += pc * pc;
             R.U[c][i] = static_cast<float>(pc);
         }
@@ -164,7 +164,7 @@ __global__ void k_serve_init(DevReq R, const double *__restrict__ init, int fixe
     const float phi = static_cast<float>(init[3 * n + i]);
     const float perp = sqrtf(fmaxf(0.0f, 1.0f - mu0 * mu0));
     const float cp = cosf(phi), sp = sinf(phi);
-    const double pm = p_magnitude * pnorm_inv;
+    const double pm = p_magnitude / pnorm;
     R.P2[i] = pm * pm;
     const float scale = static_cast<float>(pm);
     R.U[0][i] = scale * (mu0 * bhx + perp * (cp * e1x + sp * e2x));
@@ -294,7 +294,7 @@ void launch_init(pt_serve *srv, SReq &r, const double *dinit) {
     const unsigned blocks = static_cast<unsigned>((r.d.n + 255) / 256);
     k_serve_init<STORE><<<blocks, 256, 0, srv->ctx->stream>>>(r.dev, dinit, r.d.inject_fixed_mu,
                                                                static_cast<float>(r.d.mu0),
-                                                               1.0 / (r.d.mass_kg * kC), r.d.p_magnitude);
+                                                               r.d.mass_kg * kC, r.d.p_magnitude);
     PT_HIP_CHECK(hipGetLastError());
 }
 

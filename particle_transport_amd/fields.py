@@ -225,16 +225,8 @@ def to_si(fields, velocity_unit_in_m_per_s=KMPS_TO_MPS):
     return tuple(out)
 
 
-def generate_to_file(path, n_nodes, *, seed, dB_over_B0=1.0, kmin=1.0, kmax=64.0, B0_uG=5.0, u_rms_kms=20.0,
-                     mode_decompose=False, device="auto"):
-    """Write the synthetic fields in SI units (tesla, m/s) to ``path`` as a
-    float32 .npy array shaped (n_fields, 6, nz, ny, nx) (component c of field
-    f is the Julia (nx,ny,nz) array ``a[f, c].transpose(2, 1, 0)``).  Field 0
-    is the total field; with mode_decompose fields 1-3 are Alfven/fast/slow.
-
-    Run this in its own process (``python -m particle_transport_amd.fields``)
-    when torch is used for the FFTs: torch carries its own HIP runtime, which
-    cannot share a process with libpt_hip.so's."""
+def _generate_si(n_nodes, *, seed, dB_over_B0, kmin, kmax, B0_uG, u_rms_kms, mode_decompose, device):
+    """Yield (field index, component index, (nz,ny,nx) float32 numpy array) in SI units."""
     if device == "auto":
         try:
             import torch
@@ -244,31 +236,50 @@ def generate_to_file(path, n_nodes, *, seed, dB_over_B0=1.0, kmin=1.0, kmax=64.0
     gen = generate_turbulence(n_nodes, B0_uG=B0_uG, dB_over_B0=dB_over_B0, u_rms_kms=u_rms_kms, kmin=kmin,
                               kmax=kmax, seed=seed, device=device, dtype="float32", mode_decompose=mode_decompose)
     names = ("total", "alfven", "fast", "slow") if mode_decompose else ("total",)
-    M = n_nodes
-    out = np.lib.format.open_memmap(path, mode="w+", dtype=np.float32, shape=(len(names), 6, M, M, M))
     for f, name in enumerate(names):
         for c, a in enumerate(gen[name]):
             scale = UGAUSS_TO_T if c < 3 else KMPS_TO_MPS
             if device is None:
-                out[f, c] = (np.asarray(a).transpose(2, 1, 0) * scale).astype(np.float32)
+                yield f, c, (np.asarray(a).transpose(2, 1, 0) * scale).astype(np.float32)
             else:
-                out[f, c] = (a.permute(2, 1, 0) * scale).float().cpu().numpy()
-    out.flush()
-    del out
-    return path
+                yield f, c, (a.permute(2, 1, 0) * scale).float().cpu().numpy()
 
 
-def load_generated(path):
-    """Memory-mapped fields written by generate_to_file: a list (one entry per
-    field) of 6-tuples of (nx,ny,nz) Fortran-ordered float32 views."""
-    a = np.load(path, mmap_mode="r")
-    return [tuple(a[f, c].transpose(2, 1, 0) for c in range(6)) for f in range(a.shape[0])]
+def generate_to_stream(stream, n_nodes, **kw):
+    """Write the synthetic fields in SI units (tesla, m/s) to a binary stream:
+    one header line "n_fields n_nodes\n", then float32 arrays (nz,ny,nx) for
+    field 0 components 0..5, field 1 ..., x fastest.  Field 0 is the total
+    field; with mode_decompose fields 1-3 are Alfven/fast/slow.
+
+    Meant for a child process (``python -m particle_transport_amd.fields``)
+    when torch runs the FFTs: torch carries its own HIP runtime, which cannot
+    share a process with libpt_hip.so's."""
+    nf = 4 if kw.get("mode_decompose") else 1
+    stream.write(f"{nf} {n_nodes}\n".encode())
+    for _, _, a in _generate_si(n_nodes, **kw):
+        stream.write(np.ascontiguousarray(a).data)
+    stream.flush()
+
+
+def read_stream(stream):
+    """Read generate_to_stream's output: a list (one entry per field) of
+    6-tuples of (nx,ny,nz) Fortran-ordered float32 arrays."""
+    nf, M = (int(v) for v in stream.readline().split())
+    out = np.empty((nf, 6, M, M, M), dtype=np.float32)
+    view = memoryview(out.reshape(-1)).cast("B")
+    got = 0
+    while got < len(view):
+        k = stream.readinto(view[got:got + (1 << 26)])
+        if not k:
+            raise EOFError("field stream ended early")
+        got += k
+    return [tuple(out[f, c].transpose(2, 1, 0) for c in range(6)) for f in range(nf)]
 
 
 if __name__ == "__main__":
     import argparse
-    ap = argparse.ArgumentParser(description="write synthetic turbulence fields (SI, float32 .npy)")
-    ap.add_argument("--out", required=True)
+    import sys
+    ap = argparse.ArgumentParser(description="write synthetic turbulence fields (SI, float32) to stdout")
     ap.add_argument("--nodes", type=int, required=True)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--dB", type=float, default=1.0)
@@ -277,5 +288,5 @@ if __name__ == "__main__":
     ap.add_argument("--modes", action="store_true")
     ap.add_argument("--cpu", action="store_true")
     a = ap.parse_args()
-    generate_to_file(a.out, a.nodes, seed=a.seed, dB_over_B0=a.dB, kmin=a.kmin, kmax=a.kmax, mode_decompose=a.modes,
-                     device=None if a.cpu else "auto")
+    generate_to_stream(sys.stdout.buffer, a.nodes, seed=a.seed, dB_over_B0=a.dB, kmin=a.kmin, kmax=a.kmax,
+                       B0_uG=5.0, u_rms_kms=20.0, mode_decompose=a.modes, device=None if a.cpu else "auto")

@@ -149,7 +149,7 @@ class Ensemble:
 
     def advance(self, field=None, *, dt, first_step, n_steps, total_steps, stride=1, out_kind="none",
                 out_dtype=np.float32, charge=Q_E, mass=M_P, steps_per_launch=0, out_a=None, out_b=None,
-                out_capacity=None):
+                out_capacity=None, sort_every_steps=0):
         """pt_ens_advance.  With out_a/out_b None, host arrays are allocated:
         phase space -> (nsave, 3, n) each (Julia (n,3,nsave)); mu -> (nsave, n).
         _lib.DeviceBuffer outputs keep the cadence samples in HBM."""
@@ -176,7 +176,8 @@ class Ensemble:
             pa = pb = None
             cap = 0
         prm = _lib.AdvanceParams(float(dt), float(charge), float(mass), int(first_step), int(n_steps),
-                                 int(total_steps), int(stride), int(kind), int(steps_per_launch))
+                                 int(total_steps), int(stride), int(kind), int(steps_per_launch),
+                                 int(sort_every_steps))
         alive = np.zeros(max(int(n_steps), 1), dtype=np.int64)
         nsaved = ctypes.c_int64()
         check(_lib.load().pt_ens_advance(self.h, field.h, ctypes.byref(prm), pa, pb, code, on_dev, int(cap),
@@ -187,6 +188,10 @@ class Ensemble:
         return SimpleNamespace(out_a=out_a if nsv else None, out_b=out_b if nsv else None,
                                alive_counts=alive[:int(n_steps)], n_saved=nsaved.value, kernel_ms=ms.value,
                                launches=nl.value)
+
+    def sort(self):
+        """Morton re-sort of the particles by brick (pt_ens_sort)."""
+        check(_lib.load().pt_ens_sort(self.h))
 
     def state(self):
         """(x (3,n) metres, p (3,n) kg m/s, alive (n,) bool) as float64 host arrays."""
