@@ -94,6 +94,9 @@ int pt_ctx_bytes_in_use(const pt_ctx *ctx, int64_t *bytes);
 int pt_buffer_alloc(pt_ctx *ctx, int64_t bytes, void **dev_ptr);
 int pt_buffer_free(pt_ctx *ctx, void *dev_ptr, int64_t bytes);
 int pt_memcpy(pt_ctx *ctx, void *dst, const void *src, int64_t bytes, int direction);
+/* Page-locked host memory for fast cache downloads (hipHostMalloc). */
+int pt_host_alloc(int64_t bytes, void **host_ptr);
+int pt_host_free(void *host_ptr);
 
 /* ---- fields ------------------------------------------------------------- */
 

@@ -26,7 +26,7 @@ EXPORTS = (
     "pt_field_create", "pt_field_destroy", "pt_field_mean_abs_b", "pt_mean_abs_b",
     "pt_ens_create", "pt_ens_destroy", "pt_ens_advance", "pt_ens_get_state", "pt_ens_last_timing", "pt_ens_sort",
     "pt_mu_from_phase", "pt_dmumu_accumulate", "pt_dpp_accumulate", "pt_dpp_accumulate_pmag",
-    "pt_energy_snapshots", "pt_buffer_alloc", "pt_buffer_free", "pt_memcpy",
+    "pt_energy_snapshots", "pt_buffer_alloc", "pt_buffer_free", "pt_memcpy", "pt_host_alloc", "pt_host_free",
     "pt_serve_create", "pt_serve_destroy", "pt_serve_add_field", "pt_serve_submit", "pt_serve_step",
     "pt_serve_drain", "pt_serve_poll", "pt_serve_fetch", "pt_serve_copy_cache", "pt_serve_release",
     "pt_serve_stats",
@@ -93,6 +93,8 @@ def load():
         "pt_buffer_alloc": (_int, [_vp, _i64, ctypes.POINTER(_vp)]),
         "pt_buffer_free": (_int, [_vp, _vp, _i64]),
         "pt_memcpy": (_int, [_vp, _vp, _vp, _i64, _int]),
+        "pt_host_alloc": (_int, [_i64, ctypes.POINTER(_vp)]),
+        "pt_host_free": (_int, [_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -154,6 +156,26 @@ class DeviceBuffer:
     def free(self):
         if self.ptr:
             check(load().pt_buffer_free(self.ctx.h, ctypes.c_void_p(self.ptr), self.nbytes))
+            self.ptr = None
+
+
+class PinnedArray:
+    """A numpy view of page-locked host memory (pt_host_alloc)."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.dtype = np.dtype(dtype)
+        self.shape = tuple(shape)
+        nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(load().pt_host_alloc(nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+        buf = (ctypes.c_char * max(nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(self.shape))).reshape(self.shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            check(load().pt_host_free(ctypes.c_void_p(self.ptr)))
             self.ptr = None
 
 

@@ -88,6 +88,14 @@ struct pt_ens {
     int32_t *perm = nullptr;     // original index of the particle at each slot (after a sort)
     int64_t *deaths = nullptr;   // device per-step death counters
     int64_t deaths_cap = 0;
+    // Morton re-sort: second copy of state/alive/perm (swapped with the live
+    // one after each sort) and the radix-sort workspace, allocated at the
+    // first sort and kept for the ensemble's lifetime.
+    void *state_alt = nullptr;
+    uint8_t *alive_alt = nullptr;
+    int32_t *perm_alt = nullptr;
+    void *sort_ws = nullptr;
+    size_t sort_ws_bytes = 0;
     double kernel_ms = 0.0;
     int64_t launches = 0;
 };

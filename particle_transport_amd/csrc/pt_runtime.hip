@@ -428,6 +428,19 @@ int pt_memcpy(pt_ctx *ctx, void *dst, const void *src, int64_t bytes, int direct
     });
 }
 
+int pt_host_alloc(int64_t bytes, void **host_ptr) {
+    return guarded([&] {
+        PT_REQUIRE(host_ptr && bytes >= 0, "pt_host_alloc: bad argument");
+        PT_HIP_CHECK(hipHostMalloc(host_ptr, static_cast<size_t>(bytes ? bytes : 1), hipHostMallocDefault));
+    });
+}
+
+int pt_host_free(void *host_ptr) {
+    return guarded([&] {
+        if (host_ptr) PT_HIP_CHECK(hipHostFree(host_ptr));
+    });
+}
+
 int pt_field_create(pt_ctx *ctx, int64_t nx, int64_t ny, int64_t nz, double box_length_m,
                     int storage, int src_dtype, int src_on_device, const void *const comps[6],
                     pt_field **out) {
@@ -628,6 +641,10 @@ int pt_ens_destroy(pt_ens *ens) {
         dev_free(ctx, ens->alive, static_cast<size_t>(ens->n));
         dev_free(ctx, ens->deaths, static_cast<size_t>(ens->deaths_cap) * 8);
         if (ens->perm) dev_free(ctx, ens->perm, static_cast<size_t>(ens->n) * 4);
+        dev_free(ctx, ens->state_alt, ens->state_bytes);
+        dev_free(ctx, ens->alive_alt, static_cast<size_t>(ens->n));
+        dev_free(ctx, ens->perm_alt, ens->perm_alt ? static_cast<size_t>(ens->n) * 4 : 0);
+        dev_free(ctx, ens->sort_ws, ens->sort_ws_bytes);
         delete ens;
     });
 }

@@ -7,6 +7,8 @@
 // particle's original index, which the pushers use as the output column.
 #include <hipcub/hipcub.hpp>
 
+#include <utility>
+
 #include "pt_internal.hpp"
 
 namespace pt {
@@ -62,13 +64,28 @@ void morton_sort(pt_ens *e) {
     hipStream_t st = ctx->stream;
     const int64_t n = e->n;
     const size_t N = static_cast<size_t>(n);
-    // keys/slots in and out, then radix-sort temporary storage
+    // Workspace owned by the ensemble (never the context scratch, which holds
+    // host-bound output staging while a segmented advance is in progress):
+    // keys/slots in and out, then the radix-sort temporary storage.
     size_t temp_bytes = 0;
     PT_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, static_cast<uint32_t *>(nullptr),
                                                     static_cast<uint32_t *>(nullptr), static_cast<int32_t *>(nullptr),
                                                     static_cast<int32_t *>(nullptr), static_cast<int>(n), 0, 32, st));
     const size_t head = ((16 * N + 255) / 256) * 256;
-    char *scr = static_cast<char *>(ctx_scratch(ctx, head + temp_bytes));
+    if (e->sort_ws_bytes < head + temp_bytes) {
+        PT_HIP_CHECK(hipStreamSynchronize(st));
+        dev_free(ctx, e->sort_ws, e->sort_ws_bytes);
+        e->sort_ws = nullptr;
+        e->sort_ws_bytes = 0;
+        e->sort_ws = dev_alloc(ctx, head + temp_bytes);
+        e->sort_ws_bytes = head + temp_bytes;
+    }
+    if (!e->state_alt) {
+        e->state_alt = dev_alloc(ctx, e->state_bytes);
+        e->alive_alt = static_cast<uint8_t *>(dev_alloc(ctx, N));
+        e->perm_alt = static_cast<int32_t *>(dev_alloc(ctx, N * 4));
+    }
+    char *scr = static_cast<char *>(e->sort_ws);
     auto *keys_in = reinterpret_cast<uint32_t *>(scr);
     auto *keys_out = keys_in + N;
     auto *slots_in = reinterpret_cast<int32_t *>(keys_out + N);
@@ -79,22 +96,19 @@ void morton_sort(pt_ens *e) {
     PT_HIP_CHECK(hipGetLastError());
     PT_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(scr + head, temp_bytes, keys_in, keys_out, slots_in, slots_out,
                                                     static_cast<int>(n), 0, 32, st));
-    void *new_state = dev_alloc(ctx, e->state_bytes);
-    auto *new_alive = static_cast<uint8_t *>(dev_alloc(ctx, N));
-    auto *new_perm = static_cast<int32_t *>(dev_alloc(ctx, N * 4));
-    void *old_state = e->state;
-    uint8_t *old_alive = e->alive;
-    int32_t *old_perm = e->perm;
-    e->state = new_state;
+    // Gather into the spare copy, then swap it in.  Stream order makes the
+    // old arrays free for the next sort without a host synchronisation.
+    std::swap(e->state, e->state_alt);
     SplitState dst = split_state(e);
-    k_gather_state<<<blocks, 256, 0, st>>>(src, dst, old_alive, new_alive, old_perm, new_perm, slots_out, n);
+    k_gather_state<<<blocks, 256, 0, st>>>(src, dst, e->alive, e->alive_alt, e->perm, e->perm_alt, slots_out, n);
     PT_HIP_CHECK(hipGetLastError());
-    PT_HIP_CHECK(hipStreamSynchronize(st));
-    dev_free(ctx, old_state, e->state_bytes);
-    dev_free(ctx, old_alive, N);
-    if (old_perm) dev_free(ctx, old_perm, N * 4);
-    e->alive = new_alive;
-    e->perm = new_perm;
+    std::swap(e->alive, e->alive_alt);
+    if (!e->perm) {
+        e->perm = e->perm_alt;
+        e->perm_alt = static_cast<int32_t *>(dev_alloc(ctx, N * 4));
+    } else {
+        std::swap(e->perm, e->perm_alt);
+    }
 }
 
 }  // namespace pt

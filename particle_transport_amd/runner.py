@@ -321,7 +321,17 @@ def _trajectory_setup(cfg, fields, energy_GeV, time_reference, caller):
 
 
 def _run_loop(cfg, fields, setup, out_kind, writer, ctx, field, initial_state, chunk_saves):
-    """Shared step loop of run_gpu_ensemble / run_mu_cache_ensemble."""
+    """Shared step loop of run_gpu_ensemble / run_mu_cache_ensemble.
+
+    Each pt_ens_advance call covers chunk_saves cadence samples and writes them
+    into one of two device buffers; a second context (its own stream) copies
+    the finished buffer into pinned host memory and hands it to the cache
+    writer on a worker thread while the next call pushes into the other
+    buffer, so the download and the file writes overlap the pusher (the
+    reference downloads and writes synchronously at every save step,
+    phase_space_gpu_runner.jl:578-590)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     s = setup
     n = int(cfg["n_particles"])
     if initial_state is None:
@@ -338,32 +348,60 @@ def _run_loop(cfg, fields, setup, out_kind, writer, ctx, field, initial_state, c
     alive_fraction = np.empty(s.nsteps)
     stride = s.st.trajectory_time_stride
     nsave = len(s.save_idx)
+    out_T = writer.dtype
+    shape = (chunk_saves, 3, n) if out_kind == "phase" else (chunk_saves, n)
+    names = ("positions", "momenta") if out_kind == "phase" else ("mu",)
+    dev = [[_lib.DeviceBuffer(ctx, shape, out_T) for _ in names] for _ in range(2)]
+    pinned = [[_lib.PinnedArray(shape, out_T) for _ in names] for _ in range(2)]
+    copy_ctx = Context(ctx.device)
+    pool = ThreadPoolExecutor(max_workers=1)
+    busy = [None, None]
     kernel_ms = 0.0
+
+    def drain(buf, offset, count):
+        for k, name in enumerate(names):
+            nbytes = count * int(np.prod(shape[1:])) * np.dtype(out_T).itemsize
+            check(_lib.load().pt_memcpy(copy_ctx.h, ctypes.c_void_p(pinned[buf][k].ptr),
+                                        ctypes.c_void_p(dev[buf][k].ptr), nbytes, 1))
+            writer[name][offset:offset + count] = pinned[buf][k].array[:count]
+
     try:
         t0 = time.perf_counter()
-        si = 1
-        save_ptr = 0
+        si, save_ptr, it = 1, 0, 0
         progress = int(cfg.get("progress_every", 5000))
         while si <= s.nsteps:
             count = min(stride * chunk_saves, s.nsteps - si + 1)
+            buf = it % 2
+            if busy[buf] is not None:
+                busy[buf].result()  # the copy out of this buffer must finish before it is rewritten
             r = ens.advance(field, dt=float(s.dt), first_step=si, n_steps=count, total_steps=s.nsteps,
-                            stride=stride, out_kind=out_kind, out_dtype=writer.dtype if writer else np.float32,
-                            steps_per_launch=int(cfg.get("steps_per_launch", 0)))
+                            stride=stride, out_kind=out_kind, out_dtype=out_T, out_a=dev[buf][0],
+                            out_b=dev[buf][1] if out_kind == "phase" else None, out_capacity=chunk_saves,
+                            steps_per_launch=int(cfg.get("steps_per_launch", 0)),
+                            sort_every_steps=int(cfg.get("sort_every_steps", 0)))
             alive_fraction[si - 1:si - 1 + count] = r.alive_counts / float(n)
-            if r.n_saved and writer is not None:
-                if out_kind == "phase":
-                    writer["positions"][save_ptr:save_ptr + r.n_saved] = r.out_a
-                    writer["momenta"][save_ptr:save_ptr + r.n_saved] = r.out_b
-                else:
-                    writer["mu"][save_ptr:save_ptr + r.n_saved] = r.out_a
+            if r.n_saved:
+                busy[buf] = pool.submit(drain, buf, save_ptr, r.n_saved)
             save_ptr += r.n_saved
             kernel_ms += r.kernel_ms
             if progress and (si - 1) // progress != (si - 1 + count) // progress:
                 print(f"Energy step {si + count - 1}/{s.nsteps}")
             si += count
+            it += 1
+        for f in busy:
+            if f is not None:
+                f.result()
         elapsed = time.perf_counter() - t0
         assert save_ptr == nsave
     finally:
+        pool.shutdown(wait=True)
+        for group in dev:
+            for d in group:
+                d.free()
+        for group in pinned:
+            for h in group:
+                h.free()
+        copy_ctx.close()
         ens.close()
         if own_field:
             field.close()
