@@ -130,6 +130,19 @@ class DeviceBuffer:
         p = ctypes.c_void_p()
         check(load().pt_buffer_alloc(ctx.h, self.nbytes, ctypes.byref(p)))
         self.ptr = p.value
+        self._owner = True
+
+    def view(self, shape):
+        """A non-owning buffer over the first prod(shape) elements (C order)."""
+        v = object.__new__(DeviceBuffer)
+        v.ctx, v.dtype = self.ctx, self.dtype
+        v.shape = tuple(int(s) for s in shape)
+        v.nbytes = int(np.prod(v.shape)) * v.dtype.itemsize
+        if v.nbytes > self.nbytes:
+            raise ValueError("view larger than the buffer")
+        v.ptr = self.ptr
+        v._owner = False
+        return v
 
     @classmethod
     def from_host(cls, ctx, arr):
@@ -154,9 +167,9 @@ class DeviceBuffer:
         return ctypes.c_void_p(self.ptr + i * stride)
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self._owner:
             check(load().pt_buffer_free(self.ctx.h, ctypes.c_void_p(self.ptr), self.nbytes))
-            self.ptr = None
+        self.ptr = None
 
 
 class PinnedArray:

@@ -6,6 +6,13 @@
 
 #include "pt_internal.hpp"
 
+// Multiply-adds are fused only inside one source expression (and where fmaf
+// is written), never across statements by the backend: the same core then
+// rounds identically in every kernel that inlines it (the single-campaign
+// pusher and the serving kernel give bit-identical trajectories).  In effect
+// for the rest of each translation unit that includes this header.
+#pragma clang fp contract(on)
+
 namespace pt {
 namespace core {
 
@@ -79,11 +86,10 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
     inv_gn = __builtin_amdgcn_rsqf(static_cast<float>(1.0 + P2n));
 }
 
-// Advance one axis of the split position by du cells; returns false if an
-// open boundary kills the particle.
-__device__ __forceinline__ void move_axis(int32_t &cell, float &off, float du, int32_t nc,
+// Advance one axis of the split position by u * step cells (one rounding).
+__device__ __forceinline__ void move_axis(int32_t &cell, float &off, float u, float step, int32_t nc,
                                           bool periodic) {
-    off += du;
+    off = fmaf(u, step, off);
     if (!(off >= 0.0f && off < 1.0f)) {
         const float fl = floorf(off);
         cell += static_cast<int32_t>(fl);
@@ -317,7 +323,7 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
         float inv_gn;
         boris_split(U[0], U[1], U[2], P2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) move_axis(cell[c], off[c], U[c] * (g.k[c] * inv_gn), g.nc[c], periodic);
+        for (int c = 0; c < 3; ++c) move_axis(cell[c], off[c], U[c], g.k[c] * inv_gn, g.nc[c], periodic);
         if (!periodic && (outside(cell[0], off[0], g.nc[0]) || outside(cell[1], off[1], g.nc[1]) ||
                           outside(cell[2], off[2], g.nc[2]))) {
             alive = false;

@@ -281,33 +281,9 @@ def build_output_dataframe(lag_steps, t_s, t_norm, acc):
                 n_pairs_used=acc.pair_counts.copy())
 
 
-def run_combined_from_mu_cache(cfg, ctx, mu_cache, t_s, t_norm, t_gyroperiods=None):
-    """run_combined_from_mu_cache (multimode_multienergy_cache_pipeline.jl:1633-1812)
-    over a mu cache shaped (nsave, n) (Julia (n, nsave)), numpy or a
-    DeviceBuffer already on the device.  Returns the "delta_mu2" and "dmumu"
-    product groups of delta_mu2_dmumu_full.h5 as dicts."""
-    edges, centers = build_mu_edges_full(cfg)
-    start_mode = dmumu_start_mode(cfg)
-    bin_abs = mu_bin_abs(cfg)
-    t_s = np.asarray(t_s, dtype=np.float64)
-    t_norm = np.asarray(t_norm, dtype=np.float64)
-    t_gp = t_gyroperiods_from_axes(t_s, t_norm) if t_gyroperiods is None else np.asarray(t_gyroperiods)
-    nsteps, total = mu_cache.shape
-    validate_time_axes(t_s, t_norm, t_gp, key_name="Mu-cache time axes", require_uniform=True)
-    pidx = build_particle_indices(total, cfg)
-    grid = resolve_lag_grid(cfg, t_gp)
-    lag_steps = grid.lag_steps
-    tau_s = np.array([t_s[l] - t_s[0] for l in lag_steps])
-    tau_norm = grid.tau_norm
-    visits = guard_exact_pair_cost(nsteps, len(pidx), lag_steps, cfg)
-    acc = new_accumulators(len(lag_steps), len(centers))
-    chunk = min(int(cfg.get("particle_chunk_size", len(pidx))), len(pidx))
-    if chunk <= 0:
-        raise PipelineError("particle_chunk_size must be positive.")
-    # Chunks are merged with Chan's formula in order, as the reference merges them.
-    for c0 in range(0, len(pidx), chunk):
-        process_mu_chunk_dmumu(ctx, mu_cache, lag_steps, edges, acc, start_mode, bin_abs, layout="particle_fastest",
-                               particle_index=pidx[c0:c0 + chunk] - 1)
+def _finish_products(cfg, acc, lag_steps, grid, tau_s, tau_norm, t_s, t_norm, edges, centers, visits, pidx):
+    """The "delta_mu2" and "dmumu" groups of delta_mu2_dmumu_full.h5 from the
+    merged accumulators (compute_delta_mu2_dmumu_full.jl:1180-1196)."""
     df = build_output_dataframe(lag_steps, t_s, t_norm, acc)
     arrays = compute_dmumu_arrays_full(acc.dmumu_counts, acc.dmumu_sum_delta, acc.dmumu_sum_delta2, tau_s, tau_norm,
                                        cfg)
@@ -323,3 +299,110 @@ def run_combined_from_mu_cache(cfg, ctx, mu_cache, t_s, t_norm, t_gyroperiods=No
               sum_delta_mu=acc.dmumu_sum_delta, sum_delta_mu2=acc.dmumu_sum_delta2)
     return dict(delta_mu2=df, dmumu=dm, estimated_pair_visits=visits, particle_indices=pidx, lag_grid=grid,
                 accumulators=acc)
+
+
+def _setup_pass(cfg, t_s, t_norm, t_gyroperiods, nsteps, total, key_name):
+    edges, centers = build_mu_edges_full(cfg)
+    t_s = np.asarray(t_s, dtype=np.float64)
+    t_norm = np.asarray(t_norm, dtype=np.float64)
+    t_gp = t_gyroperiods_from_axes(t_s, t_norm) if t_gyroperiods is None else np.asarray(t_gyroperiods)
+    validate_time_axes(t_s, t_norm, t_gp, key_name=key_name, require_uniform=True)
+    pidx = build_particle_indices(total, cfg)
+    if len(pidx) == 0:
+        raise PipelineError("No particles selected.")
+    grid = resolve_lag_grid(cfg, t_gp)
+    lag_steps = grid.lag_steps
+    tau_s = np.array([t_s[l] - t_s[0] for l in lag_steps])
+    visits = guard_exact_pair_cost(nsteps, len(pidx), lag_steps, cfg)
+    chunk = min(int(cfg.get("particle_chunk_size", len(pidx))), len(pidx))
+    if chunk <= 0:
+        raise PipelineError("particle_chunk_size must be positive.")
+    return SimpleNamespace(edges=edges, centers=centers, t_s=t_s, t_norm=t_norm, pidx=pidx, grid=grid,
+                           lag_steps=lag_steps, tau_s=tau_s, tau_norm=grid.tau_norm, visits=visits, chunk=chunk,
+                           acc=new_accumulators(len(lag_steps), len(centers)))
+
+
+def run_combined_from_mu_cache(cfg, ctx, mu_cache, t_s, t_norm, t_gyroperiods=None):
+    """run_combined_from_mu_cache (multimode_multienergy_cache_pipeline.jl:1633-1812)
+    over a mu cache shaped (nsave, n) (Julia (n, nsave)), numpy or a
+    DeviceBuffer already on the device.  Returns the "delta_mu2" and "dmumu"
+    product groups of delta_mu2_dmumu_full.h5 as dicts."""
+    nsteps, total = mu_cache.shape
+    u = _setup_pass(cfg, t_s, t_norm, t_gyroperiods, nsteps, total, "Mu-cache time axes")
+    start_mode, bin_abs = dmumu_start_mode(cfg), mu_bin_abs(cfg)
+    # Chunks are merged with Chan's formula in order, as the reference merges them.
+    for c0 in range(0, len(u.pidx), u.chunk):
+        process_mu_chunk_dmumu(ctx, mu_cache, u.lag_steps, u.edges, u.acc, start_mode, bin_abs,
+                               layout="particle_fastest", particle_index=u.pidx[c0:c0 + u.chunk] - 1)
+    return _finish_products(cfg, u.acc, u.lag_steps, u.grid, u.tau_s, u.tau_norm, u.t_s, u.t_norm, u.edges,
+                            u.centers, u.visits, u.pidx)
+
+
+def reconstruct_mu_chunk(ctx, field, positions, momenta, out=None):
+    """reconstruct_mu_chunk_gpu (compute_delta_mu2_curve.jl:186-225): mu for
+    every (sample, particle) of a phase-space chunk, B trilinear from a planar
+    field in the field's precision.  positions/momenta: numpy (nsave, 3,
+    chunk) (Julia (chunk, 3, nsave)) or DeviceBuffers of that shape.  Writes
+    the reference's mu_chunk, Julia (nsave, chunk) = numpy (chunk, nsave)
+    float32, into `out` (a DeviceBuffer, kept on the device) or returns it as
+    a host array."""
+    keep = []
+
+    def src(a):
+        if isinstance(a, _lib.DeviceBuffer):
+            return ctypes.c_void_p(a.ptr), (_lib.PT_F64 if a.dtype == np.float64 else _lib.PT_F32), 1, a.shape
+        arr = np.ascontiguousarray(a)
+        if arr.dtype not in (np.float32, np.float64):
+            arr = arr.astype(np.float64)
+        keep.append(arr)
+        return ptr(arr), (_lib.PT_F64 if arr.dtype == np.float64 else _lib.PT_F32), 0, arr.shape
+
+    pp, pdt, pdev, pshape = src(positions)
+    mp, mdt, mdev, mshape = src(momenta)
+    if pshape != mshape or len(pshape) != 3 or pshape[1] != 3:
+        raise PipelineError("positions and momenta must both have shape (particles, 3, steps).")
+    if (pdt, pdev) != (mdt, mdev):
+        raise PipelineError("positions and momenta must share dtype and device")
+    nsave, _, chunk = pshape
+    if out is None:
+        host = np.empty((chunk, nsave), dtype=np.float32)
+        dst, on_dev = ptr(host), 0
+    else:
+        if out.dtype != np.float32 or out.shape != (chunk, nsave):
+            raise PipelineError("out must be a float32 DeviceBuffer shaped (chunk, nsave)")
+        host, dst, on_dev = None, ctypes.c_void_p(out.ptr), 1
+    check(_lib.load().pt_mu_from_phase(ctx.h, field.h, chunk, nsave, pdt, pdev, pp, mp, _lib.PT_F32, on_dev, dst))
+    return out if host is None else host
+
+
+def run_combined_full(cfg, ctx, field, positions, momenta, t_s, t_norm, t_gyroperiods=None):
+    """run_combined_full (compute_delta_mu2_dmumu_full.jl:1012-1259) over a
+    phase-space cache: per particle chunk, mu is reconstructed on the device
+    (reconstruct_mu_chunk) into a device buffer that the pair pass reads
+    directly, so mu never travels to the host.  positions/momenta: numpy
+    (nsave, 3, n) as the phase-space cache stores them (Julia (n, 3, nsave)),
+    e.g. memory maps; each chunk's columns are gathered on the host like
+    read_particle_batch (:411-437).  field: a planar Field with B (the
+    reference loads B in compute_precision, Float32 by default).  Returns the
+    product groups like run_combined_from_mu_cache."""
+    nsteps, three, total = tuple(positions.shape)
+    if three != 3 or tuple(momenta.shape) != (nsteps, 3, total):
+        raise PipelineError("positions and momenta must both have shape (particles, 3, steps).")
+    if field.storage not in ("f32_planar", "f64_planar"):
+        raise PipelineError("mu reconstruction reads a planar field (f32_planar or f64_planar)")
+    u = _setup_pass(cfg, t_s, t_norm, t_gyroperiods, nsteps, total, "Trajectory HDF5 time axes")
+    start_mode, bin_abs = dmumu_start_mode(cfg), mu_bin_abs(cfg)
+    T = np.float64 if field.storage == "f64_planar" else np.float32
+    buf = _lib.DeviceBuffer(ctx, (u.chunk, nsteps), np.float32)
+    try:
+        for c0 in range(0, len(u.pidx), u.chunk):
+            cols = u.pidx[c0:c0 + u.chunk] - 1
+            pos = np.ascontiguousarray(np.asarray(positions)[:, :, cols], dtype=T)
+            mom = np.ascontiguousarray(np.asarray(momenta)[:, :, cols], dtype=T)
+            mu = reconstruct_mu_chunk(ctx, field, pos, mom, out=buf.view((len(cols), nsteps)))
+            process_mu_chunk_dmumu(ctx, mu, u.lag_steps, u.edges, u.acc, start_mode, bin_abs,
+                                   layout="time_fastest")
+    finally:
+        buf.free()
+    return _finish_products(cfg, u.acc, u.lag_steps, u.grid, u.tau_s, u.tau_norm, u.t_s, u.t_norm, u.edges,
+                            u.centers, u.visits, u.pidx)

@@ -41,13 +41,15 @@ def oracle_for(cfg, F, res, energy, out_kind):
 
 def test_mu_cache_runner_and_dmumu_product(tmp_path, field17):
     F = field17
-    cfg = base_cfg()
+    # 4.01 T_g0: floor(duration / dt) is 200 well away from an integer edge, so
+    # the last bit of B0 cannot move the step count
+    cfg = base_cfg(trajectory_duration_gyroperiods=4.01)
     res = runner.run_total_field_mu_cache_ensemble(cfg, F, str(tmp_path / "mu"), energy_GeV=10.0, chunk_saves=3)
     s, o = oracle_for(cfg, F, res, 10.0, 2)
     mu = np.asarray(res.cache["mu"])
     assert mu.shape == (len(s.save_idx), 300) == o["mu"].shape
     assert np.max(np.abs(mu - o["mu"].astype(np.float32))) <= 1e-6  # FP64 path, float32 cache
-    assert s.st.trajectory_time_stride == 10 and s.nsteps == 201  # 4 T_g0 at 50 steps/T_g0, save every 0.2 T_g0
+    assert s.st.trajectory_time_stride == 10 and s.nsteps == 201  # 4.01 T_g0 at 50 steps/T_g0, save every 0.2 T_g0
     meta = json.load(open(os.path.join(str(tmp_path / "mu"), "metadata.json")))
     assert meta["cache_mode"] == "mu" and meta["integration_step_count"] == 201
     assert np.allclose(meta["alive_fraction"], 1.0)
@@ -70,7 +72,7 @@ def test_mu_cache_runner_and_dmumu_product(tmp_path, field17):
 
 def test_phase_space_runner_open_boundary_and_dpp(tmp_path, field17):
     F = field17
-    cfg = base_cfg(boundary="open", seed=9, trajectory_duration_gyroperiods=3.0)
+    cfg = base_cfg(boundary="open", seed=9, trajectory_duration_gyroperiods=3.01)
     tr = runner.resolve_total_field_time_reference(dict(cfg, trajectory_mode="total"), F, energy_GeV=1e5)
     res = runner.run_gpu_ensemble(cfg, F, energy_GeV=1e5, time_reference=tr, output_path=str(tmp_path / "ps"),
                                   chunk_saves=4)
@@ -107,3 +109,44 @@ def test_float32_precision_uses_production_pusher(tmp_path, field17):
     mu = np.asarray(res.cache["mu"])
     assert mu.shape == o["mu"].shape
     assert np.max(np.abs(mu - o["mu"])) <= 5e-4  # FP32 fields and arithmetic over 200 pushes
+
+
+def test_phase_space_dmumu_product_matches_oracle(field17):
+    """run_combined_full: mu reconstructed on the device per chunk (FP64 planar
+    field, rounded to the float32 mu_chunk) feeding the pair pass, vs the
+    oracle's reconstruct_mu_chunk + pair accumulation on the same cache."""
+    F = field17
+    L = fields.box_length_m(10.0)
+    g = oracle.make_geom(17, 17, 17, L)
+    rng = np.random.default_rng(17)
+    n = 333
+    x = np.ascontiguousarray(rng.random((3, n)) * L)
+    d = rng.standard_normal((3, n))
+    p = np.ascontiguousarray(d / np.linalg.norm(d, axis=0) * 5.4e-18)  # ~10 GeV protons
+    # test data only: dt = 2e6 s is many gyro-radians per push, so mu decorrelates
+    # quickly and every bin and lag sees varied pairs
+    o = oracle.run(F, g, x, p, 2.0e6, 1, 301, 301, 10, True, 1)
+    pos = o["pos"].astype(np.float32)
+    mom = o["mom"].astype(np.float32)
+    nsave = pos.shape[0]
+    t_s = 2.0e6 * 10 * np.arange(nsave)
+    t_norm = t_s * 1e-7
+    dcfg = dict(mu_min=0.0, mu_max=1.0, n_mu_bins=12, mu_bin_abs=True, dmumu_start_mode="sliding",
+                first_particle=1, n_particles_to_use=None, particle_selection="range", particle_chunk_size=100,
+                lag_mode="uniform_samples", lag_range_policy="fixed", lag_boundary_policy="strict",
+                lag_min_gyroperiods=None, lag_max_gyroperiods=None, min_lag_steps=1, max_lag_steps=None,
+                n_lag_samples=7, min_count_per_cell=5, max_pair_visits_without_allow=5e9, allow_huge=False)
+    with runner.Context(0) as ctx:
+        fld = runner.Field(ctx, F, L, "f64_planar")
+        prod = dmumu.run_combined_full(dcfg, ctx, fld, pos, mom, t_s, t_norm)
+        fld.close()
+    mu_ref = oracle.reconstruct_mu_chunk(pos.astype(np.float64), mom.astype(np.float64), F[:3], g)  # (n, nsave)
+    lags = prod["dmumu"]["lag_steps"]
+    acc = oracle.new_dmumu_accumulators(len(lags), 12)
+    for c0 in range(0, n, 100):
+        oracle.dmumu(mu_ref[c0:c0 + 100].astype(np.float32), lags, prod["dmumu"]["mu_edges"], False, True, acc)
+    assert acc["dmumu_counts"].sum() > 0
+    assert np.array_equal(prod["dmumu"]["count_pairs_full"], acc["dmumu_counts"])
+    assert np.allclose(prod["dmumu"]["sum_delta_mu2"], acc["dmumu_sum_delta2"], rtol=1e-11)
+    assert np.allclose(prod["delta_mu2"]["delta_mu2_particle_mean"], acc["particle_means"], rtol=1e-11)
+    assert np.array_equal(prod["delta_mu2"]["n_pairs_used"], acc["pair_counts"])
