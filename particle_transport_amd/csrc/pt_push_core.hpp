@@ -45,19 +45,20 @@ struct Phys {
 // in FP64 (|U+e|^2 = |U|^2 + 2 U.e + e.e), the rotation is applied to the FP32
 // vector, and the result is rescaled to |U| = sqrt(P2).  Without this the FP32
 // rotation's rounding drifts |p| by ~5e-8 per push, systematically.
-__device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, double &P2, float bx, float by,
-                                            float bz, float ux, float uy, float uz, float &inv_gn) {
+__device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, double &P2, float &onep2, float bx,
+                                            float by, float bz, float ux, float uy, float uz, float &inv_gn) {
     // E~ = -(u~ x B~)
     const float ex = -(uy * bz - uz * by);
     const float ey = -(uz * bx - ux * bz);
     const float ez = -(ux * by - uy * bx);
     // kick terms are ~1e-5 of |U|^2: FP32 for them, one FP64 add to accumulate
     const float ee = ex * ex + ey * ey + ez * ez;
-    const double P2m = P2 + static_cast<double>(fmaf(2.0f, Ux * ex + Uy * ey + Uz * ez, ee));
+    const float k1 = fmaf(2.0f, Ux * ex + Uy * ey + Uz * ez, ee);
     const float pmx = Ux + ex, pmy = Uy + ey, pmz = Uz + ez;
-    // 1/gamma- from the hardware reciprocal square root: an error of an ulp
+    // 1/gamma- from the hardware reciprocal square root of 1 + |U-|^2, with
+    // onep2 = float(1 + P2) carried from the previous push: an error of an ulp
     // changes the rotation angle by that relative amount and leaves |p| alone.
-    const float r = __builtin_amdgcn_rsqf(static_cast<float>(1.0 + P2m));
+    const float r = __builtin_amdgcn_rsqf(onep2 + k1);
     const float tx = bx * r, ty = by * r, tz = bz * r;
     const float t2 = tx * tx + ty * ty + tz * tz;
     // s = 2t/(1+t^2): one Newton step on the hardware reciprocal (~0.5 ulp).
@@ -71,7 +72,8 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
     const float qx = pmx + (ppy * sz - ppz * sy);
     const float qy = pmy + (ppz * sx - ppx * sz);
     const float qz = pmz + (ppx * sy - ppy * sx);
-    const double P2n = P2m + static_cast<double>(fmaf(2.0f, qx * ex + qy * ey + qz * ez, ee));
+    const float k2 = fmaf(2.0f, qx * ex + qy * ey + qz * ez, ee);
+    const double P2n = P2 + static_cast<double>(k1 + k2);
     Ux = qx + ex;
     Uy = qy + ey;
     Uz = qz + ez;
@@ -83,14 +85,17 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
     Uy *= sc;
     Uz *= sc;
     P2 = P2n;
-    inv_gn = __builtin_amdgcn_rsqf(static_cast<float>(1.0 + P2n));
+    onep2 = static_cast<float>(1.0 + P2n);
+    inv_gn = __builtin_amdgcn_rsqf(onep2);
 }
 
-// Advance one axis of the split position by u * step cells (one rounding).
+// Advance one axis of the split position by u * step cells (one rounding);
+// `moved` is set when the cell changes.
 __device__ __forceinline__ void move_axis(int32_t &cell, float &off, float u, float step, int32_t nc,
-                                          bool periodic) {
+                                          bool periodic, bool &moved) {
     off = fmaf(u, step, off);
     if (!(off >= 0.0f && off < 1.0f)) {
+        moved = true;
         const float fl = floorf(off);
         cell += static_cast<int32_t>(fl);
         off -= fl;
@@ -125,6 +130,19 @@ __device__ __forceinline__ void gather_coord(int32_t cell, float off, int32_t nc
     g = g < 0 ? 0 : g;
 }
 
+// Gather cell of one axis for the polynomial path: the coefficients are
+// evaluated at the offset itself, and `bias` (nonzero only at x == L, where
+// the last cell is evaluated at t = 1 + off) is folded into them.
+__device__ __forceinline__ int32_t gather_cell(int32_t cell, int32_t nc, float &bias) {
+    int32_t g = cell;
+    bias = 0.0f;
+    if (g > nc - 1) {
+        bias = static_cast<float>(g - (nc - 1));
+        g = nc - 1;
+    }
+    return g < 0 ? 0 : g;
+}
+
 template <class T>
 __device__ __forceinline__ void put(void *out, int64_t idx, double v) {
     static_cast<T *>(out)[idx] = static_cast<T>(v);
@@ -132,10 +150,11 @@ __device__ __forceinline__ void put(void *out, int64_t idx, double v) {
 
 __device__ __forceinline__ float mu_f32(float px, float py, float pz, float bx, float by, float bz,
                                         float qsign) {
-    const float pn = sqrtf(px * px + py * py + pz * pz);
-    const float bn = sqrtf(bx * bx + by * by + bz * bz);
-    if (!isfinite(pn) || !isfinite(bn) || pn == 0.0f || bn == 0.0f) return NAN;
-    const float mu = qsign * (px * bx + py * by + pz * bz) / (pn * bn);
+    const float p2 = px * px + py * py + pz * pz;
+    const float b2 = bx * bx + by * by + bz * bz;
+    if (!isfinite(p2) || !isfinite(b2) || p2 == 0.0f || b2 == 0.0f) return NAN;
+    // 1/(|p||B|) from the hardware reciprocal square roots (~1 ulp each)
+    const float mu = qsign * (px * bx + py * by + pz * bz) * (__builtin_amdgcn_rsqf(p2) * __builtin_amdgcn_rsqf(b2));
     return fminf(fmaxf(mu, -1.0f), 1.0f);
 }
 
@@ -240,6 +259,22 @@ __device__ __forceinline__ void coeffs(const float s[8], float scale, float offs
     a[7] = ((dx11 - dx01) - (dx10 - dx00)) * scale;
 }
 
+// Coefficients of f(x + bx, y + by, z + bz), one axis at a time.
+__device__ __forceinline__ void shift_coeffs(float a[8], float bx, float by, float bz) {
+    a[0] = fmaf(a[1], bx, a[0]);
+    a[2] = fmaf(a[4], bx, a[2]);
+    a[3] = fmaf(a[5], bx, a[3]);
+    a[6] = fmaf(a[7], bx, a[6]);
+    a[0] = fmaf(a[2], by, a[0]);
+    a[1] = fmaf(a[4], by, a[1]);
+    a[3] = fmaf(a[6], by, a[3]);
+    a[5] = fmaf(a[7], by, a[5]);
+    a[0] = fmaf(a[3], bz, a[0]);
+    a[1] = fmaf(a[5], bz, a[1]);
+    a[2] = fmaf(a[6], bz, a[2]);
+    a[4] = fmaf(a[7], bz, a[4]);
+}
+
 __device__ __forceinline__ float eval_poly(const float a[8], float x, float y, float z) {
     const float z0 = fmaf(a[3], z, a[0]);
     const float z1 = fmaf(a[5], z, a[1]);
@@ -273,19 +308,23 @@ __host__ inline Cadence make_cadence(int64_t si0, int K, int64_t total_steps, in
 }
 
 // The production step loop of one lane: K steps with state and the current
-// cell's coefficients in registers.  save(slot, alive, cell, off, U, v) is
-// called at every cadence step (before the push, as the reference's *_step
-// copies); die(si) when an open boundary removes the particle.
+// cell's coefficients in registers.  The coefficients are reloaded only after
+// the particle changes cell (move_axis reports it), so a push costs the
+// six polynomial evaluations and the Boris step.  save(slot, alive, cell,
+// off, U, P2, v) is called at every cadence step (before the push, as the
+// reference's *_step copies); die(si) when an open boundary removes the
+// particle.
 template <int STORE, class SaveFn, class DieFn>
 __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[3], double &P2, bool &alive,
                                          const FieldView &fv, const SplitGeom &g, const Phys &ph,
                                          const Cadence &cad, int periodic, SaveFn save, DieFn die) {
-    int32_t cg0 = -1, cg1 = -1, cg2 = -1;
     float a[6][8];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
         for (int q = 0; q < 8; ++q) a[c][q] = 0.0f;
+    bool reload = true;
+    float onep2 = static_cast<float>(1.0 + P2);
     int32_t phase = cad.phase0;
     int64_t slot = cad.slot0;
     // pushes happen while si < total_steps: at most n_push of the K steps
@@ -296,34 +335,35 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
         phase = phase + 1 == cad.stride ? 0 : phase + 1;
         const bool push = alive && k < n_push;
         if (!do_save && !push) continue;
-        float v[6] = {0, 0, 0, 0, 0, 0};
-        if (alive) {
-            int32_t gi, gj, gk;
-            float tx, ty, tz;
-            gather_coord(cell[0], off[0], g.nc[0], gi, tx);
-            gather_coord(cell[1], off[1], g.nc[1], gj, ty);
-            gather_coord(cell[2], off[2], g.nc[2], gk, tz);
-            if (gi != cg0 || gj != cg1 || gk != cg2) {
-                float sv[6][8];
-                load_corners<STORE>(fv, gi, gj, gk, sv);
+        if (alive && reload) {
+            float b[3];
+            const int32_t gi = gather_cell(cell[0], g.nc[0], b[0]);
+            const int32_t gj = gather_cell(cell[1], g.nc[1], b[1]);
+            const int32_t gk = gather_cell(cell[2], g.nc[2], b[2]);
+            float sv[6][8];
+            load_corners<STORE>(fv, gi, gj, gk, sv);
 #pragma unroll
-                for (int c = 0; c < 6; ++c) coeffs(sv[c], fv.scale[c] * ph.fac[c], fv.offset[c] * ph.fac[c], a[c]);
-                cg0 = gi;
-                cg1 = gj;
-                cg2 = gk;
+            for (int c = 0; c < 6; ++c) coeffs(sv[c], fv.scale[c] * ph.fac[c], fv.offset[c] * ph.fac[c], a[c]);
+            if (b[0] != 0.0f || b[1] != 0.0f || b[2] != 0.0f) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) shift_coeffs(a[c], b[0], b[1], b[2]);
             }
-#pragma unroll
-            for (int c = 0; c < 6; ++c) v[c] = eval_poly(a[c], tx, ty, tz);
+            reload = false;
         }
+        // dead lanes evaluate stale coefficients only to keep the loop
+        // uniform; save() writes NaN for them
+        float v[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v[c] = eval_poly(a[c], off[0], off[1], off[2]);
         if (do_save) {
             save(slot, alive, cell, off, U, P2, v);
             ++slot;
         }
         if (!push) continue;
         float inv_gn;
-        boris_split(U[0], U[1], U[2], P2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn);
+        boris_split(U[0], U[1], U[2], P2, onep2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) move_axis(cell[c], off[c], U[c], g.k[c] * inv_gn, g.nc[c], periodic);
+        for (int c = 0; c < 3; ++c) move_axis(cell[c], off[c], U[c], g.k[c] * inv_gn, g.nc[c], periodic, reload);
         if (!periodic && (outside(cell[0], off[0], g.nc[0]) || outside(cell[1], off[1], g.nc[1]) ||
                           outside(cell[2], off[2], g.nc[2]))) {
             alive = false;

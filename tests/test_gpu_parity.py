@@ -153,6 +153,45 @@ def test_fast_open_boundary_deaths_exact(ctx):
     fld.close()
 
 
+@pytest.mark.parametrize("pusher,storage", [("fast", "f32_brick"), ("fast", "f16_brick"),
+                                            ("plain_f32", "f32_planar")])
+def test_particles_on_the_upper_faces(ctx, pusher, storage):
+    """x == L (or y, z, or all three) is inside the box and gathers the last
+    cell at t = 1 (trilinear_sample's clamp of i0 to nx-1).  The production
+    pusher evaluates that cell's polynomial with the offset folded into its
+    coefficients: mu of the initial state must match the oracle there, and
+    particles moving outward must die at the first push exactly as in the
+    oracle (open boundary).  After that only particles the oracle leaves
+    within FP32 offset resolution of a face (1e-7 cells) could differ; at
+    1e4 GeV inward movers advance far more than that per push, so the counts
+    and the NaN pattern must be exact."""
+    F, x, p, dt, g = make_case(n=256, energy_gev=1e4)
+    x[0, 0:64] = g.xmax
+    x[1, 64:128] = g.ymax
+    x[2, 128:192] = g.zmax
+    x[:, 192:224] = np.array([[g.xmax], [g.ymax], [g.zmax]])
+    fld = Field(ctx, F, L_BOX, storage)
+    ens = Ensemble(ctx, fld, x, p, pusher=pusher, periodic=False)
+    r = ens.advance(dt=dt, first_step=1, n_steps=3, total_steps=3, stride=1, out_kind="mu")
+    o = oracle.run(F, g, x.copy(), p.copy(), dt, 1, 3, 3, 1, False, 2)
+    ph = oracle.run(F, g, x.copy(), p.copy(), dt, 1, 3, 3, 1, False, 1)
+    tol = 2e-5 if storage != "f16_brick" else 2e-3
+    assert np.max(np.abs(r.out_a[0] - o["mu"][0])) <= tol  # every particle alive at sample 1
+    assert r.alive_counts[0] == o["alive_counts"][0]  # outward movers die at the first push
+    assert o["alive_counts"][0] < 256 - 40
+    faces = np.array([g.xmax, g.ymax, g.zmax])[None, :, None]
+    with np.errstate(invalid="ignore"):
+        gap = np.fmin.reduce((faces - ph["pos"][1:]) / g.dx, axis=1)  # cells inside the faces after each push
+    grazer = np.nan_to_num(gap, nan=np.inf) < 1e-7
+    assert np.max(np.abs(r.alive_counts - o["alive_counts"])) <= grazer.any(axis=0).sum()
+    keep = ~grazer.any(axis=0)
+    assert np.array_equal(np.isnan(r.out_a[:, keep]), np.isnan(o["mu"][:, keep]))
+    live = ~np.isnan(o["mu"]) & ~np.isnan(r.out_a)
+    assert np.max(np.abs(r.out_a[live] - o["mu"][live])) <= tol
+    ens.close()
+    fld.close()
+
+
 def test_ref_f64_open_boundary_alive_counts_exact(ctx):
     F, x, p, dt, g = make_case(n=512, energy_gev=1e5, steps_per_gp=50)
     steps = 400
