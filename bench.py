@@ -222,6 +222,86 @@ def dmumu_check(ctx, F32, L, x, p, dt, n_sub, steps, stride, groups=16):
     return res
 
 
+def stats_rates(ctx, F32, L, x, p, dt, log_fn):
+    """Pair-visit rates of the statistics kernels on series the production
+    pusher writes into HBM (FP16 field): D_mumu over a mu cache of 262,144
+    particles x 1001 samples (128 log-spaced lags, 40 |mu| bins, sliding
+    starts), D_pp over a momenta cache of 131,072 particles x 1001 samples
+    (128 lags).  One pair visit = one (particle, start, lag) triple."""
+    import numpy as np
+
+    from particle_transport_amd import dmumu, dpp, serving
+    from particle_transport_amd._lib import DeviceBuffer
+    from particle_transport_amd.runner import Ensemble, Field
+    fld = Field(ctx, F32, L, "f16_brick")
+    nsave = 1001
+    lags = serving.log_lag_steps(nsave, 128)
+    out = {}
+    n = 262144
+    mu = DeviceBuffer(ctx, (nsave, n), np.float32)
+    ens = Ensemble(ctx, fld, x[:, :n], p[:, :n], pusher="fast")
+    ens.advance(dt=dt, first_step=1, n_steps=nsave, total_steps=nsave, stride=1, out_kind="mu", out_a=mu,
+                steps_per_launch=200)
+    ens.close()
+    edges = np.linspace(0.0, 1.0, 41)
+    for rep in range(2):  # the first call also builds the kernel's device tables
+        acc = dmumu.new_accumulators(len(lags), 40)
+        t0 = time.perf_counter()
+        dmumu.process_mu_chunk_dmumu(ctx, mu, lags, edges, acc, "sliding", True, layout="particle_fastest")
+        el = time.perf_counter() - t0
+    visits = dmumu.estimate_pair_visits(nsave, n, lags)
+    out["dmumu"] = {"pair_visits_per_s": visits / el, "pair_visits": visits, "seconds": el,
+                    "shape": f"{n} particles x {nsave} samples, {len(lags)} lags, 40 bins, sliding"}
+    log_fn(f"[bench] D_mumu: {visits / el:.4g} pair visits/s")
+    mu.free()
+    n = 131072
+    mom = DeviceBuffer(ctx, (nsave, 3, n), np.float32)
+    pos = DeviceBuffer(ctx, (nsave, 3, n), np.float32)
+    ens = Ensemble(ctx, fld, x[:, :n], p[:, :n], pusher="fast")
+    ens.advance(dt=dt, first_step=1, n_steps=nsave, total_steps=nsave, stride=1, out_kind="phase", out_a=pos,
+                out_b=mom, steps_per_launch=200)
+    ens.close()
+    pos.free()
+    p0 = float(np.linalg.norm(p[:, 0]))
+    for rep in range(2):
+        acc = dpp.new_accumulators(len(lags))
+        t0 = time.perf_counter()
+        dpp.process_momenta_chunk_dpp(ctx, mom, lags, p0, acc)
+        el = time.perf_counter() - t0
+    visits = dmumu.estimate_pair_visits(nsave, n, lags)
+    out["dpp"] = {"pair_visits_per_s": visits / el, "pair_visits": visits, "seconds": el,
+                  "shape": f"{n} particles x {nsave} samples, {len(lags)} lags, FP32 momenta -> FP64 |p|"}
+    log_fn(f"[bench] D_pp: {visits / el:.4g} pair visits/s")
+    mom.free()
+    fld.close()
+    return out
+
+
+def pcie_inclusive(F32, L, x, p, energy_gev, log_fn, n=4194304, duration_gp=20.0):
+    """The runner path end to end: run_mu_cache_ensemble with the mu cache in
+    host memory (double-buffered device chunks, pinned staging, writer
+    thread), FP16 field, 200 steps/T_g0, mu every 0.1 T_g0.  Rate = particle
+    pushes / the runner's loop time, downloads included."""
+    from particle_transport_amd import runner
+    cfg = dict(box_length_pc=10.0, boundary="periodic", precision="Float32", integration_steps_per_gyroperiod=200.0,
+               trajectory_duration_gyroperiods=duration_gp, trajectory_save_interval_gyroperiods=0.1, use_cfl=False,
+               cfl=0.2, eta=0.1, tOmega0_max=1.0, trajectory_time_stride=1, seed=1, n_particles=n,
+               trajectory_mode="total", mu_cache_output_precision="Float32", progress_every=0, steps_per_launch=200)
+    with runner.Context(0) as ctx:
+        fld = runner.Field(ctx, F32, L, "f16_brick")
+        tr = runner.resolve_total_field_time_reference(cfg, F32, energy_GeV=energy_gev, ctx=ctx)
+        state = (x[0, :n], x[1, :n], x[2, :n], p[0, :n], p[1, :n], p[2, :n])
+        res = runner.run_mu_cache_ensemble(cfg, F32, None, energy_GeV=energy_gev, time_reference=tr, ctx=ctx,
+                                           field=fld, initial_state=state, chunk_saves=10)
+        fld.close()
+    pushes = n * (res.trajectory_time.n_integration_steps - 1)
+    log_fn(f"[bench] runner with host mu cache: {pushes / res.elapsed:.4g} particle-steps/s")
+    return {"value": pushes / res.elapsed, "unit": "particle-steps/s", "elapsed_s": res.elapsed,
+            "kernel_s": res.kernel_ms / 1e3, "cache_gib": res.cache_gib,
+            "path": "run_mu_cache_ensemble: FP16 field, mu cache to host memory through double-buffered device "
+                    "chunks of 10 samples and pinned staging"}
+
+
 def serving_run(args, log_fn):
     """The serving stream on four resident 512^3 BF16 fields: Poisson arrivals
     at 20/s, per-request field/energy/count/duration/injection/output drawn as
@@ -308,6 +388,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-stats", action="store_true", help="skip the D_mumu / D_pp pair-visit rates")
+    ap.add_argument("--no-pcie", action="store_true", help="skip the runner path with a host mu cache")
     ap.add_argument("--serving-requests", type=int, default=100,
                     help="requests of the serving stream to replay (0 = skip; 2000 = the full stream)")
     ap.add_argument("--modes", action="store_true", help="also run the 96-campaign mode-decomposed batch")
@@ -407,6 +489,9 @@ def main():
         log("[bench] D_mumu check on 65,536 particles")
         check = dmumu_check(ctx, F32, L, x, p, dt, 65536, 4001, stride)
 
+    stats = None if args.no_stats else stats_rates(ctx, F32, L, x, p, dt, log)
+    pcie = None if args.no_pcie else pcie_inclusive(F32, L, x, p, args.energy_gev, log)
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU oracle on {args.cpu_particles} particles x {args.cpu_steps} steps")
@@ -461,6 +546,8 @@ def main():
                       "speedup_of_headline": value / plain_rate,
                       "speedup_of_bf16": timing["bf16_brick"]["value"] / plain_rate},
         "dmumu_check": check,
+        "statistics": stats,
+        "pcie_inclusive": pcie,
         "serving": serve,
         "modes_batch": modes,
         "cpu_baseline": cpu,
