@@ -333,7 +333,7 @@ def serving_run(args, log_fn):
 
 
 def modes_run(args, log_fn):
-    """Mode-decomposed batch: total + Alfven/fast/slow 256^3 fields (BF16),
+    """Mode-decomposed batch: total + Alfven/fast/slow 256^3 FP32 fields,
     clock from the total field, 6 energies log-spaced 1 GeV..100 TeV, fixed
     mu0 in {0.1,0.5,0.9} plus isotropic, uneven counts 2^14..2^20 per
     campaign: 96 campaigns submitted together and packed into the same ticks."""
@@ -345,7 +345,7 @@ def modes_run(args, log_fn):
     L = fields.box_length_m(10.0)
     ctx, sctx = Context(0), Context(0)
     gen = make_fields(args.mode_cells + 1, 0.5, 16.0, seed=555, modes=True)
-    flds = [Field(ctx, F, L, "bf16_brick") for F in gen]
+    flds = [Field(ctx, F, L, "f32_brick") for F in gen]
     del gen
     B0 = flds[0].B0  # shared total-field clock
     rng = np.random.default_rng(96)
@@ -370,7 +370,7 @@ def modes_run(args, log_fn):
     ctx.close()
     return {"campaigns": res.requests, "particle_steps_per_s": res.particle_steps_per_s, "wall_s": res.wall_s,
             "latency_p50_s": res.latency_p50_s, "latency_p99_s": res.latency_p99_s,
-            "config": f"4 x {args.mode_cells}^3 BF16 fields (total/alfven/fast/slow), 6 energies x 4 injections, "
+            "config": f"4 x {args.mode_cells}^3 FP32 fields (total/alfven/fast/slow), 6 energies x 4 injections, "
                       f"{args.mode_gp} T_g0 each"}
 
 

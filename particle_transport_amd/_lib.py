@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -172,24 +173,33 @@ class DeviceBuffer:
         self.ptr = None
 
 
+def _host_free(ptr_value):
+    try:
+        load().pt_host_free(ctypes.c_void_p(ptr_value))
+    except Exception:  # interpreter shutdown: the process releases it anyway
+        pass
+
+
 class PinnedArray:
-    """A numpy view of page-locked host memory (pt_host_alloc)."""
+    """A numpy view of page-locked host memory (pt_host_alloc).  The memory
+    lives as long as any numpy view of it: it is released when the last view
+    (``array``, slices of it, arrays handed out) is gone."""
 
     def __init__(self, shape, dtype=np.float32):
         self.dtype = np.dtype(dtype)
-        self.shape = tuple(shape)
+        self.shape = tuple(int(s) for s in shape)
         nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
         p = ctypes.c_void_p()
         check(load().pt_host_alloc(nbytes, ctypes.byref(p)))
         self.ptr = p.value
         buf = (ctypes.c_char * max(nbytes, 1)).from_address(self.ptr)
+        weakref.finalize(buf, _host_free, self.ptr)  # every view keeps buf alive
         self.array = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(self.shape))).reshape(self.shape)
 
     def free(self):
-        if self.ptr:
-            self.array = None
-            check(load().pt_host_free(ctypes.c_void_p(self.ptr)))
-            self.ptr = None
+        """Drop this handle's view (the memory goes with the last view)."""
+        self.array = None
+        self.ptr = None
 
 
 def exported_symbols():

@@ -200,19 +200,33 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblk) {
 template <int STORE>
 __device__ __forceinline__ void load_corners(const FieldView &fv, int32_t gi, int32_t gj, int32_t gk,
                                              float s[6][8]) {
+    if constexpr (STORE == PT_STORE_F32_PLANAR) {
 #pragma unroll
-    for (int ci = 0; ci < 8; ++ci) {
-        const int64_t i = gi + (ci & 1), j = gj + ((ci >> 1) & 1), k = gk + (ci >> 2);
-        if constexpr (STORE == PT_STORE_F32_PLANAR) {
+        for (int ci = 0; ci < 8; ++ci) {
+            const int64_t i = gi + (ci & 1), j = gj + ((ci >> 1) & 1), k = gk + (ci >> 2);
             const int64_t nn = fv.n[0] * fv.n[1] * fv.n[2];
             const int64_t idx = i + fv.n[0] * (j + fv.n[1] * k);
             const float *F = static_cast<const float *>(fv.data);
 #pragma unroll
             for (int c = 0; c < 6; ++c) s[c][ci] = F[c * nn + idx];
-        } else {
-            const int64_t rec = brick_node_index(fv.nb, i, j, k);
+        }
+    } else {
+        // Record of corner (0,0,0), then per-axis steps to the +1 neighbour:
+        // inside an 8^3 brick the steps are 1, 8 and 64 records; from the last
+        // node of a brick they jump to the next brick along that axis.  Every
+        // record index fits in 32 bits (< 2^31 / 16 bytes for any field the
+        // library accepts).
+        const int32_t nb0 = static_cast<int32_t>(fv.nb[0]), nb1 = static_cast<int32_t>(fv.nb[1]);
+        const int32_t base = static_cast<int32_t>(brick_node_index(fv.nb, gi, gj, gk));
+        const int32_t sx = (gi & 7) == 7 ? 512 - 7 : 1;
+        const int32_t sy = (gj & 7) == 7 ? nb0 * 512 - 56 : 8;
+        const int32_t sz = (gk & 7) == 7 ? nb0 * nb1 * 512 - 448 : 64;
+        const int32_t rec[8] = {base, base + sx, base + sy, base + sy + sx,
+                                base + sz, base + sz + sx, base + sz + sy, base + sz + sy + sx};
+#pragma unroll
+        for (int ci = 0; ci < 8; ++ci) {
             if constexpr (STORE == PT_STORE_F32_BRICK) {
-                const float4 *R = static_cast<const float4 *>(fv.data) + rec * 2;
+                const float4 *R = static_cast<const float4 *>(fv.data) + static_cast<int64_t>(rec[ci]) * 2;
                 const float4 a = R[0];
                 const float4 b = R[1];
                 s[0][ci] = a.x;
@@ -222,7 +236,7 @@ __device__ __forceinline__ void load_corners(const FieldView &fv, int32_t gi, in
                 s[4][ci] = b.x;
                 s[5][ci] = b.y;
             } else {
-                const uint4 w = static_cast<const uint4 *>(fv.data)[rec];
+                const uint4 w = static_cast<const uint4 *>(fv.data)[rec[ci]];
                 if constexpr (STORE == PT_STORE_BF16_BRICK) {
                     s[0][ci] = __uint_as_float(w.x << 16);
                     s[1][ci] = __uint_as_float(w.x & 0xffff0000u);

@@ -464,6 +464,14 @@ int pt_field_create(pt_ctx *ctx, int64_t nx, int64_t ny, int64_t nz, double box_
             f->g.nb[c] = (nn[c] + 7) / 8;
         }
         f->g.L = box_length_m;
+        if (storage == PT_STORE_F32_BRICK || storage == PT_STORE_BF16_BRICK || storage == PT_STORE_F16_BRICK) {
+            const int64_t records = f->g.nb[0] * f->g.nb[1] * f->g.nb[2] * 512;
+            if (records > INT32_MAX) {
+                delete f;
+                throw Error{PT_ERR_INVALID, "pt_field_create: brick storage holds at most 2^31 node records "
+                                             "(about 1288^3 nodes); this grid needs " + std::to_string(records)};
+            }
+        }
         void *staging = nullptr;
         size_t staging_bytes = 0;
         try {

@@ -13,6 +13,7 @@
 //  Integer counts are exact; sums are FP64.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "pt_internal.hpp"
@@ -114,18 +115,23 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     __syncthreads();
 
     const int gy = blockIdx.y;
+    // The last group is padded with its first lag: every load stays in range
+    // and the padded slots are never written out (li < nlags below).
     int64_t lag[kG];
 #pragma unroll
     for (int g = 0; g < kG; ++g) {
         const int li = gy * kG + g;
-        lag[g] = li < nlags ? lags[li] : INT64_MAX / 4;
+        lag[g] = lags[li < nlags ? li : gy * kG];
     }
     const int64_t j = static_cast<int64_t>(blockIdx.x) * kTPB + threadIdx.x;
     const bool active = j < n_sel;
 
-    unsigned rc[kG];
+    // Run accumulators of the current start bin (every pair is added; a run
+    // whose start was outside the binned range is dropped at the flush) and
+    // the particle's totals over all valid pairs (sum dmu^2, count), folded
+    // in at each flush.
+    unsigned rc[kG], pc[kG];
     double r1[kG], r2[kG], ps[kG];
-    unsigned pc[kG];
 #pragma unroll
     for (int g = 0; g < kG; ++g) {
         rc[g] = 0;
@@ -135,29 +141,42 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     int cur_bin = -1;
     float cur_lo = INFINITY, cur_hi = -INFINITY;
 
+    auto flush = [&]() {
+#pragma unroll
+        for (int g = 0; g < kG; ++g) {
+            if (cur_bin >= 0 && rc[g]) {
+                atomicAdd(&b_cnt[g * nbins + cur_bin], static_cast<unsigned long long>(rc[g]));
+                atomicAdd(&b_s1[g * nbins + cur_bin], r1[g]);
+                atomicAdd(&b_s2[g * nbins + cur_bin], r2[g]);
+            }
+            pc[g] += rc[g];
+            ps[g] += r2[g];
+            rc[g] = 0;
+            r1[g] = r2[g] = 0.0;
+        }
+    };
+
     if (active) {
         const int64_t col = pidx ? pidx[j] : j;
-        const float *m = mu + col * sp;
-        int64_t lag_min = lag[0];
+        const float *row = mu + col * sp;  // sample s of this particle
+        int64_t lag_min = lag[0], lag_max = lag[0], off[kG];
 #pragma unroll
-        for (int g = 1; g < kG; ++g) lag_min = lag[g] < lag_min ? lag[g] : lag_min;
+        for (int g = 0; g < kG; ++g) {
+            lag_min = lag[g] < lag_min ? lag[g] : lag_min;
+            lag_max = lag[g] > lag_max ? lag[g] : lag_max;
+            off[g] = lag[g] * ss;
+        }
         const int64_t s_end = injection ? (lag_min < nsave ? 1 : 0) : (nsave - lag_min);
-        for (int64_t s = 0; s < s_end; ++s) {
-            const float a = m[s * ss];
-            if (!isfinite(a)) continue;
+        const int64_t s_full = nsave - lag_max;  // below it every lag of the group is in range
+
+        // One start sample s: bin of mu[s] (a run boundary only when it
+        // changes), then the pair with every lag, without branches.
+        auto visit = [&](int64_t s, auto checked) {
+            const float a = row[0];
+            if (!isfinite(a)) return;
             const float v = bin_abs ? fabsf(a) : a;
             if (!(v >= cur_lo && v < cur_hi)) {
-                if (cur_bin >= 0) {
-#pragma unroll
-                    for (int g = 0; g < kG; ++g)
-                        if (rc[g]) {
-                            atomicAdd(&b_cnt[g * nbins + cur_bin], static_cast<unsigned long long>(rc[g]));
-                            atomicAdd(&b_s1[g * nbins + cur_bin], r1[g]);
-                            atomicAdd(&b_s2[g * nbins + cur_bin], r2[g]);
-                            rc[g] = 0;
-                            r1[g] = r2[g] = 0.0;
-                        }
-                }
+                flush();
                 if (v < lo[0]) {
                     cur_bin = -1;
                     cur_lo = -INFINITY;
@@ -183,31 +202,21 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
             const double ad = static_cast<double>(a);
 #pragma unroll
             for (int g = 0; g < kG; ++g) {
-                if (s + lag[g] < nsave) {
-                    const float b = m[(s + lag[g]) * ss];
-                    if (isfinite(b)) {
-                        const double d = static_cast<double>(b) - ad;
-                        const double d2 = d * d;
-                        ps[g] += d2;
-                        pc[g] += 1;
-                        if (cur_bin >= 0) {
-                            rc[g] += 1;
-                            r1[g] += d;
-                            r2[g] += d2;
-                        }
-                    }
+                if constexpr (decltype(checked)::value) {
+                    if (s + lag[g] >= nsave) continue;
                 }
+                const float b = row[off[g]];
+                const bool ok = isfinite(b);
+                const double d = ok ? static_cast<double>(b) - ad : 0.0;
+                rc[g] += ok ? 1u : 0u;
+                r1[g] += d;
+                r2[g] = fma(d, d, r2[g]);
             }
-        }
-        if (cur_bin >= 0) {
-#pragma unroll
-            for (int g = 0; g < kG; ++g)
-                if (rc[g]) {
-                    atomicAdd(&b_cnt[g * nbins + cur_bin], static_cast<unsigned long long>(rc[g]));
-                    atomicAdd(&b_s1[g * nbins + cur_bin], r1[g]);
-                    atomicAdd(&b_s2[g * nbins + cur_bin], r2[g]);
-                }
-        }
+        };
+        int64_t s = 0;
+        for (; s < s_end && s < s_full; ++s, row += ss) visit(s, std::false_type{});
+        for (; s < s_end; ++s, row += ss) visit(s, std::true_type{});
+        flush();
     }
 
     const int nbx = gridDim.x;
@@ -279,10 +288,23 @@ __device__ __forceinline__ double pmag(const double *__restrict__ pm, int64_t s,
     return pm[s * total + col];
 }
 
-template <class Src>
+// |p| in FP64 of every (sample, selected particle): momentum_magnitude
+// (compute_dpp_full.jl:232-234) evaluated once per sample, not once per pair.
+__global__ void k_pmag(const float *__restrict__ mom, int64_t nsave, int64_t total, int64_t c0, int64_t nc,
+                       const int64_t *__restrict__ pidx, double *__restrict__ out) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nsave * nc) return;
+    const int64_t s = t / nc, jj = t % nc;
+    const int64_t col = pidx ? pidx[c0 + jj] : c0 + jj;
+    out[t] = pmag(mom, s, total, col);
+}
+
+// D_pp pairs over an FP64 |p| series [save][total]: the walk of k_dmumu
+// without bins.  inv_p0 = 1/p0 (the normalised sums differ from Delta p / p0
+// by at most an ulp per term).
 __global__ void __launch_bounds__(kTPB)
-k_dpp(const Src *__restrict__ mom, int64_t nsave, int64_t total, int64_t n_sel,
-      const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags, double p0,
+k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
+      const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags, double inv_p0,
       double *__restrict__ part_d, long long *__restrict__ part_i) {
     __shared__ double sh_d[kTPB / 64];
     __shared__ long long sh_i[kTPB / 64];
@@ -291,7 +313,7 @@ k_dpp(const Src *__restrict__ mom, int64_t nsave, int64_t total, int64_t n_sel,
 #pragma unroll
     for (int g = 0; g < kG; ++g) {
         const int li = gy * kG + g;
-        lag[g] = li < nlags ? lags[li] : INT64_MAX / 4;
+        lag[g] = lags[li < nlags ? li : gy * kG];
     }
     const int64_t j = static_cast<int64_t>(blockIdx.x) * kTPB + threadIdx.x;
     unsigned cnt[kG];
@@ -303,28 +325,37 @@ k_dpp(const Src *__restrict__ mom, int64_t nsave, int64_t total, int64_t n_sel,
     }
     if (j < n_sel) {
         const int64_t col = pidx ? pidx[j] : j;
-        int64_t lag_min = lag[0];
+        const double *row = pm + col;
+        int64_t lag_min = lag[0], lag_max = lag[0], off[kG];
 #pragma unroll
-        for (int g = 1; g < kG; ++g) lag_min = lag[g] < lag_min ? lag[g] : lag_min;
-        for (int64_t s = 0; s + lag_min < nsave; ++s) {
-            const double pst = pmag(mom, s, total, col);
-            if (!isfinite(pst)) continue;
+        for (int g = 0; g < kG; ++g) {
+            lag_min = lag[g] < lag_min ? lag[g] : lag_min;
+            lag_max = lag[g] > lag_max ? lag[g] : lag_max;
+            off[g] = lag[g] * total;
+        }
+        const int64_t s_end = nsave - lag_min, s_full = nsave - lag_max;
+        auto visit = [&](int64_t s, auto checked) {
+            const double pst = row[0];
+            if (!isfinite(pst)) return;
 #pragma unroll
             for (int g = 0; g < kG; ++g) {
-                if (s + lag[g] < nsave) {
-                    const double pe = pmag(mom, s + lag[g], total, col);
-                    if (isfinite(pe)) {
-                        const double d = pe - pst;
-                        const double dn = d / p0;
-                        cnt[g] += 1;
-                        s1[g] += d;
-                        s2[g] += d * d;
-                        s3[g] += dn;
-                        s4[g] += dn * dn;
-                    }
+                if constexpr (decltype(checked)::value) {
+                    if (s + lag[g] >= nsave) continue;
                 }
+                const double pe = row[off[g]];
+                const bool ok = isfinite(pe);
+                const double d = ok ? pe - pst : 0.0;
+                const double dn = d * inv_p0;
+                cnt[g] += ok ? 1u : 0u;
+                s1[g] += d;
+                s2[g] = fma(d, d, s2[g]);
+                s3[g] += dn;
+                s4[g] = fma(dn, dn, s4[g]);
             }
-        }
+        };
+        int64_t s = 0;
+        for (; s < s_end && s < s_full; ++s, row += total) visit(s, std::false_type{});
+        for (; s < s_end; ++s, row += total) visit(s, std::true_type{});
     }
     const int nbx = gridDim.x;
 #pragma unroll
@@ -697,32 +728,59 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         if (particle_index)
             for (int64_t j = 0; j < n_sel; ++j)
                 PT_REQUIRE(particle_index[j] >= 0 && particle_index[j] < total, "pt_dpp_accumulate: particle index out of range");
+        PT_REQUIRE(std::isfinite(p0) && p0 > 0.0, "pt_dpp_accumulate: p0 must be finite and positive");
         if (n_sel == 0) return;
         PT_HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t st = ctx->stream;
-        DevBuf hm, hidx, hlag, hout;
+        DevBuf hm, hidx, hlag, hout, hpm;
         const Src *dm = to_device(momenta, on_device, static_cast<size_t>(nsave * comps * total), hm, st);
         const int64_t *didx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
         const int64_t *dlag = to_device(lags.data(), 0, lags.size(), hlag, st);
-        const int nbx = static_cast<int>((n_sel + kTPB - 1) / kTPB);
+        // FP32 momenta: |p| once per (sample, particle) into an FP64 scratch
+        // series, a chunk of particles at a time (at most ~2 GiB).
+        int64_t chunk = n_sel;
+        if constexpr (!std::is_same<Src, double>::value) {
+            const int64_t cap = std::max<int64_t>(kTPB, ((int64_t{1} << 31) / (8 * nsave)) / kTPB * kTPB);
+            chunk = std::min(n_sel, cap);
+        }
         const int ngy = static_cast<int>((nlags + kG - 1) / kG);
-        const size_t nl = static_cast<size_t>(nlags), parts = nl * static_cast<size_t>(nbx);
+        const int nbx_max = static_cast<int>((chunk + kTPB - 1) / kTPB);
+        const size_t nl = static_cast<size_t>(nlags), parts = nl * static_cast<size_t>(nbx_max);
         char *base = static_cast<char *>(hout.alloc(parts * (32 + 8) + nl * (32 + 8)));
         auto *part_d = reinterpret_cast<double *>(base);
         auto *part_i = reinterpret_cast<long long *>(part_d + parts * 4);
         auto *out_d = reinterpret_cast<double *>(part_i + parts);
         auto *out_i = reinterpret_cast<long long *>(out_d + nl * 4);
-        dim3 grid(static_cast<unsigned>(nbx), static_cast<unsigned>(ngy));
-        k_dpp<<<grid, kTPB, 0, st>>>(dm, nsave, total, n_sel, didx, dlag, static_cast<int>(nlags), p0, part_d, part_i);
-        PT_HIP_CHECK(hipGetLastError());
-        k_merge_dpp<<<static_cast<unsigned>((nlags + 63) / 64), 64, 0, st>>>(part_d, part_i, static_cast<int>(nlags), nbx,
-                                                                             out_d, out_i);
-        PT_HIP_CHECK(hipGetLastError());
-        std::vector<double> hd(nl * 4);
-        std::vector<long long> hi(nl);
-        PT_HIP_CHECK(hipMemcpyAsync(hd.data(), out_d, nl * 32, hipMemcpyDeviceToHost, st));
-        PT_HIP_CHECK(hipMemcpyAsync(hi.data(), out_i, nl * 8, hipMemcpyDeviceToHost, st));
-        PT_HIP_CHECK(hipStreamSynchronize(st));
+        std::vector<double> hd(nl * 4, 0.0), cd(nl * 4);
+        std::vector<long long> hi(nl, 0), ci(nl);
+        auto run = [&](const double *pm, int64_t row_len, int64_t nsel_c, const int64_t *idx) {
+            const int nbx = static_cast<int>((nsel_c + kTPB - 1) / kTPB);
+            dim3 grid(static_cast<unsigned>(nbx), static_cast<unsigned>(ngy));
+            k_dpp<<<grid, kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag, static_cast<int>(nlags), 1.0 / p0,
+                                         part_d, part_i);
+            PT_HIP_CHECK(hipGetLastError());
+            k_merge_dpp<<<static_cast<unsigned>((nlags + 63) / 64), 64, 0, st>>>(part_d, part_i,
+                                                                                 static_cast<int>(nlags), nbx, out_d,
+                                                                                 out_i);
+            PT_HIP_CHECK(hipGetLastError());
+            PT_HIP_CHECK(hipMemcpyAsync(cd.data(), out_d, nl * 32, hipMemcpyDeviceToHost, st));
+            PT_HIP_CHECK(hipMemcpyAsync(ci.data(), out_i, nl * 8, hipMemcpyDeviceToHost, st));
+            PT_HIP_CHECK(hipStreamSynchronize(st));
+            for (size_t q = 0; q < nl * 4; ++q) hd[q] += cd[q];
+            for (size_t q = 0; q < nl; ++q) hi[q] += ci[q];
+        };
+        if constexpr (std::is_same<Src, double>::value) {
+            run(dm, total, n_sel, didx);
+        } else {
+            auto *pm = static_cast<double *>(hpm.alloc(static_cast<size_t>(nsave * chunk) * 8));
+            for (int64_t c0 = 0; c0 < n_sel; c0 += chunk) {
+                const int64_t nc = std::min(chunk, n_sel - c0);
+                const unsigned blocks = static_cast<unsigned>((nsave * nc + 255) / 256);
+                k_pmag<<<blocks, 256, 0, st>>>(dm, nsave, total, c0, nc, didx, pm);
+                PT_HIP_CHECK(hipGetLastError());
+                run(pm, nc, nc, nullptr);
+            }
+        }
         for (size_t li = 0; li < nl; ++li) {
             if (lags[li] >= nsave) continue;
             counts[li] += hi[li];

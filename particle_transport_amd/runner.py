@@ -223,14 +223,16 @@ def _saves_in_range(first, count, stride):
 class CacheWriter:
     """Cadence cache in the reference's dataset layout (HDF5.jl writes a Julia
     (n,3,nsave) array, seen from C as (nsave,3,n); numpy uses the C view).
-    Without an output directory the datasets stay in memory; with one they are
-    .npy memory maps next to a metadata.json (h5py is not part of this build)."""
+    Without an output directory the datasets stay in memory, page-locked so
+    that device chunks are copied straight into them; with one they are .npy
+    memory maps next to a metadata.json (h5py is not part of this build)."""
 
     def __init__(self, path, datasets: dict, dtype=np.float32):
         self.path = path
         self.dtype = dtype
         self.data = {}
         self.meta = {}
+        self.pinned = not path
         if path:
             os.makedirs(path, exist_ok=True)
         for name, shape in datasets.items():
@@ -238,7 +240,11 @@ class CacheWriter:
                 self.data[name] = np.lib.format.open_memmap(os.path.join(path, name + ".npy"), mode="w+",
                                                             dtype=dtype, shape=shape)
             else:
-                self.data[name] = np.empty(shape, dtype=dtype)
+                try:
+                    self.data[name] = _lib.PinnedArray(shape, dtype).array
+                except _lib.PTError:  # more than the host will page-lock: pageable + staging
+                    self.pinned = False
+                    self.data[name] = np.empty(shape, dtype=dtype)
 
     def __getitem__(self, k):
         return self.data[k]
@@ -352,7 +358,9 @@ def _run_loop(cfg, fields, setup, out_kind, writer, ctx, field, initial_state, c
     shape = (chunk_saves, 3, n) if out_kind == "phase" else (chunk_saves, n)
     names = ("positions", "momenta") if out_kind == "phase" else ("mu",)
     dev = [[_lib.DeviceBuffer(ctx, shape, out_T) for _ in names] for _ in range(2)]
-    pinned = [[_lib.PinnedArray(shape, out_T) for _ in names] for _ in range(2)]
+    # a page-locked in-memory cache takes the copies directly; a file-backed
+    # one goes through page-locked staging buffers
+    pinned = None if writer.pinned else [[_lib.PinnedArray(shape, out_T) for _ in names] for _ in range(2)]
     copy_ctx = Context(ctx.device)
     pool = ThreadPoolExecutor(max_workers=1)
     busy = [None, None]
@@ -361,9 +369,14 @@ def _run_loop(cfg, fields, setup, out_kind, writer, ctx, field, initial_state, c
     def drain(buf, offset, count):
         for k, name in enumerate(names):
             nbytes = count * int(np.prod(shape[1:])) * np.dtype(out_T).itemsize
-            check(_lib.load().pt_memcpy(copy_ctx.h, ctypes.c_void_p(pinned[buf][k].ptr),
-                                        ctypes.c_void_p(dev[buf][k].ptr), nbytes, 1))
-            writer[name][offset:offset + count] = pinned[buf][k].array[:count]
+            if pinned is None:
+                dst = writer[name][offset:offset + count]
+                check(_lib.load().pt_memcpy(copy_ctx.h, ctypes.c_void_p(dst.ctypes.data),
+                                            ctypes.c_void_p(dev[buf][k].ptr), nbytes, 1))
+            else:
+                check(_lib.load().pt_memcpy(copy_ctx.h, ctypes.c_void_p(pinned[buf][k].ptr),
+                                            ctypes.c_void_p(dev[buf][k].ptr), nbytes, 1))
+                writer[name][offset:offset + count] = pinned[buf][k].array[:count]
 
     try:
         t0 = time.perf_counter()
@@ -398,7 +411,7 @@ def _run_loop(cfg, fields, setup, out_kind, writer, ctx, field, initial_state, c
         for group in dev:
             for d in group:
                 d.free()
-        for group in pinned:
+        for group in pinned or ():
             for h in group:
                 h.free()
         copy_ctx.close()

@@ -281,6 +281,27 @@ def test_dpp_and_energy_snapshots_match_oracle(ctx):
     assert np.allclose(e[np.isfinite(eo)], eo[np.isfinite(eo)], rtol=1e-14)
 
 
+def test_dpp_from_fp64_pmag_cache_matches_oracle(ctx):
+    """pt_dpp_accumulate_pmag (the serving path's FP64 |p| cache [save][n])
+    equals the oracle's D_pp on the FP32 momenta those |p| come from."""
+    from particle_transport_amd import _lib, serving
+    rng = np.random.default_rng(7)
+    nsave, n = 333, 1000
+    mom = (rng.normal(0, 1e-18, (nsave, 3, n)) + 5e-17).astype(np.float32)
+    mom[:, :, 5:9][rng.random((nsave, 3, 4)) < 0.05] = np.nan
+    pm = np.sqrt((mom.astype(np.float64) ** 2).sum(axis=1))  # momentum_magnitude in FP64
+    lags = list(range(1, 40, 3)) + [100, 332, 333]
+    p0 = float(pm[0, 0])
+    buf = _lib.DeviceBuffer.from_host(ctx, np.ascontiguousarray(pm))
+    acc = serving.dpp_on_device(ctx, buf.ptr, nsave, n, lags, p0)
+    buf.free()
+    oacc = oracle.new_dpp_accumulators(len(lags))
+    oracle.dpp(mom, lags, p0, oacc)
+    assert np.array_equal(acc.counts, oacc["counts"])
+    for k in ("sum_delta_p", "sum_delta_p2", "sum_delta_p_norm", "sum_delta_p_norm2"):
+        assert np.allclose(getattr(acc, k), oacc[k], rtol=1e-11, atol=1e-12 * np.max(np.abs(oacc[k]))), k
+
+
 @pytest.mark.parametrize("storage", ["f64_planar", "f32_planar"])
 def test_mu_from_phase_matches_oracle(ctx, storage):
     F, x, p, dt, g = make_case(n=200)
