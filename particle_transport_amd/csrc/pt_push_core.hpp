@@ -45,8 +45,13 @@ struct Phys {
 // in FP64 (|U+e|^2 = |U|^2 + 2 U.e + e.e), the rotation is applied to the FP32
 // vector, and the result is rescaled to |U| = sqrt(P2).  Without this the FP32
 // rotation's rounding drifts |p| by ~5e-8 per push, systematically.
+//
+// `renorm` (uniform across the wave) applies the rescale; between rescales
+// |U| is off by at most the rotation drift of the pushes since the last one,
+// while P2, gamma and the position update stay exact to FP64/FP32 rounding.
 __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, double &P2, float &onep2, float bx,
-                                            float by, float bz, float ux, float uy, float uz, float &inv_gn) {
+                                            float by, float bz, float ux, float uy, float uz, float &inv_gn,
+                                            bool renorm = true) {
     // E~ = -(u~ x B~)
     const float ex = -(uy * bz - uz * by);
     const float ey = -(uz * bx - ux * bz);
@@ -77,13 +82,15 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
     Ux = qx + ex;
     Uy = qy + ey;
     Uz = qz + ez;
-    // rescale to sqrt(P2n): the ratio is 1 + O(1e-7), so sqrt(1+d) = 1 + d/2 - d^2/8
-    const float u2 = Ux * Ux + Uy * Uy + Uz * Uz;
-    const float dd = static_cast<float>(P2n - static_cast<double>(u2)) * __builtin_amdgcn_rcpf(u2);
-    const float sc = fmaf(dd, fmaf(-0.125f, dd, 0.5f), 1.0f);
-    Ux *= sc;
-    Uy *= sc;
-    Uz *= sc;
+    if (renorm) {
+        // rescale to sqrt(P2n): the ratio is 1 + O(1e-6), so sqrt(1+d) = 1 + d/2 - d^2/8
+        const float u2 = Ux * Ux + Uy * Uy + Uz * Uz;
+        const float dd = static_cast<float>(P2n - static_cast<double>(u2)) * __builtin_amdgcn_rcpf(u2);
+        const float sc = fmaf(dd, fmaf(-0.125f, dd, 0.5f), 1.0f);
+        Ux *= sc;
+        Uy *= sc;
+        Uz *= sc;
+    }
     P2 = P2n;
     onep2 = static_cast<float>(1.0 + P2n);
     inv_gn = __builtin_amdgcn_rsqf(onep2);
@@ -339,6 +346,7 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
         for (int q = 0; q < 8; ++q) a[c][q] = 0.0f;
     bool reload = true;
     float onep2 = static_cast<float>(1.0 + P2);
+    const int32_t si_mod4 = static_cast<int32_t>(cad.si0 & 3);
     int32_t phase = cad.phase0;
     int64_t slot = cad.slot0;
     // pushes happen while si < total_steps: at most n_push of the K steps
@@ -374,10 +382,22 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
             ++slot;
         }
         if (!push) continue;
+        // |U| is rescaled after the pushes of global steps si = 4, 8, ... and
+        // before each save (phase == 0: the next step saves), so every output
+        // carries the rescaled magnitude and the result does not depend on
+        // how the steps are split into launches
+        const bool renorm = (((si_mod4 + k) & 3) == 0) | (phase == 0);
         float inv_gn;
-        boris_split(U[0], U[1], U[2], P2, onep2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn);
+        boris_split(U[0], U[1], U[2], P2, onep2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn, renorm);
+        // all three offsets first; one rare branch for any cell change
 #pragma unroll
-        for (int c = 0; c < 3; ++c) move_axis(cell[c], off[c], U[c], g.k[c] * inv_gn, g.nc[c], periodic, reload);
+        for (int c = 0; c < 3; ++c) off[c] = fmaf(U[c], g.k[c] * inv_gn, off[c]);
+        const bool inside = (off[0] >= 0.0f) & (off[0] < 1.0f) & (off[1] >= 0.0f) & (off[1] < 1.0f) &
+                            (off[2] >= 0.0f) & (off[2] < 1.0f);
+        if (!inside) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) move_axis(cell[c], off[c], 0.0f, 0.0f, g.nc[c], periodic, reload);
+        }
         if (!periodic && (outside(cell[0], off[0], g.nc[0]) || outside(cell[1], off[1], g.nc[1]) ||
                           outside(cell[2], off[2], g.nc[2]))) {
             alive = false;

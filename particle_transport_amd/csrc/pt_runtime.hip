@@ -201,9 +201,14 @@ __global__ void k_split_to_si(SplitState s, const int32_t *__restrict__ perm, co
     if (i >= n) return;
     const int64_t o = perm ? perm[i] : i;  // original particle index of slot i
     double d[3] = {d0, d1, d2};
+    // the pushers rescale |U| on a cadence; the readout applies the exact
+    // magnitude sqrt(P2) carried in FP64
+    double u2 = 0.0;
+    for (int c = 0; c < 3; ++c) u2 += static_cast<double>(s.P[c][i]) * static_cast<double>(s.P[c][i]);
+    const double sc = u2 > 0.0 ? sqrt(s.P2[i] / u2) : 1.0;
     for (int c = 0; c < 3; ++c) {
         x[c * n + o] = (static_cast<double>(s.cell[c][i]) + static_cast<double>(s.off[c][i])) * d[c];
-        p[c * n + o] = static_cast<double>(s.P[c][i]) * pnorm;
+        p[c * n + o] = static_cast<double>(s.P[c][i]) * sc * pnorm;
     }
     alive_out[o] = alive[i];
 }
