@@ -104,8 +104,11 @@ k_push_plain(SplitState s, uint8_t *__restrict__ alive_arr, const float *__restr
 
 // ------------------------------------------------------------ fast ---------
 
+// Five waves per SIMD (at most 102 VGPRs): the launch constants the compiler
+// then keeps in scratch are read only on the rare paths (coefficient reload,
+// cell change, save), never in the push itself.
 template <int STORE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_push_fast(SplitState s, uint8_t *__restrict__ alive_arr, FieldView fv, SplitGeom g, Phys ph,
             int64_t n, Cadence cad, int out_kind, int out_f64, void *out_a, void *out_b, int periodic,
             int64_t first_step, unsigned long long *deaths, const int32_t *__restrict__ perm) {

@@ -47,7 +47,8 @@ struct DevReq {
 };
 
 template <int STORE>
-__global__ void __launch_bounds__(256) k_serve(const DevReq *__restrict__ reqs, int nreq) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_serve(const DevReq *__restrict__ reqs, int nreq) {
     const int64_t b = blockIdx.x;
     int lo = 0, hi = nreq - 1;
     while (lo < hi) {
