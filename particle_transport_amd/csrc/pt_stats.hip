@@ -84,6 +84,22 @@ __device__ __forceinline__ long long block_sum_i(long long v, long long *sh) {
     return r;
 }
 
+// Linear block id -> (particle tile, lag group).  Hardware dispatches block
+// b to XCD b % 8; the lag groups of one tile are given consecutive ids on the
+// same XCD, so they walk the same series together out of that XCD's L2
+// instead of each reading it from HBM.  tile >= nbx: an idle block.
+struct TileGroup {
+    int tile, group;
+};
+__device__ __forceinline__ TileGroup tile_group(int ngy) {
+    const int b = static_cast<int>(blockIdx.x);
+    const int r = b >> 3;
+    return {(r / ngy) * 8 + (b & 7), r % ngy};
+}
+__host__ inline unsigned tile_group_blocks(int nbx, int ngy) {
+    return static_cast<unsigned>(((nbx + 7) / 8) * 8 * ngy);
+}
+
 // ---------------------------------------------------------------- D_mumu ---
 //
 // Bins: bin b (0-based) holds float v with lo[b] <= v < lo[b+1] (hi_last for
@@ -96,7 +112,9 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags,
         const float *__restrict__ lo_g, int nbins, float hi_last, int bin_abs, int injection,
         unsigned long long *__restrict__ g_cnt, double *__restrict__ g_s1, double *__restrict__ g_s2,
-        double *__restrict__ part_d, long long *__restrict__ part_i) {
+        double *__restrict__ part_d, long long *__restrict__ part_i, int nbx, int ngy) {
+    const TileGroup tg = tile_group(ngy);
+    if (tg.tile >= nbx) return;  // whole block: before any barrier
     extern __shared__ unsigned char smem[];
     unsigned long long *b_cnt = reinterpret_cast<unsigned long long *>(smem);
     double *b_s1 = reinterpret_cast<double *>(b_cnt + kG * nbins);
@@ -114,7 +132,7 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     for (int i = threadIdx.x; i < nbins; i += kTPB) lo[i] = lo_g[i];
     __syncthreads();
 
-    const int gy = blockIdx.y;
+    const int gy = tg.group;
     // The last group is padded with its first lag: every load stays in range
     // and the padded slots are never written out (li < nlags below).
     int64_t lag[kG];
@@ -123,7 +141,7 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         const int li = gy * kG + g;
         lag[g] = lags[li < nlags ? li : gy * kG];
     }
-    const int64_t j = static_cast<int64_t>(blockIdx.x) * kTPB + threadIdx.x;
+    const int64_t j = static_cast<int64_t>(tg.tile) * kTPB + threadIdx.x;
     const bool active = j < n_sel;
 
     // Run accumulators of the current start bin (every pair is added; a run
@@ -219,7 +237,6 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         flush();
     }
 
-    const int nbx = gridDim.x;
 #pragma unroll
     for (int g = 0; g < kG; ++g) {
         const int li = gy * kG + g;
@@ -232,7 +249,7 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         const double pss = block_sum_d(pc[g] > 0 ? ps[g] : 0.0, sh_d);
         const long long pcs = block_sum_i(static_cast<long long>(pc[g]), sh_i);
         if (threadIdx.x == 0 && li < nlags) {
-            const int64_t o = static_cast<int64_t>(li) * nbx + blockIdx.x;
+            const int64_t o = static_cast<int64_t>(li) * nbx + tg.tile;
             part_d[o * 3 + 0] = bw.mean;
             part_d[o * 3 + 1] = bw.m2;
             part_d[o * 3 + 2] = pss;
@@ -305,17 +322,19 @@ __global__ void k_pmag(const float *__restrict__ mom, int64_t nsave, int64_t tot
 __global__ void __launch_bounds__(kTPB)
 k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
       const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags, double inv_p0,
-      double *__restrict__ part_d, long long *__restrict__ part_i) {
+      double *__restrict__ part_d, long long *__restrict__ part_i, int nbx, int ngy) {
+    const TileGroup tg = tile_group(ngy);
+    if (tg.tile >= nbx) return;
     __shared__ double sh_d[kTPB / 64];
     __shared__ long long sh_i[kTPB / 64];
-    const int gy = blockIdx.y;
+    const int gy = tg.group;
     int64_t lag[kG];
 #pragma unroll
     for (int g = 0; g < kG; ++g) {
         const int li = gy * kG + g;
         lag[g] = lags[li < nlags ? li : gy * kG];
     }
-    const int64_t j = static_cast<int64_t>(blockIdx.x) * kTPB + threadIdx.x;
+    const int64_t j = static_cast<int64_t>(tg.tile) * kTPB + threadIdx.x;
     unsigned cnt[kG];
     double s1[kG], s2[kG], s3[kG], s4[kG];
 #pragma unroll
@@ -357,7 +376,6 @@ k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel
         for (; s < s_end && s < s_full; ++s, row += total) visit(s, std::false_type{});
         for (; s < s_end; ++s, row += total) visit(s, std::true_type{});
     }
-    const int nbx = gridDim.x;
 #pragma unroll
     for (int g = 0; g < kG; ++g) {
         const int li = gy * kG + g;
@@ -367,7 +385,7 @@ k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel
         const double cc = block_sum_d(s3[g], sh_d);
         const double dd = block_sum_d(s4[g], sh_d);
         if (threadIdx.x == 0 && li < nlags) {
-            const int64_t o = static_cast<int64_t>(li) * nbx + blockIdx.x;
+            const int64_t o = static_cast<int64_t>(li) * nbx + tg.tile;
             part_i[o] = c;
             part_d[o * 4 + 0] = a;
             part_d[o * 4 + 1] = b;
@@ -676,10 +694,9 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
         PT_HIP_CHECK(hipMemsetAsync(base, 0, nl * nb * 24, st));
 
         const size_t shm = static_cast<size_t>(kG) * nb * 24 + nb * 4;
-        dim3 grid(static_cast<unsigned>(nbx), static_cast<unsigned>(ngy));
-        k_dmumu<<<grid, kTPB, shm, st>>>(dmu, nsave, n_sel, stride_s, stride_p, didx, dlag, static_cast<int>(nlags),
-                                         dlo, static_cast<int>(n_bins), hi_last, bin_abs ? 1 : 0, start_mode,
-                                         g_cnt, g_s1, g_s2, part_d, part_i);
+        k_dmumu<<<tile_group_blocks(nbx, ngy), kTPB, shm, st>>>(
+            dmu, nsave, n_sel, stride_s, stride_p, didx, dlag, static_cast<int>(nlags), dlo, static_cast<int>(n_bins),
+            hi_last, bin_abs ? 1 : 0, start_mode, g_cnt, g_s1, g_s2, part_d, part_i, nbx, ngy);
         PT_HIP_CHECK(hipGetLastError());
         k_merge_partials<<<static_cast<unsigned>((nlags + 63) / 64), 64, 0, st>>>(part_d, part_i, static_cast<int>(nlags),
                                                                                   nbx, out_d, out_i);
@@ -755,9 +772,9 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         std::vector<long long> hi(nl, 0), ci(nl);
         auto run = [&](const double *pm, int64_t row_len, int64_t nsel_c, const int64_t *idx) {
             const int nbx = static_cast<int>((nsel_c + kTPB - 1) / kTPB);
-            dim3 grid(static_cast<unsigned>(nbx), static_cast<unsigned>(ngy));
-            k_dpp<<<grid, kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag, static_cast<int>(nlags), 1.0 / p0,
-                                         part_d, part_i);
+            k_dpp<<<tile_group_blocks(nbx, ngy), kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag,
+                                                                static_cast<int>(nlags), 1.0 / p0, part_d, part_i,
+                                                                nbx, ngy);
             PT_HIP_CHECK(hipGetLastError());
             k_merge_dpp<<<static_cast<unsigned>((nlags + 63) / 64), 64, 0, st>>>(part_d, part_i,
                                                                                  static_cast<int>(nlags), nbx, out_d,
