@@ -305,6 +305,150 @@ __device__ __forceinline__ float eval_poly(const float a[8], float x, float y, f
 }
 
 
+// The 8 corner values of one component of cell (gi,gj,gk), in storage
+// units (corner ci = dx + 2 dy + 4 dz).
+template <int STORE>
+__device__ __forceinline__ void load_component_corners(const FieldView &fv, int32_t gi, int32_t gj, int32_t gk,
+                                                       int comp, float s[8]) {
+    if constexpr (STORE == PT_STORE_F32_PLANAR) {
+        const int64_t nn = fv.n[0] * fv.n[1] * fv.n[2];
+        const float *F = static_cast<const float *>(fv.data) + comp * nn;
+#pragma unroll
+        for (int ci = 0; ci < 8; ++ci) {
+            const int64_t i = gi + (ci & 1), j = gj + ((ci >> 1) & 1), k = gk + (ci >> 2);
+            s[ci] = F[i + fv.n[0] * (j + fv.n[1] * k)];
+        }
+    } else {
+        const int32_t nb0 = static_cast<int32_t>(fv.nb[0]), nb1 = static_cast<int32_t>(fv.nb[1]);
+        const int32_t base = static_cast<int32_t>(brick_node_index(fv.nb, gi, gj, gk));
+        const int32_t sx = (gi & 7) == 7 ? 512 - 7 : 1;
+        const int32_t sy = (gj & 7) == 7 ? nb0 * 512 - 56 : 8;
+        const int32_t sz = (gk & 7) == 7 ? nb0 * nb1 * 512 - 448 : 64;
+#pragma unroll
+        for (int ci = 0; ci < 8; ++ci) {
+            const int64_t rec = base + ((ci & 1) ? sx : 0) + ((ci & 2) ? sy : 0) + ((ci & 4) ? sz : 0);
+            if constexpr (STORE == PT_STORE_F32_BRICK) {
+                s[ci] = static_cast<const float *>(fv.data)[rec * 8 + comp];
+            } else {
+                const uint32_t w = static_cast<const uint32_t *>(fv.data)[rec * 4 + (comp >> 1)];
+                const uint32_t h = (comp & 1) ? (w >> 16) : (w & 0xffffu);
+                if constexpr (STORE == PT_STORE_BF16_BRICK)
+                    s[ci] = __uint_as_float(h << 16);
+                else
+                    s[ci] = f16_bits_to_float(static_cast<uint16_t>(h));
+            }
+        }
+    }
+}
+
+// Wave-cooperative coefficient reload.  A lane that changed cell posts its
+// gather cell in the wave's LDS area; six helper lanes per posted cell (one
+// per field component) load that component's 8 corners and compute its 8
+// polynomial coefficients; the lane then reads its 48 coefficients back.
+// Up to kReloadBatch cells are served per pass, so a wave pays one pass when
+// a few of its lanes change cell, instead of every lane's share of a full
+// per-lane reload.  Must be called with all 64 lanes of the wave active.
+constexpr int kReloadBatch = 10;
+struct ReloadLds {
+    int4 slot[kReloadBatch];
+    float coef[kReloadBatch][6][8];
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct ReloadLane {
+    int lane, entry, comp;  // helper role: posted cell `entry`, component `comp`
+    float scale, offset;    // decode of `comp` times its physical factor
+};
+
+__device__ __forceinline__ ReloadLane reload_lane(const FieldView &fv, const Phys &ph) {
+    ReloadLane r;
+    r.lane = static_cast<int>(threadIdx.x & 63);
+    r.entry = r.lane / 6;
+    r.comp = r.lane - 6 * r.entry;
+    r.scale = fv.scale[0] * ph.fac[0];
+    r.offset = fv.offset[0] * ph.fac[0];
+#pragma unroll
+    for (int c = 1; c < 6; ++c) {
+        if (r.comp == c) {
+            r.scale = fv.scale[c] * ph.fac[c];
+            r.offset = fv.offset[c] * ph.fac[c];
+        }
+    }
+    return r;
+}
+
+// Per-lane reload of the coefficients of the particle's own cell, all loads
+// in flight at once (used before the step loop, where every lane reloads).
+template <int STORE>
+__device__ __forceinline__ void lane_reload(const int32_t cell[3], const FieldView &fv, const SplitGeom &g,
+                                            const Phys &ph, float a[6][8]) {
+    float b[3];
+    const int32_t gi = gather_cell(cell[0], g.nc[0], b[0]);
+    const int32_t gj = gather_cell(cell[1], g.nc[1], b[1]);
+    const int32_t gk = gather_cell(cell[2], g.nc[2], b[2]);
+    float sv[6][8];
+    load_corners<STORE>(fv, gi, gj, gk, sv);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) coeffs(sv[c], fv.scale[c] * ph.fac[c], fv.offset[c] * ph.fac[c], a[c]);
+    if (b[0] != 0.0f || b[1] != 0.0f || b[2] != 0.0f) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) shift_coeffs(a[c], b[0], b[1], b[2]);
+    }
+}
+
+template <int STORE>
+__device__ __forceinline__ void coop_reload(bool &need, const int32_t cell[3], const FieldView &fv,
+                                            const SplitGeom &g, const ReloadLane &rl, ReloadLds *lds,
+                                            float a[6][8]) {
+    uint64_t pending = __ballot(need);
+    while (pending) {
+        const int rank = __popcll(pending & ((uint64_t{1} << rl.lane) - 1));
+        const bool mine = need && rank < kReloadBatch;
+        const int m = __popcll(pending) < kReloadBatch ? __popcll(pending) : kReloadBatch;
+        if (mine) {
+            float b[3];
+            const int32_t gi = gather_cell(cell[0], g.nc[0], b[0]);
+            const int32_t gj = gather_cell(cell[1], g.nc[1], b[1]);
+            const int32_t gk = gather_cell(cell[2], g.nc[2], b[2]);
+            // a live particle is at most one cell past the last gather cell (x == L)
+            const int bias = (b[0] != 0.0f ? 1 : 0) | (b[1] != 0.0f ? 2 : 0) | (b[2] != 0.0f ? 4 : 0);
+            lds->slot[rank] = make_int4(gi, gj, gk, bias);
+        }
+        wave_lds_sync();
+        if (rl.entry < m) {
+            const int4 t = lds->slot[rl.entry];
+            float sv[8], c8[8];
+            load_component_corners<STORE>(fv, t.x, t.y, t.z, rl.comp, sv);
+            coeffs(sv, rl.scale, rl.offset, c8);
+            if (t.w) shift_coeffs(c8, (t.w & 1) ? 1.0f : 0.0f, (t.w & 2) ? 1.0f : 0.0f, (t.w & 4) ? 1.0f : 0.0f);
+            float4 *dst = reinterpret_cast<float4 *>(lds->coef[rl.entry][rl.comp]);
+            dst[0] = make_float4(c8[0], c8[1], c8[2], c8[3]);
+            dst[1] = make_float4(c8[4], c8[5], c8[6], c8[7]);
+        }
+        wave_lds_sync();
+        if (mine) {
+            const float4 *src = reinterpret_cast<const float4 *>(lds->coef[rank]);
+#pragma unroll
+            for (int q = 0; q < 12; ++q) {
+                const float4 v = src[q];
+                float *d = &a[q >> 1][(q & 1) * 4];
+                d[0] = v.x;
+                d[1] = v.y;
+                d[2] = v.z;
+                d[3] = v.w;
+            }
+            need = false;
+        }
+        pending &= ~__ballot(mine);
+        wave_lds_sync();  // the next pass overwrites the slots and coefficients
+    }
+}
+
 // Cadence of one launch: steps si0 .. si0+K-1; a save happens where
 // (si-1) % stride == 0.  phase0 = (si0-1) % stride and slot0 = the index of
 // the first save at or after si0 are computed once on the host, so the loop
@@ -335,18 +479,24 @@ __host__ inline Cadence make_cadence(int64_t si0, int K, int64_t total_steps, in
 // off, U, P2, v) is called at every cadence step (before the push, as the
 // reference's *_step copies); die(si) when an open boundary removes the
 // particle.
+//
+// All 64 lanes of the wave must run the loop (lanes without a particle with
+// alive = false): the coefficient reload is wave-cooperative (coop_reload)
+// and uses the wave's `lds` area.
 template <int STORE, class SaveFn, class DieFn>
 __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[3], double &P2, bool &alive,
                                          const FieldView &fv, const SplitGeom &g, const Phys &ph,
-                                         const Cadence &cad, int periodic, SaveFn save, DieFn die) {
+                                         const Cadence &cad, int periodic, ReloadLds *lds, SaveFn save, DieFn die) {
     float a[6][8];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
         for (int q = 0; q < 8; ++q) a[c][q] = 0.0f;
-    bool reload = true;
+    if (alive) lane_reload<STORE>(cell, fv, g, ph, a);
+    bool reload = false;
     float onep2 = static_cast<float>(1.0 + P2);
     const int32_t si_mod4 = static_cast<int32_t>(cad.si0 & 3);
+    const ReloadLane rl = reload_lane(fv, ph);
     int32_t phase = cad.phase0;
     int64_t slot = cad.slot0;
     // pushes happen while si < total_steps: at most n_push of the K steps
@@ -356,22 +506,9 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
         const bool do_save = cad.want_save && phase == 0;
         phase = phase + 1 == cad.stride ? 0 : phase + 1;
         const bool push = alive && k < n_push;
+        reload = reload && alive;
+        coop_reload<STORE>(reload, cell, fv, g, rl, lds, a);  // every lane of the wave, converged here
         if (!do_save && !push) continue;
-        if (alive && reload) {
-            float b[3];
-            const int32_t gi = gather_cell(cell[0], g.nc[0], b[0]);
-            const int32_t gj = gather_cell(cell[1], g.nc[1], b[1]);
-            const int32_t gk = gather_cell(cell[2], g.nc[2], b[2]);
-            float sv[6][8];
-            load_corners<STORE>(fv, gi, gj, gk, sv);
-#pragma unroll
-            for (int c = 0; c < 6; ++c) coeffs(sv[c], fv.scale[c] * ph.fac[c], fv.offset[c] * ph.fac[c], a[c]);
-            if (b[0] != 0.0f || b[1] != 0.0f || b[2] != 0.0f) {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) shift_coeffs(a[c], b[0], b[1], b[2]);
-            }
-            reload = false;
-        }
         // dead lanes evaluate stale coefficients only to keep the loop
         // uniform; save() writes NaN for them
         float v[6];

@@ -112,27 +112,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
 k_push_fast(SplitState s, uint8_t *__restrict__ alive_arr, FieldView fv, SplitGeom g, Phys ph,
             int64_t n, Cadence cad, int out_kind, int out_f64, void *out_a, void *out_b, int periodic,
             int64_t first_step, unsigned long long *deaths, const int32_t *__restrict__ perm) {
+    __shared__ ReloadLds lds[4];  // one per wave of the 256-lane block
     const int64_t nblk = gridDim.x;
     const int64_t p = xcd_block(blockIdx.x, nblk) * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int64_t col = perm ? perm[p] : p;
-    bool alive = alive_arr[p] != 0;
+    // Lanes past the last particle stay in the loop as dead helpers of the
+    // wave-cooperative reload; a wave without any particle leaves at once.
+    const bool valid = p < n;
+    if (__ballot(valid) == 0) return;
+    const int64_t q = valid ? p : 0;  // in-range index for their loads
+    const int64_t col = perm ? perm[q] : q;
+    bool alive = valid && alive_arr[q] != 0;
     int32_t cell[3];
     float off[3], U[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        cell[c] = s.cell[c][p];
-        off[c] = s.off[c][p];
-        U[c] = s.P[c][p] * ph.mr;
+        cell[c] = s.cell[c][q];
+        off[c] = s.off[c][q];
+        U[c] = s.P[c][q] * ph.mr;
     }
     const double mr2 = static_cast<double>(ph.mr) * static_cast<double>(ph.mr);
-    double P2 = s.P2[p] * mr2;
+    double P2 = s.P2[q] * mr2;
     run_lane<STORE>(
-        cell, off, U, P2, alive, fv, g, ph, cad, periodic,
+        cell, off, U, P2, alive, fv, g, ph, cad, periodic, &lds[threadIdx.x >> 6],
         [&](int64_t slot, bool alv, const int32_t *ce, const float *of, const float *u, double, const float *v) {
-            write_save(out_kind, out_f64, out_a, out_b, slot, n, col, alv, ce, of, u, v, g, ph);
+            if (valid) write_save(out_kind, out_f64, out_a, out_b, slot, n, col, alv, ce, of, u, v, g, ph);
         },
         [&](int64_t si) { atomicAdd(&deaths[si - first_step], 1ull); });
+    if (!valid) return;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         s.cell[c][p] = cell[c];

@@ -6,10 +6,12 @@
 // per tick: the grid is the concatenation of every request's 256-particle
 // blocks, each block finds its request in a device table (binary search on
 // the block prefix) and runs the production step loop (core::run_lane) with
-// that request's own field, dt, charge, mass, duration and cadence.  A tick
-// is enqueued without waiting for it; the next pt_serve_step completes it,
-// retires finished requests and admits queued ones, so requests arriving
-// while others run join at the next tick and share the device.
+// that request's own field, dt, charge, mass, duration and cadence.  Up to
+// two ticks are in flight: pt_serve_step completes the older one when two are
+// enqueued (retiring the requests whose last steps it carried), admits queued
+// requests and enqueues the next tick, so the device always has work while
+// the host runs; a step with nothing new to enqueue completes everything in
+// flight.  Requests arriving while others run join at the next tick.
 //
 // The tick launch is an ordinary kernel rather than a spin-waiting resident
 // kernel: the host changes the request mix between ticks, and no kernel ever
@@ -20,6 +22,7 @@
 #include <deque>
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "pt_push_core.hpp"
@@ -61,31 +64,46 @@ k_serve(const DevReq *__restrict__ reqs, int nreq) {
     const DevReq &R = reqs[lo];
     const int64_t p = (b - R.block0) * blockDim.x + threadIdx.x;
     const int64_t n = R.n;
-    if (p >= n) return;
-    bool alive = R.alive[p] != 0;
+    // lanes past the request's last particle stay as dead helpers of the
+    // wave-cooperative reload; a wave without any particle leaves at once
+    const bool valid = p < n;
+    if (__ballot(valid) == 0) return;
+    __shared__ ReloadLds lds[4];
+    const int64_t q = valid ? p : 0;
+    bool alive = valid && R.alive[q] != 0;
     int32_t cell[3];
     float off[3], U[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        cell[c] = R.cell[c][p];
-        off[c] = R.off[c][p];
-        U[c] = R.U[c][p];
+        cell[c] = R.cell[c][q];
+        off[c] = R.off[c][q];
+        U[c] = R.U[c][q];
     }
-    double P2 = R.P2[p];
+    double P2 = R.P2[q];
     const int kind = R.out_kind;
     void *out = R.out;
     const float qsign = R.ph.qsign;
     const double pout = R.ph.pout;
+    // The request's parameters as values: read through R, the loop's stores
+    // (cache, death counter) could alias the table and force reloads each push.
+    const FieldView fv = R.fv;
+    const SplitGeom g = R.g;
+    const Phys ph = R.ph;
+    const Cadence cad = R.cad;
+    const int periodic = R.periodic;
+    unsigned long long *const deaths = R.deaths;
     run_lane<STORE>(
-        cell, off, U, P2, alive, R.fv, R.g, R.ph, R.cad, R.periodic,
+        cell, off, U, P2, alive, fv, g, ph, cad, periodic, &lds[threadIdx.x >> 6],
         [&](int64_t slot, bool alv, const int32_t *, const float *, const float *u, double p2, const float *v) {
+            if (!valid) return;
             if (kind == PT_OUT_MU) {
                 static_cast<float *>(out)[slot * n + p] = alv ? mu_f32(u[0], u[1], u[2], v[0], v[1], v[2], qsign) : NAN;
             } else {
                 static_cast<double *>(out)[slot * n + p] = alv ? sqrt(p2) * pout : NAN;
             }
         },
-        [&](int64_t) { atomicAdd(R.deaths, 1ull); });
+        [&](int64_t) { atomicAdd(deaths, 1ull); });
+    if (!valid) return;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         R.cell[c][p] = cell[c];
@@ -193,6 +211,15 @@ int guarded(F &&fn) {
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
+// One launch of k_serve: its request table (pinned host copy and device
+// copy), events and the work it carries (request id, first step, steps).
+struct Tick {
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevReq *d_table = nullptr, *h_table = nullptr;
+    int cap = 0;
+    std::vector<std::tuple<int64_t, int64_t, int64_t>> work;
+};
+
 struct SReq {
     int64_t id = 0;
     int state = PT_REQ_QUEUED;
@@ -200,7 +227,8 @@ struct SReq {
     std::vector<double> init;
     char *mem = nullptr;
     size_t bytes = 0;
-    int64_t si = 1;
+    int64_t si = 1;          // next step not yet completed on the device
+    int64_t si_planned = 1;  // next step not yet enqueued
     int64_t nsave = 0;
     int64_t alive_final = 0;
     bool failed = false;
@@ -221,11 +249,11 @@ struct pt_serve {
     std::deque<int64_t> queue;
     std::vector<int64_t> active;
     int64_t reserved = 0, next_id = 1, n_done = 0;
-    pt::DevReq *d_table = nullptr, *h_table = nullptr;
-    int table_cap = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool pending = false;
-    std::vector<std::pair<int64_t, int64_t>> tick_work;  // (request id, steps)
+    // Two ticks in flight: step() enqueues the next tick before waiting for
+    // the previous one, so the device always has a tick queued while the
+    // host admits, completes and hands out requests.
+    pt::Tick tick[2];
+    std::deque<int> inflight;  // slots of enqueued, not yet completed ticks, oldest first
     int64_t particle_steps = 0;
     double kernel_ms = 0.0;
 };
@@ -331,80 +359,91 @@ void admit(pt_serve *srv, SReq &r) {
     r.state = PT_REQ_RUNNING;
 }
 
-void complete_tick(pt_serve *srv) {
-    if (!srv->pending) return;
-    PT_HIP_CHECK(hipEventSynchronize(srv->ev1));
+// Wait for the oldest in-flight tick and retire its work: requests whose
+// last step it carried are done.
+void complete_oldest(pt_serve *srv) {
+    if (srv->inflight.empty()) return;
+    Tick &t = srv->tick[srv->inflight.front()];
+    PT_HIP_CHECK(hipEventSynchronize(t.ev1));
     float ms = 0.f;
-    PT_HIP_CHECK(hipEventElapsedTime(&ms, srv->ev0, srv->ev1));
+    PT_HIP_CHECK(hipEventElapsedTime(&ms, t.ev0, t.ev1));
     srv->kernel_ms += ms;
-    srv->pending = false;
-    std::vector<int64_t> still;
-    for (auto &w : srv->tick_work) {
-        SReq &r = srv->reqs.at(w.first);
-        const int64_t pushes = std::max<int64_t>(0, std::min(w.second, r.d.total_steps - r.si));
+    srv->inflight.pop_front();
+    for (const auto &w : t.work) {
+        SReq &r = srv->reqs.at(std::get<0>(w));
+        const int64_t si0 = std::get<1>(w), K = std::get<2>(w);
+        const int64_t pushes = std::max<int64_t>(0, std::min(K, r.d.total_steps - si0));
         srv->particle_steps += pushes * r.d.n;
-        r.si += w.second;
-    }
-    for (int64_t id : srv->active) {
-        SReq &r = srv->reqs.at(id);
+        r.si = si0 + K;
         if (r.si > r.d.total_steps) {
             unsigned long long deaths = 0;
             PT_HIP_CHECK(hipMemcpy(&deaths, r.dev.deaths, 8, hipMemcpyDeviceToHost));
             r.alive_final = r.d.n - static_cast<int64_t>(deaths);
             r.state = PT_REQ_DONE;
             srv->n_done += 1;
-        } else {
-            still.push_back(id);
+            srv->active.erase(std::remove(srv->active.begin(), srv->active.end(), r.id), srv->active.end());
         }
     }
-    srv->active.swap(still);
-    srv->tick_work.clear();
+    t.work.clear();
+}
+
+void complete_all(pt_serve *srv) {
+    while (!srv->inflight.empty()) complete_oldest(srv);
 }
 
 template <int STORE>
-void launch_tick(pt_serve *srv, int nreq, int64_t blocks) {
-    k_serve<STORE><<<static_cast<unsigned>(blocks), 256, 0, srv->ctx->stream>>>(srv->d_table, nreq);
+void launch_tick(pt_serve *srv, const DevReq *table, int nreq, int64_t blocks) {
+    k_serve<STORE><<<static_cast<unsigned>(blocks), 256, 0, srv->ctx->stream>>>(table, nreq);
     PT_HIP_CHECK(hipGetLastError());
 }
 
+// Enqueue one tick for every running request with steps not yet enqueued,
+// in a free slot (the caller has completed its previous tick).
 void enqueue_tick(pt_serve *srv) {
-    const int nreq = static_cast<int>(srv->active.size());
+    std::vector<SReq *> work;
+    for (int64_t id : srv->active) {
+        SReq &r = srv->reqs.at(id);
+        if (r.si_planned <= r.d.total_steps) work.push_back(&r);
+    }
+    const int nreq = static_cast<int>(work.size());
     if (nreq == 0) return;
-    if (srv->table_cap < nreq) {
-        PT_HIP_CHECK(hipStreamSynchronize(srv->ctx->stream));
-        if (srv->d_table) (void)hipFree(srv->d_table);
-        if (srv->h_table) (void)hipHostFree(srv->h_table);
-        srv->d_table = nullptr;
-        srv->h_table = nullptr;
-        const int cap = std::max(nreq, 2 * srv->table_cap);
-        PT_HIP_CHECK(hipMalloc(&srv->d_table, sizeof(DevReq) * cap));
-        PT_HIP_CHECK(hipHostMalloc(&srv->h_table, sizeof(DevReq) * cap, hipHostMallocDefault));
-        srv->table_cap = cap;
+    const int slot = srv->inflight.empty() ? 0 : 1 - srv->inflight.back();
+    Tick &t = srv->tick[slot];
+    if (t.cap < nreq) {
+        if (t.d_table) (void)hipFree(t.d_table);
+        if (t.h_table) (void)hipHostFree(t.h_table);
+        t.d_table = nullptr;
+        t.h_table = nullptr;
+        const int cap = std::max(nreq, 2 * t.cap);
+        PT_HIP_CHECK(hipMalloc(&t.d_table, sizeof(DevReq) * cap));
+        PT_HIP_CHECK(hipHostMalloc(&t.h_table, sizeof(DevReq) * cap, hipHostMallocDefault));
+        t.cap = cap;
     }
     int64_t blocks = 0;
-    srv->tick_work.clear();
+    t.work.clear();
     for (int i = 0; i < nreq; ++i) {
-        SReq &r = srv->reqs.at(srv->active[static_cast<size_t>(i)]);
-        const int64_t K = std::min<int64_t>(srv->steps_per_tick, r.d.total_steps - r.si + 1);
-        r.dev.cad = make_cadence(r.si, static_cast<int>(K), r.d.total_steps, r.d.save_stride,
+        SReq &r = *work[static_cast<size_t>(i)];
+        const int64_t K = std::min<int64_t>(srv->steps_per_tick, r.d.total_steps - r.si_planned + 1);
+        r.dev.cad = make_cadence(r.si_planned, static_cast<int>(K), r.d.total_steps, r.d.save_stride,
                                  r.d.out_kind != PT_OUT_NONE, 0);
         r.dev.block0 = blocks;
         blocks += (r.d.n + 255) / 256;
-        srv->h_table[i] = r.dev;
-        srv->tick_work.emplace_back(r.id, K);
+        t.h_table[i] = r.dev;
+        t.work.emplace_back(r.id, r.si_planned, K);
+        r.si_planned += K;
     }
     PT_REQUIRE(blocks < (1LL << 31), "pt_serve_step: tick grid too large");
     hipStream_t st = srv->ctx->stream;
-    PT_HIP_CHECK(hipEventRecord(srv->ev0, st));
-    PT_HIP_CHECK(hipMemcpyAsync(srv->d_table, srv->h_table, sizeof(DevReq) * nreq, hipMemcpyHostToDevice, st));
+    PT_HIP_CHECK(hipEventRecord(t.ev0, st));
+    PT_HIP_CHECK(hipMemcpyAsync(t.d_table, t.h_table, sizeof(DevReq) * nreq, hipMemcpyHostToDevice, st));
     switch (srv->storage) {
-        case PT_STORE_F32_BRICK: launch_tick<PT_STORE_F32_BRICK>(srv, nreq, blocks); break;
-        case PT_STORE_BF16_BRICK: launch_tick<PT_STORE_BF16_BRICK>(srv, nreq, blocks); break;
-        case PT_STORE_F16_BRICK: launch_tick<PT_STORE_F16_BRICK>(srv, nreq, blocks); break;
+        case PT_STORE_F32_BRICK: launch_tick<PT_STORE_F32_BRICK>(srv, t.d_table, nreq, blocks); break;
+        case PT_STORE_BF16_BRICK: launch_tick<PT_STORE_BF16_BRICK>(srv, t.d_table, nreq, blocks); break;
+        case PT_STORE_F16_BRICK: launch_tick<PT_STORE_F16_BRICK>(srv, t.d_table, nreq, blocks); break;
         default: throw Error{PT_ERR_STATE, "server storage not set"};
     }
-    PT_HIP_CHECK(hipEventRecord(srv->ev1, st));
-    srv->pending = true;
+    PT_HIP_CHECK(hipEventRecord(t.ev1, st));
+    srv->inflight.push_back(slot);
 }
 
 }  // namespace
@@ -427,8 +466,10 @@ int pt_serve_create(pt_ctx *ctx, int64_t hbm_budget_bytes, int32_t steps_per_tic
         s->budget = hbm_budget_bytes;
         s->steps_per_tick = steps_per_tick;
         s->max_active = max_active;
-        PT_HIP_CHECK(hipEventCreate(&s->ev0));
-        PT_HIP_CHECK(hipEventCreate(&s->ev1));
+        for (auto &t : s->tick) {
+            PT_HIP_CHECK(hipEventCreate(&t.ev0));
+            PT_HIP_CHECK(hipEventCreate(&t.ev1));
+        }
         *out = s;
     });
 }
@@ -437,7 +478,7 @@ int pt_serve_destroy(pt_serve *srv) {
     return guarded([&] {
         if (!srv) return;
         try {
-            complete_tick(srv);
+            complete_all(srv);
         } catch (...) {
         }
         hipStream_t st = srv->ctx->stream;
@@ -447,10 +488,12 @@ int pt_serve_destroy(pt_serve *srv) {
                 srv->ctx->bytes_in_use -= static_cast<int64_t>(kv.second.bytes);
             }
         (void)hipStreamSynchronize(st);
-        if (srv->d_table) (void)hipFree(srv->d_table);
-        if (srv->h_table) (void)hipHostFree(srv->h_table);
-        (void)hipEventDestroy(srv->ev0);
-        (void)hipEventDestroy(srv->ev1);
+        for (auto &t : srv->tick) {
+            if (t.d_table) (void)hipFree(t.d_table);
+            if (t.h_table) (void)hipHostFree(t.h_table);
+            if (t.ev0) (void)hipEventDestroy(t.ev0);
+            if (t.ev1) (void)hipEventDestroy(t.ev1);
+        }
         delete srv;
     });
 }
@@ -531,7 +574,8 @@ int pt_serve_step(pt_serve *srv, int64_t *n_running, int64_t *n_queued, int64_t 
     return guarded([&] {
         PT_REQUIRE(srv, "pt_serve_step: null server");
         PT_HIP_CHECK(hipSetDevice(srv->ctx->device));
-        complete_tick(srv);
+        // keep at most one tick in flight while this call enqueues the next
+        while (srv->inflight.size() > 1) complete_oldest(srv);
         while (!srv->queue.empty() && static_cast<int32_t>(srv->active.size()) < srv->max_active) {
             SReq &r = srv->reqs.at(srv->queue.front());
             if (srv->reserved + static_cast<int64_t>(r.bytes) > srv->budget) break;  // FIFO: wait for memory
@@ -544,7 +588,10 @@ int pt_serve_step(pt_serve *srv, int64_t *n_running, int64_t *n_queued, int64_t 
                 srv->active.push_back(r.id);
             }
         }
+        const size_t before = srv->inflight.size();
         enqueue_tick(srv);
+        // nothing new to run: finish what is in flight, so its requests are done
+        if (srv->inflight.size() == before) complete_all(srv);
         if (n_running) *n_running = static_cast<int64_t>(srv->active.size());
         if (n_queued) *n_queued = static_cast<int64_t>(srv->queue.size());
         if (n_done) *n_done = srv->n_done;
@@ -554,7 +601,7 @@ int pt_serve_step(pt_serve *srv, int64_t *n_running, int64_t *n_queued, int64_t 
 int pt_serve_drain(pt_serve *srv) {
     return guarded([&] {
         PT_REQUIRE(srv, "pt_serve_drain: null server");
-        complete_tick(srv);
+        complete_all(srv);
     });
 }
 

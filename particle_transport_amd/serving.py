@@ -227,55 +227,77 @@ def submit_request(server, req, L):
                          out_kind=req.out_kind, mu0=req.mu0, phase=phase, p_magnitude=pmag)
 
 
-def finish_request(server, stats_ctx, rid, req):
-    """The request's product: D_mumu (64 log lags x 20 |mu| bins) or D_pp."""
-    cp, nsave, n, alive = server.fetch(rid)
+def request_product(stats_ctx, cache_ptr, nsave, n, req):
+    """The request's product from its device cache: D_mumu (64 log lags x 20
+    |mu| bins) or D_pp."""
     lags = log_lag_steps(nsave, 64)
     if len(lags) == 0:
         return None
     if req.out_kind == OUT_MU:
-        return dmumu_on_device(stats_ctx, cp, nsave, n, lags, np.linspace(0.0, 1.0, 21))
+        return dmumu_on_device(stats_ctx, cache_ptr, nsave, n, lags, np.linspace(0.0, 1.0, 21))
     pmag = req.gamma * M_P * C_LIGHT * math.sqrt(1 - 1 / req.gamma ** 2)
-    return dpp_on_device(stats_ctx, cp, nsave, n, lags, pmag)
+    return dpp_on_device(stats_ctx, cache_ptr, nsave, n, lags, pmag)
+
+
+def finish_request(server, stats_ctx, rid, req):
+    """request_product of a finished request of `server`."""
+    cp, nsave, n, alive = server.fetch(rid)
+    return request_product(stats_ctx, cp, nsave, n, req)
 
 
 def run_stream(push_ctx, stats_ctx, fields, reqs, *, budget_bytes, steps_per_tick=200, max_active=1024,
                box_length_m, time_scale=1.0, keep_results=False, log=None):
-    """Replay the stream in real time (arrival times x time_scale)."""
+    """Replay the stream in real time (arrival times x time_scale).  The loop
+    keeps the server stepping; each finished request's product is computed on
+    `stats_ctx` by a worker thread (the bindings release the GIL), and the
+    request is released once its product is done.  Latency = arrival to
+    product."""
+    from concurrent.futures import ThreadPoolExecutor
+
     srv = Server(push_ctx, budget_bytes, steps_per_tick, max_active)
     fids = [srv.add_field(f) for f in fields]
     assert fids == list(range(len(fields)))
     pending = sorted(reqs, key=lambda r: r.arrival)
     outstanding = {}
+    finishing = {}
     results = {}
     lat = []
+    pool = ThreadPoolExecutor(max_workers=1)
     t0 = time.perf_counter()
     k = 0
+    last_log = 0.0
     try:
-        while k < len(pending) or outstanding:
+        while k < len(pending) or outstanding or finishing:
             now = time.perf_counter() - t0
             while k < len(pending) and pending[k].arrival * time_scale <= now:
                 rq = pending[k]
-                rid = submit_request(srv, rq, box_length_m)
-                outstanding[rid] = rq
+                outstanding[submit_request(srv, rq, box_length_m)] = rq
                 k += 1
             running, queued, done = srv.step()
             for rid in [r for r in outstanding if srv.poll(r) == DONE]:
                 rq = outstanding.pop(rid)
-                res = finish_request(srv, stats_ctx, rid, rq)
-                end = time.perf_counter() - t0
-                lat.append(end - rq.arrival * time_scale)
+                cp, nsave, n, _ = srv.fetch(rid)
+                finishing[rid] = (pool.submit(request_product, stats_ctx, cp, nsave, n, rq), rq)
+            for rid in [r for r, (f, _) in finishing.items() if f.done()]:
+                fut, rq = finishing.pop(rid)
+                res = fut.result()
+                lat.append(time.perf_counter() - t0 - rq.arrival * time_scale)
                 if keep_results:
                     results[rq.i] = (res, srv.copy_cache(rid, np.float32 if rq.out_kind == OUT_MU else np.float64))
                 srv.release(rid)
-            if log and k % 50 == 0:
+            if log and now - last_log >= 1.0:
+                last_log = now
                 log(f"[serve] t={now:.1f}s submitted={k} running={running} queued={queued} done={done}")
-            if running == 0 and queued == 0 and not outstanding and k < len(pending):
-                time.sleep(max(0.0, pending[k].arrival * time_scale - (time.perf_counter() - t0)))
+            if running == 0 and queued == 0 and not outstanding:
+                if finishing:
+                    time.sleep(0.0005)
+                elif k < len(pending):
+                    time.sleep(max(0.0, pending[k].arrival * time_scale - (time.perf_counter() - t0)))
         srv.drain()
         wall = time.perf_counter() - t0
         st = srv.stats()
     finally:
+        pool.shutdown(wait=True)
         srv.close()
     lat = np.asarray(lat)
     return SimpleNamespace(latency_p50_s=float(np.percentile(lat, 50)), latency_p99_s=float(np.percentile(lat, 99)),

@@ -115,6 +115,35 @@ def test_fp32_paths_track_fp64_oracle(ctx, pusher, storage):
     assert alive.all()
 
 
+@pytest.mark.parametrize("periodic", [True, False])
+def test_planar_and_brick_fp32_fields_give_identical_trajectories(ctx, periodic):
+    """FP32 planar and FP32 brick storage hold the same values and the
+    coefficient reload computes from them identically, so the production
+    pusher's outputs must be bit-identical.  At 1e5 GeV particles cross cells
+    every few pushes: almost every push of every wave runs the
+    wave-cooperative reload with several lanes at once (n = 1000 also leaves
+    a partial last wave whose idle lanes serve as helpers)."""
+    F, x, p, dt, g = make_case(n=1000, energy_gev=1e5, steps_per_gp=50)
+    outs = []
+    for storage in ("f32_planar", "f32_brick"):
+        fld = Field(ctx, F, L_BOX, storage)
+        ens = Ensemble(ctx, fld, x, p, pusher="fast", periodic=periodic)
+        r = ens.advance(dt=dt, first_step=1, n_steps=600, total_steps=600, stride=3, out_kind="phase",
+                        steps_per_launch=128)
+        outs.append((r.out_a, r.out_b, r.alive_counts))
+        ens.close()
+        fld.close()
+    (pa, ma, ca), (pb, mb, cb) = outs
+    assert np.array_equal(ca, cb)
+    assert np.array_equal(pa, pb, equal_nan=True)
+    assert np.array_equal(ma, mb, equal_nan=True)
+    o = oracle.run(F, g, x.copy(), p.copy(), dt, 1, 600, 600, 3, periodic, 1)
+    live = np.isfinite(o["pos"]) & np.isfinite(pa)
+    assert np.max(np.abs(pa[live] - o["pos"][live]) / g.dx) <= 1e-3  # FP32 path vs FP64 over 600 pushes
+    if not periodic:
+        assert ca[-1] < 1000
+
+
 @pytest.mark.parametrize("storage,tol", [("bf16_brick", 2e-2), ("f16_brick", 2e-3)])
 def test_16bit_fields_mu_close_to_fp64(ctx, storage, tol):
     """BF16/FP16 brick fields (per-component offset + scale) vs FP64 fields:
