@@ -187,10 +187,10 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         const int64_t s_end = injection ? (lag_min < nsave ? 1 : 0) : (nsave - lag_min);
         const int64_t s_full = nsave - lag_max;  // below it every lag of the group is in range
 
-        // One start sample s: bin of mu[s] (a run boundary only when it
-        // changes), then the pair with every lag, without branches.
-        auto visit = [&](int64_t s, auto checked) {
-            const float a = row[0];
+        // One start sample: bin of mu[s] (a run boundary only when it
+        // changes), then the pair with every lag, without branches.  `b[g]`
+        // holds mu[s + lag[g]] (any value where `valid[g]` is false).
+        auto visit = [&](float a, const float *b, const bool *valid) {
             if (!isfinite(a)) return;
             const float v = bin_abs ? fabsf(a) : a;
             if (!(v >= cur_lo && v < cur_hi)) {
@@ -220,20 +220,47 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
             const double ad = static_cast<double>(a);
 #pragma unroll
             for (int g = 0; g < kG; ++g) {
-                if constexpr (decltype(checked)::value) {
-                    if (s + lag[g] >= nsave) continue;
-                }
-                const float b = row[off[g]];
-                const bool ok = isfinite(b);
-                const double d = ok ? static_cast<double>(b) - ad : 0.0;
+                const bool ok = valid[g] && isfinite(b[g]);
+                const double d = ok ? static_cast<double>(b[g]) - ad : 0.0;
                 rc[g] += ok ? 1u : 0u;
                 r1[g] += d;
                 r2[g] = fma(d, d, r2[g]);
             }
         };
+        const bool all[kG] = {true, true, true, true, true, true, true, true};
         int64_t s = 0;
-        for (; s < s_end && s < s_full; ++s, row += ss) visit(s, std::false_type{});
-        for (; s < s_end; ++s, row += ss) visit(s, std::true_type{});
+        // Every lag in range: the loads of sample s+1 are issued before the
+        // pairs of sample s are accumulated, so their latency overlaps it.
+        const int64_t s_main = s_end < s_full ? s_end : s_full;
+        if (s_main > 0) {
+            float an = row[0], bn[kG];
+#pragma unroll
+            for (int g = 0; g < kG; ++g) bn[g] = row[off[g]];
+            for (; s < s_main; ++s) {
+                const float a = an;
+                float b[kG];
+#pragma unroll
+                for (int g = 0; g < kG; ++g) b[g] = bn[g];
+                row += ss;
+                if (s + 1 < s_main) {
+                    an = row[0];
+#pragma unroll
+                    for (int g = 0; g < kG; ++g) bn[g] = row[off[g]];
+                }
+                visit(a, b, all);
+            }
+        }
+        for (; s < s_end; ++s, row += ss) {
+            float b[kG];
+            bool valid[kG];
+#pragma unroll
+            for (int g = 0; g < kG; ++g) {
+                valid[g] = s + lag[g] < nsave;  // uniform
+                b[g] = 0.0f;
+                if (valid[g]) b[g] = row[off[g]];
+            }
+            visit(row[0], b, valid);
+        }
         flush();
     }
 
@@ -353,17 +380,13 @@ k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel
             off[g] = lag[g] * total;
         }
         const int64_t s_end = nsave - lag_min, s_full = nsave - lag_max;
-        auto visit = [&](int64_t s, auto checked) {
-            const double pst = row[0];
+        // `pe[g]` = |p| at s + lag[g] (any value where `valid[g]` is false)
+        auto visit = [&](double pst, const double *pe, const bool *valid) {
             if (!isfinite(pst)) return;
 #pragma unroll
             for (int g = 0; g < kG; ++g) {
-                if constexpr (decltype(checked)::value) {
-                    if (s + lag[g] >= nsave) continue;
-                }
-                const double pe = row[off[g]];
-                const bool ok = isfinite(pe);
-                const double d = ok ? pe - pst : 0.0;
+                const bool ok = valid[g] && isfinite(pe[g]);
+                const double d = ok ? pe[g] - pst : 0.0;
                 const double dn = d * inv_p0;
                 cnt[g] += ok ? 1u : 0u;
                 s1[g] += d;
@@ -372,9 +395,39 @@ k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel
                 s4[g] = fma(dn, dn, s4[g]);
             }
         };
+        const bool all[kG] = {true, true, true, true, true, true, true, true};
         int64_t s = 0;
-        for (; s < s_end && s < s_full; ++s, row += total) visit(s, std::false_type{});
-        for (; s < s_end; ++s, row += total) visit(s, std::true_type{});
+        // every lag in range: sample s+1 is loaded before sample s is used
+        const int64_t s_main = s_end < s_full ? s_end : s_full;
+        if (s_main > 0) {
+            double an = row[0], bn[kG];
+#pragma unroll
+            for (int g = 0; g < kG; ++g) bn[g] = row[off[g]];
+            for (; s < s_main; ++s) {
+                const double a = an;
+                double b[kG];
+#pragma unroll
+                for (int g = 0; g < kG; ++g) b[g] = bn[g];
+                row += total;
+                if (s + 1 < s_main) {
+                    an = row[0];
+#pragma unroll
+                    for (int g = 0; g < kG; ++g) bn[g] = row[off[g]];
+                }
+                visit(a, b, all);
+            }
+        }
+        for (; s < s_end; ++s, row += total) {
+            double b[kG];
+            bool valid[kG];
+#pragma unroll
+            for (int g = 0; g < kG; ++g) {
+                valid[g] = s + lag[g] < nsave;  // uniform
+                b[g] = 0.0;
+                if (valid[g]) b[g] = row[off[g]];
+            }
+            visit(row[0], b, valid);
+        }
     }
 #pragma unroll
     for (int g = 0; g < kG; ++g) {
