@@ -134,6 +134,39 @@ int pt_ens_create(pt_ctx *ctx, const pt_field *field, int pusher, int64_t n, int
                   pt_ens **out);
 int pt_ens_destroy(pt_ens *ens);
 
+/* Injection settings of init_particles (phase_space_gpu_runner.jl:196-245),
+ * drawn on the device.  The reference draws from MersenneTwister(seed), one
+ * particle after another; here every particle draws from its own counters of
+ * Philox4x64-10 (key (seed, 0), counter (particle index, block, 0, 0), four
+ * 64-bit words per block, u = (word >> 11) * 2^-53), in the reference's
+ * per-particle order: three position draws unless the position is fixed, then
+ * z = 2u - 1 and phi = 2 pi u (rand_unit_vector, :61-66) or, for fixed mu,
+ * phi = 2 pi u about the local B (fixed_mu_unit_vector, :170-194).  The state
+ * of particle i does not depend on how the particles are split over calls. */
+typedef struct {
+    uint64_t seed;            /* cfg[:seed] */
+    int32_t fixed_position;   /* injection_position_mode: 0 random, 1 fixed */
+    int32_t fixed_mu;         /* injection_mode: 0 isotropic, 1 fixed_mu */
+    double position_m[3];     /* fixed_injection_position (:145-168), metres */
+    double mu0;               /* injection_mu0, in [-1, 1] */
+    double p_magnitude;       /* pscale = gamma0 * M_P * v0 (:212), kg m/s */
+} pt_inject_params;
+
+/* Replaces the host loop of init_particles (phase_space_gpu_runner.jl:196-245):
+ * writes the FP64 initial state of global particles first_index ..
+ * first_index+n-1 into out = [x1 x2 x3 p1 p2 p3][n] (metres, kg m/s), host or
+ * device memory.  Fixed-mu injection interpolates B (FP64 trilinear,
+ * trilinear_sample :247-276) from the values the field stores and fails with
+ * the reference's error where |B| is zero or non-finite. */
+int pt_inject(pt_ctx *ctx, const pt_field *field, int64_t first_index, int64_t n,
+              const pt_inject_params *params, int out_on_device, double *out);
+
+/* pt_ens_create with the initial state drawn on the device by pt_inject
+ * (first_index 0): init_particles and the state upload of run_gpu_ensemble
+ * (phase_space_gpu_runner.jl:502-513) with no host O(n) work. */
+int pt_ens_create_injected(pt_ctx *ctx, const pt_field *field, int pusher, int64_t n,
+                           const pt_inject_params *params, int periodic, pt_ens **out);
+
 typedef struct {
     double dt_s;          /* dt = trajectory_time.dt_s (phase_space_gpu_runner.jl:498) */
     double charge_C;      /* Q_E for the reference's protons (:9) */

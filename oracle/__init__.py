@@ -21,6 +21,11 @@ Parity pinning of each function (what independent check pins it):
   - ``dpp``             pure-Python brute force on hand-built momentum series.
   - ``energy_snapshots`` the relativistic closed form E = (gamma-1) m c^2.
   - ``mean_abs_b``      the reference's own known answer (3,4,0) -> 5.
+  - ``philox4x64_10``   numpy.random.Philox (independent implementation) on
+                        arbitrary keys and counters.
+  - ``inject``          positions equal numpy Philox doubles mapped onto the
+                        box; fixed-mu gives p.b/(|p||b|) == mu0 and |p| ==
+                        pscale; isotropic moments <u_z> = 0, <u_z^2> = 1/3.
 """
 from __future__ import annotations
 
@@ -95,6 +100,14 @@ def lib():
         L.oracle_energy_snapshots.argtypes = [_fp, _i64, _i64p, _i64, _dp, _i64]
         L.oracle_mean_abs_b.restype = _d
         L.oracle_mean_abs_b.argtypes = [_dp, _dp, _dp, _i64]
+        _u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.oracle_philox4x64_10.restype = None
+        L.oracle_philox4x64_10.argtypes = [_u64p, _u64p, _u64p]
+        L.oracle_inject_draw.restype = _d
+        L.oracle_inject_draw.argtypes = [ctypes.c_uint64, _i64, ctypes.c_int]
+        L.oracle_inject.restype = ctypes.c_int
+        L.oracle_inject.argtypes = [ctypes.c_uint64, _i64, _i64, ctypes.c_int, _dp, ctypes.c_int, _d, _d,
+                                    ctypes.POINTER(_dp), ctypes.POINTER(Geom), _dp]
         _lib = L
     return _lib
 
@@ -248,3 +261,31 @@ def mean_abs_b(bx, by, bz):
     by = np.ascontiguousarray(np.ravel(by, order="F"), dtype=np.float64)
     bz = np.ascontiguousarray(np.ravel(bz, order="F"), dtype=np.float64)
     return lib().oracle_mean_abs_b(_ptr(bx), _ptr(by), _ptr(bz), bx.size)
+
+
+def philox4x64_10(counter, key):
+    """Philox4x64-10 block: four uint64 counter words, two key words."""
+    c = (ctypes.c_uint64 * 4)(*[int(v) for v in counter])
+    k = (ctypes.c_uint64 * 2)(*[int(v) for v in key])
+    o = (ctypes.c_uint64 * 4)()
+    lib().oracle_philox4x64_10(c, k, o)
+    return [int(v) for v in o]
+
+
+def inject_draw(seed, i, j):
+    """Draw j of global particle i of the device injection, in [0, 1)."""
+    return lib().oracle_inject_draw(int(seed), int(i), int(j))
+
+
+def inject(seed, first, count, g, *, fixed_position=False, position=(0.0, 0.0, 0.0), fixed_mu=False, mu0=0.0,
+           pscale=1.0, B=None):
+    """init_particles with the device injection's draws for particles
+    first..first+count-1.  B: three (nx,ny,nz) float64 arrays (fixed mu).
+    Returns (out (6, count) float64, status)."""
+    out = np.empty((6, count))
+    pos = np.ascontiguousarray(position, dtype=np.float64)
+    keep = [np.asfortranarray(b, dtype=np.float64) for b in B] if B is not None else [np.zeros(1)] * 3
+    Bp = (_dp * 3)(*[_ptr(b) for b in keep])
+    st = lib().oracle_inject(int(seed), int(first), int(count), int(bool(fixed_position)), _ptr(pos),
+                             int(bool(fixed_mu)), float(mu0), float(pscale), Bp, ctypes.byref(g), _ptr(out))
+    return out, st

@@ -13,7 +13,7 @@
  * where the reference is 1-based the translation is stated.
  *
  * Parity pinning (which check pins which function) is listed in the header
- * comment of each function; the tests live in tests/test_oracle_*.py.
+ * comment of each function; the tests live in tests/test_oracle.py.
  */
 #include <math.h>
 #include <stdint.h>
@@ -515,4 +515,141 @@ double oracle_mean_abs_b(const double *bx, const double *by, const double *bz, i
     long double acc = 0.0L;
     for (int64_t i = 0; i < n; ++i) acc += (long double)sqrt(bx[i] * bx[i] + by[i] * by[i] + bz[i] * bz[i]);
     return (double)(acc / (long double)n);
+}
+
+/*
+ * Counter-based injection draws.  The reference draws from Julia's
+ * MersenneTwister(seed), per particle, in the order of init_particles
+ * (phase_space_gpu_runner.jl:196-245).  That stream is sequential, so the
+ * device injection draws from Philox4x64-10 (Salmon, Moraes, Dror and Shaw,
+ * "Parallel random numbers: as easy as 1, 2, 3", SC 2011) instead: key
+ * (seed, 0), counter (particle index, block, 0, 0), four 64-bit words per
+ * block.  Draw j of particle i is word j % 4 of block j / 4, mapped to
+ * [0, 1) as (w >> 11) * 2^-53.
+ * Pinned by: numpy.random.Philox, an independent implementation of the same
+ * generator, on arbitrary keys and counters, and its random() doubles.
+ */
+static inline uint64_t mulhi64(uint64_t a, uint64_t b) {
+    return (uint64_t)(((unsigned __int128)a * (unsigned __int128)b) >> 64);
+}
+
+void oracle_philox4x64_10(const uint64_t ctr_in[4], const uint64_t key_in[2], uint64_t out[4]) {
+    const uint64_t M0 = 0xD2E7470EE14C6C93ULL, M1 = 0xCA5A826395121157ULL;
+    const uint64_t W0 = 0x9E3779B97F4A7C15ULL, W1 = 0xBB67AE8584CAA73BULL;
+    uint64_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint64_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += W0;
+            k1 += W1;
+        }
+        uint64_t hi0 = mulhi64(M0, c0), lo0 = M0 * c0;
+        uint64_t hi1 = mulhi64(M1, c2), lo1 = M1 * c2;
+        uint64_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+double oracle_inject_draw(uint64_t seed, int64_t i, int j) {
+    uint64_t ctr[4] = {(uint64_t)i, (uint64_t)(j / 4), 0, 0}, key[2] = {seed, 0}, w[4];
+    oracle_philox4x64_10(ctr, key, w);
+    return (double)(w[j % 4] >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/*
+ * fixed_mu_unit_vector: phase_space_gpu_runner.jl:170-194 in FP64 with the
+ * gyrophase phi already drawn.  Returns -1 where |B| is zero or non-finite
+ * (the reference raises an error there), else 0.
+ */
+static int fixed_mu_unit(double bx, double by, double bz, double mu0, double phi, double u[3]) {
+    double bn = sqrt(bx * bx + by * by + bz * bz);
+    if (!isfinite(bn) || bn == 0.0) return -1;
+    double bhx = bx / bn, bhy = by / bn, bhz = bz / bn;
+    double e1x, e1y, e1z;
+    if (fabs(bhx) < 0.9) {
+        e1x = 0.0;
+        e1y = bhz;
+        e1z = -bhy;
+    } else {
+        e1x = -bhz;
+        e1y = 0.0;
+        e1z = bhx;
+    }
+    double e1n = sqrt(e1x * e1x + e1y * e1y + e1z * e1z);
+    e1x = e1x / e1n;
+    e1y = e1y / e1n;
+    e1z = e1z / e1n;
+    double e2x = bhy * e1z - bhz * e1y;
+    double e2y = bhz * e1x - bhx * e1z;
+    double e2z = bhx * e1y - bhy * e1x;
+    double perp = sqrt(fmax(0.0, 1.0 - mu0 * mu0));
+    double cphi = cos(phi), sphi = sin(phi);
+    u[0] = mu0 * bhx + perp * (cphi * e1x + sphi * e2x);
+    u[1] = mu0 * bhy + perp * (cphi * e1y + sphi * e2y);
+    u[2] = mu0 * bhz + perp * (cphi * e1z + sphi * e2z);
+    return 0;
+}
+
+/*
+ * init_particles: phase_space_gpu_runner.jl:196-245 in FP64, with the draws
+ * of oracle_inject_draw, for global particles first .. first+count-1.  Per
+ * particle, in the reference's order: three position draws
+ * rand * (xmax - xmin) + xmin (skipped for a fixed position), then either
+ * z = 2u - 1 and phi = 2 pi u (rand_unit_vector, :61-66) or, for fixed mu,
+ * trilinear B at the position and phi = 2 pi u.  p = pscale * unit vector.
+ * out is [x1 x2 x3 p1 p2 p3][count].  B (three column-major (nx,ny,nz)
+ * arrays) is read only for fixed mu.  Returns -1 if a fixed-mu particle sits
+ * where |B| is zero or non-finite, else 0.
+ * Pinned by: positions equal numpy Philox doubles mapped onto the box;
+ * fixed-mu momenta have p.b / (|p||b|) == mu0 and |p| == pscale; isotropic
+ * directions give <u_z> = 0, <u_z^2> = 1/3 and a uniform azimuth.
+ */
+int oracle_inject(uint64_t seed, int64_t first, int64_t count, int fixed_position, const double pos[3],
+                  int fixed_mu, double mu0, double pscale, const double *const B[3],
+                  const oracle_geom *g, double *out) {
+    const double two_pi = 6.283185307179586; /* Julia's 2 * pi as Float64 */
+    const double lo[3] = {g->xmin, g->ymin, g->zmin}, hi[3] = {g->xmax, g->ymax, g->zmax};
+    int status = 0;
+    for (int64_t q = 0; q < count; ++q) {
+        const int64_t i = first + q;
+        int j = 0;
+        double x[3], u[3];
+        for (int c = 0; c < 3; ++c) {
+            if (fixed_position) {
+                x[c] = pos[c];
+            } else {
+                x[c] = oracle_inject_draw(seed, i, j++) * (hi[c] - lo[c]) + lo[c];
+            }
+        }
+        if (fixed_mu) {
+            double b[3];
+            for (int c = 0; c < 3; ++c)
+                b[c] = oracle_trilinear(B[c], x[0], x[1], x[2], g->xmin, g->ymin, g->zmin, g->dx, g->dy,
+                                        g->dz, g->nx, g->ny, g->nz);
+            double phi = two_pi * oracle_inject_draw(seed, i, j++);
+            if (fixed_mu_unit(b[0], b[1], b[2], mu0, phi, u) != 0) {
+                status = -1;
+                u[0] = u[1] = u[2] = 0.0;
+            }
+        } else {
+            double z = 2 * oracle_inject_draw(seed, i, j++) - 1;
+            double phi = two_pi * oracle_inject_draw(seed, i, j++);
+            double r = sqrt(fmax(0.0, 1.0 - z * z));
+            u[0] = r * cos(phi);
+            u[1] = r * sin(phi);
+            u[2] = z;
+        }
+        for (int c = 0; c < 3; ++c) {
+            out[c * count + q] = x[c];
+            out[(3 + c) * count + q] = pscale * u[c];
+        }
+    }
+    return status;
 }

@@ -25,7 +25,8 @@ OUT = {"none": 0, "phase": 1, "phase-space": 1, "mu": 2}
 EXPORTS = (
     "pt_abi_version", "pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_ctx_bytes_in_use",
     "pt_field_create", "pt_field_destroy", "pt_field_mean_abs_b", "pt_mean_abs_b",
-    "pt_ens_create", "pt_ens_destroy", "pt_ens_advance", "pt_ens_get_state", "pt_ens_last_timing", "pt_ens_sort",
+    "pt_ens_create", "pt_ens_create_injected", "pt_inject", "pt_ens_destroy", "pt_ens_advance",
+    "pt_ens_get_state", "pt_ens_last_timing", "pt_ens_sort",
     "pt_mu_from_phase", "pt_dmumu_accumulate", "pt_dpp_accumulate", "pt_dpp_accumulate_pmag",
     "pt_energy_snapshots", "pt_buffer_alloc", "pt_buffer_free", "pt_memcpy", "pt_host_alloc", "pt_host_free",
     "pt_serve_create", "pt_serve_destroy", "pt_serve_add_field", "pt_serve_submit", "pt_serve_step",
@@ -45,6 +46,12 @@ class AdvanceParams(ctypes.Structure):
                 ("first_step", ctypes.c_int64), ("n_steps", ctypes.c_int64), ("total_steps", ctypes.c_int64),
                 ("save_stride", ctypes.c_int64), ("out_kind", ctypes.c_int32),
                 ("steps_per_launch", ctypes.c_int32), ("sort_every_steps", ctypes.c_int64)]
+
+
+class InjectParams(ctypes.Structure):
+    """pt_inject_params: injection settings of init_particles."""
+    _fields_ = [("seed", ctypes.c_uint64), ("fixed_position", ctypes.c_int32), ("fixed_mu", ctypes.c_int32),
+                ("position_m", ctypes.c_double * 3), ("mu0", ctypes.c_double), ("p_magnitude", ctypes.c_double)]
 
 
 _lib = None
@@ -79,6 +86,9 @@ def load():
         "pt_mean_abs_b": (_int, [_vp, _i64, _int, _int, _vp, _vp, _vp, _dp]),
         "pt_ens_create": (_int, [_vp, _vp, _int, _i64, _int, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                  _int, ctypes.POINTER(_vp)]),
+        "pt_ens_create_injected": (_int, [_vp, _vp, _int, _i64, ctypes.POINTER(InjectParams), _int,
+                                          ctypes.POINTER(_vp)]),
+        "pt_inject": (_int, [_vp, _vp, _i64, _i64, ctypes.POINTER(InjectParams), _int, _dp]),
         "pt_ens_destroy": (_int, [_vp]),
         "pt_ens_advance": (_int, [_vp, _vp, ctypes.POINTER(AdvanceParams), _vp, _vp, _int, _int, _i64,
                                   _i64p, _i64p]),

@@ -5,6 +5,7 @@ Usage:  python -m particle_transport_amd.build   (or build_library() from Python
 Per-file flags:
   pt_push_ref.hip   -ffp-contract=off  (FP64 parity path: the reference's
                                         arithmetic has no fused multiply-adds)
+  pt_inject.hip     -ffp-contract=off  (positions bit-identical to plain FP64)
   pt_stats.hip      -munsafe-fp-atomics (hardware FP64 atomics on coarse-grained
                                         hipMalloc memory)
 All files: packed FP32 instructions disabled (-target-feature -packed-fp32-ops).
@@ -33,6 +34,7 @@ SOURCES = {
     "pt_stats.hip": ["-munsafe-fp-atomics"],
     "pt_serve.hip": [],
     "pt_sort.hip": [],
+    "pt_inject.hip": ["-ffp-contract=off"],
 }
 
 

@@ -165,6 +165,10 @@ void launch_component_stats(pt_ctx *ctx, int64_t n, int src_f64, const void *src
 void launch_split_from_si(pt_ens *e, const double *dx, const double *dp);
 void morton_sort(pt_ens *e);  // pt_sort.hip
 void launch_split_to_si(pt_ens *e, double *dx, double *dp, uint8_t *alive_out);
+// pt_inject.hip: initial state of particles first..first+count-1 into out
+// ([x1 x2 x3 p1 p2 p3][count], FP64 SI); *bad = 1 where fixed-mu meets zero |B|.
+void launch_inject(const pt_field *f, const pt_inject_params &ip, int64_t first, int64_t count, double *out,
+                   int *bad, hipStream_t s);
 
 // Device scratch owned by the context, grown on demand (never shrinks).
 void *ctx_scratch(pt_ctx *ctx, size_t bytes);

@@ -358,6 +358,67 @@ static int64_t saves_in_range(int64_t first, int64_t count, int64_t stride) {
     return hi >= lo ? hi - lo + 1 : 0;
 }
 
+static void check_pusher_field(int pusher, const pt_field *field, const char *who) {
+    PT_REQUIRE(pusher == PT_PUSH_REF_F64 || pusher == PT_PUSH_PLAIN_F32 || pusher == PT_PUSH_FAST,
+               std::string(who) + ": unknown pusher");
+    if (pusher == PT_PUSH_REF_F64)
+        PT_REQUIRE(field->storage == PT_STORE_F64_PLANAR, "PT_PUSH_REF_F64 needs a PT_STORE_F64_PLANAR field");
+    if (pusher == PT_PUSH_PLAIN_F32)
+        PT_REQUIRE(field->storage == PT_STORE_F32_PLANAR, "PT_PUSH_PLAIN_F32 needs a PT_STORE_F32_PLANAR field");
+    if (pusher == PT_PUSH_FAST)
+        PT_REQUIRE(field->storage != PT_STORE_F64_PLANAR, "PT_PUSH_FAST needs an FP32 or 16-bit field");
+}
+
+// The reference's checks on the injection settings (init_particles,
+// phase_space_gpu_runner.jl:215-219; fixed_injection_position :164-166).
+static void check_inject_params(const pt_inject_params &ip, const pt_field *field) {
+    PT_REQUIRE(ip.fixed_position == 0 || ip.fixed_position == 1, "pt_inject_params.fixed_position must be 0 or 1");
+    PT_REQUIRE(ip.fixed_mu == 0 || ip.fixed_mu == 1, "pt_inject_params.fixed_mu must be 0 or 1");
+    PT_REQUIRE(std::isfinite(ip.p_magnitude), "pt_inject_params.p_magnitude must be finite");
+    if (ip.fixed_position)
+        for (int c = 0; c < 3; ++c)
+            PT_REQUIRE(std::isfinite(ip.position_m[c]) && field->g.lo[c] <= ip.position_m[c] &&
+                           ip.position_m[c] <= field->g.hi[c],
+                       "fixed injection position is outside the simulation box");
+    if (ip.fixed_mu) PT_REQUIRE(ip.mu0 >= -1.0 && ip.mu0 <= 1.0, "injection_mu0 must be in [-1, 1]");
+}
+
+// Allocates an ensemble and its state.  fill(si, flag) enqueues on the
+// context stream the writes of the FP64 SI state [x1 x2 x3 p1 p2 p3][n] into
+// the device array si; flag is one int of device scratch it may use.  The
+// split pushers' state is converted from si on the device.
+template <class Fill>
+static pt_ens *make_ensemble(pt_ctx *ctx, const pt_field *field, int pusher, int64_t n, int periodic,
+                             Fill &&fill) {
+    const size_t N = static_cast<size_t>(n);
+    auto *e = new pt_ens();
+    e->ctx = ctx;
+    e->field = field;
+    e->pusher = pusher;
+    e->n = n;
+    e->periodic = periodic ? 1 : 0;
+    e->n_alive = n;
+    try {
+        e->state_bytes = pusher == PT_PUSH_REF_F64 ? 6 * N * 8 : kSplitStateBytes * N;
+        e->state = dev_alloc(ctx, e->state_bytes);
+        e->alive = static_cast<uint8_t *>(dev_alloc(ctx, N));
+        PT_HIP_CHECK(hipMemsetAsync(e->alive, 1, N, ctx->stream));
+        const size_t si_bytes = pusher == PT_PUSH_REF_F64 ? 0 : 6 * N * 8;
+        char *scr = static_cast<char *>(ctx_scratch(ctx, si_bytes + 256));
+        double *si = pusher == PT_PUSH_REF_F64 ? static_cast<double *>(e->state) : reinterpret_cast<double *>(scr);
+        fill(si, reinterpret_cast<int *>(scr + si_bytes));
+        if (pusher != PT_PUSH_REF_F64) launch_split_from_si(e, si, si + 3 * N);
+        PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        dev_free(ctx, e->state, e->state_bytes);
+        dev_free(ctx, e->alive, N);
+        delete e;
+        throw;
+    }
+    return e;
+}
+
 }  // namespace pt
 
 using namespace pt;
@@ -579,14 +640,7 @@ int pt_ens_create(pt_ctx *ctx, const pt_field *field, int pusher, int64_t n, int
         PT_REQUIRE(ctx && field && x && p && out, "pt_ens_create: null argument");
         PT_REQUIRE(n >= 1 && n < (1LL << 31), "pt_ens_create: particle count out of range");
         PT_REQUIRE(src_dtype == PT_F64 || src_dtype == PT_F32, "pt_ens_create: bad dtype");
-        PT_REQUIRE(pusher == PT_PUSH_REF_F64 || pusher == PT_PUSH_PLAIN_F32 || pusher == PT_PUSH_FAST,
-                   "pt_ens_create: unknown pusher");
-        if (pusher == PT_PUSH_REF_F64)
-            PT_REQUIRE(field->storage == PT_STORE_F64_PLANAR, "PT_PUSH_REF_F64 needs a PT_STORE_F64_PLANAR field");
-        if (pusher == PT_PUSH_PLAIN_F32)
-            PT_REQUIRE(field->storage == PT_STORE_F32_PLANAR, "PT_PUSH_PLAIN_F32 needs a PT_STORE_F32_PLANAR field");
-        if (pusher == PT_PUSH_FAST)
-            PT_REQUIRE(field->storage != PT_STORE_F64_PLANAR, "PT_PUSH_FAST needs an FP32 or 16-bit field");
+        check_pusher_field(pusher, field, "pt_ens_create");
         for (int c = 0; c < 3; ++c) PT_REQUIRE(x[c] && p[c], "pt_ens_create: null state component");
         PT_HIP_CHECK(hipSetDevice(ctx->device));
 
@@ -614,32 +668,51 @@ int pt_ens_create(pt_ctx *ctx, const pt_field *field, int pusher, int64_t n, int
                            "pt_ens_create: initial position outside the simulation box");
                 PT_REQUIRE(std::isfinite(h[(3 + c) * N + i]), "pt_ens_create: non-finite momentum");
             }
+        *out = make_ensemble(ctx, field, pusher, n, periodic, [&](double *si, int *) {
+            PT_HIP_CHECK(hipMemcpyAsync(si, h.data(), 6 * N * 8, hipMemcpyHostToDevice, ctx->stream));
+        });
+    });
+}
 
-        auto *e = new pt_ens();
-        e->ctx = ctx;
-        e->field = field;
-        e->pusher = pusher;
-        e->n = n;
-        e->periodic = periodic ? 1 : 0;
-        e->n_alive = n;
-        try {
-            e->state_bytes = pusher == PT_PUSH_REF_F64 ? 6 * N * 8 : kSplitStateBytes * N;
-            e->state = dev_alloc(ctx, e->state_bytes);
-            e->alive = static_cast<uint8_t *>(dev_alloc(ctx, N));
-            PT_HIP_CHECK(hipMemsetAsync(e->alive, 1, N, ctx->stream));
-            if (pusher == PT_PUSH_REF_F64) {
-                PT_HIP_CHECK(hipMemcpyAsync(e->state, h.data(), 6 * N * 8, hipMemcpyHostToDevice, ctx->stream));
-            } else {
-                double *st = static_cast<double *>(ctx_scratch(ctx, 6 * N * 8));
-                PT_HIP_CHECK(hipMemcpyAsync(st, h.data(), 6 * N * 8, hipMemcpyHostToDevice, ctx->stream));
-                launch_split_from_si(e, st, st + 3 * N);
-            }
-            PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        } catch (...) {
-            dev_free(ctx, e->state, e->state_bytes);
-            dev_free(ctx, e->alive, N);
-            delete e;
-            throw;
+int pt_inject(pt_ctx *ctx, const pt_field *field, int64_t first_index, int64_t n,
+              const pt_inject_params *params, int out_on_device, double *out) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && field && params && out, "pt_inject: null argument");
+        PT_REQUIRE(n >= 1 && n < (1LL << 31), "pt_inject: particle count out of range");
+        PT_REQUIRE(first_index >= 0 && first_index <= INT64_MAX - n, "pt_inject: first_index out of range");
+        check_inject_params(*params, field);
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        const size_t bytes = 6 * static_cast<size_t>(n) * 8;
+        char *scr = static_cast<char *>(ctx_scratch(ctx, (out_on_device ? 0 : bytes) + 256));
+        double *dst = out_on_device ? out : reinterpret_cast<double *>(scr);
+        int *flag = reinterpret_cast<int *>(scr + (out_on_device ? 0 : bytes));
+        int bad = 0;
+        PT_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+        launch_inject(field, *params, first_index, n, dst, flag, ctx->stream);
+        if (!out_on_device) PT_HIP_CHECK(hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        PT_HIP_CHECK(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        PT_REQUIRE(!bad, "cannot use fixed-mu injection where local B is zero or non-finite");
+    });
+}
+
+int pt_ens_create_injected(pt_ctx *ctx, const pt_field *field, int pusher, int64_t n,
+                           const pt_inject_params *params, int periodic, pt_ens **out) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && field && params && out, "pt_ens_create_injected: null argument");
+        PT_REQUIRE(n >= 1 && n < (1LL << 31), "pt_ens_create_injected: particle count out of range");
+        check_pusher_field(pusher, field, "pt_ens_create_injected");
+        check_inject_params(*params, field);
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        int bad = 0;
+        pt_ens *e = make_ensemble(ctx, field, pusher, n, periodic, [&](double *si, int *flag) {
+            PT_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+            launch_inject(field, *params, 0, n, si, flag, ctx->stream);
+            PT_HIP_CHECK(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        });
+        if (bad) {
+            (void)pt_ens_destroy(e);
+            throw Error{PT_ERR_INVALID, "cannot use fixed-mu injection where local B is zero or non-finite"};
         }
         *out = e;
     });

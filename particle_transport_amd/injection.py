@@ -5,6 +5,10 @@ reference's names, argument meaning, per-particle draw order and errors.  The
 random stream is numpy's PCG64 seeded with ``cfg["seed"]``, not Julia's
 MersenneTwister: the reference's exact stream cannot be reproduced here, so
 parity runs hand identical initial arrays to both sides instead.
+
+``init_particles_device`` and ``inject_params`` draw the same injection on the
+device (``pt_inject`` / ``pt_ens_create_injected``) from a counter-based
+Philox4x64-10 stream per particle, selected with ``injection_rng = "device"``.
 """
 from __future__ import annotations
 
@@ -176,3 +180,51 @@ def init_particles(cfg, axes, gamma0, v0, dtype=np.float64, Bx=None, By=None, Bz
         ux, uy, uz = r * np.cos(phi), r * np.sin(phi), z
     p = [(pscale * v).astype(T) for v in (ux, uy, uz)]
     return x[0], x[1], x[2], p[0], p[1], p[2]
+
+
+def injection_rng(cfg):
+    """Where init_particles draws: "host" (numpy PCG64, the default) or
+    "device" (Philox4x64-10 counters per particle, pt_inject)."""
+    m = _norm_mode(cfg.get("injection_rng", "host"))
+    if m not in ("host", "device"):
+        raise PipelineError("injection_rng must be host or device")
+    return m
+
+
+def inject_params(cfg, axes, gamma0, v0):
+    """pt_inject_params for init_particles (phase_space_gpu_runner.jl:196-245)
+    with the reference's checks on the settings (:210-219), in FP64."""
+    from . import _lib
+
+    ip = _lib.InjectParams()
+    ip.seed = int(cfg["seed"]) & 0xFFFFFFFFFFFFFFFF
+    fixed = injection_position_mode(cfg) == "fixed"
+    ip.fixed_position = int(fixed)
+    if fixed:
+        for c, v in enumerate(fixed_injection_position(cfg, axes, np.float64)):
+            ip.position_m[c] = float(v)
+    fixed_mu = injection_mode(cfg) == "fixed_mu"
+    ip.fixed_mu = int(fixed_mu)
+    mu0 = float(cfg.get("injection_mu0", 0.0))
+    if fixed_mu and not (-1.0 <= mu0 <= 1.0):
+        raise PipelineError("injection_mu0 must be in [-1, 1]")
+    ip.mu0 = mu0
+    ip.p_magnitude = float(gamma0) * M_P * float(v0)
+    return ip
+
+
+def init_particles_device(cfg, ctx, field, axes, gamma0, v0, first_index=0, n=None):
+    """init_particles (phase_space_gpu_runner.jl:196-245) drawn on the device
+    by pt_inject for global particles first_index .. first_index+n-1 (n
+    defaults to cfg["n_particles"]).  Fixed-mu injection interpolates B from
+    the values the field stores.  Returns x1, x2, x3, p1, p2, p3 (FP64)."""
+    import ctypes
+
+    from . import _lib
+
+    n = int(cfg["n_particles"]) if n is None else int(n)
+    ip = inject_params(cfg, axes, gamma0, v0)
+    out = np.empty((6, n), dtype=np.float64)
+    _lib.check(_lib.load().pt_inject(ctx.h, field.h, int(first_index), n, ctypes.byref(ip), 0,
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return tuple(out[c] for c in range(6))

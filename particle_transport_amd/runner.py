@@ -25,7 +25,9 @@ import numpy as np
 from . import _lib
 from ._lib import PT_F32, PT_F64, PUSHER, STORAGE, check, ptr
 from .fields import build_uniform_coords, estimate_min_dx
-from .injection import M_P, Q_E, energy_to_gamma, energy_to_speed, init_particles
+from .injection import (M_P, Q_E, energy_to_gamma, energy_to_speed, init_particles, inject_params,
+                        injection_mode, injection_position_mode, injection_position_tuple,
+                        injection_position_unit, injection_rng)
 from .time_units import (TIME_REFERENCE_MODE, PipelineError, campaign_time_reference, reference_Omega0,
                          require_explicit_campaign_time_reference, resolve_save_stride,
                          resolve_trajectory_time_grid, sampled_step_indices)
@@ -146,6 +148,23 @@ class Ensemble:
         self.field = field
         self.n = n
         self.pusher = pusher
+
+    @classmethod
+    def injected(cls, ctx: Context, field: Field, n: int, params, *, pusher: str = "ref_f64",
+                 periodic: bool = True):
+        """pt_ens_create_injected: the initial state is drawn on the device
+        (init_particles with per-particle Philox4x64-10 counters); params is
+        injection.inject_params(...)."""
+        h = ctypes.c_void_p()
+        check(_lib.load().pt_ens_create_injected(ctx.h, field.h, PUSHER[pusher], int(n), ctypes.byref(params),
+                                                 int(bool(periodic)), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self.h = h
+        self.ctx = ctx
+        self.field = field
+        self.n = int(n)
+        self.pusher = pusher
+        return self
 
     def advance(self, field=None, *, dt, first_step, n_steps, total_steps, stride=1, out_kind="none",
                 out_dtype=np.float32, charge=Q_E, mass=M_P, steps_per_launch=0, out_a=None, out_b=None,
@@ -340,17 +359,23 @@ def _run_loop(cfg, fields, setup, out_kind, writer, ctx, field, initial_state, c
 
     s = setup
     n = int(cfg["n_particles"])
-    if initial_state is None:
-        x1, x2, x3, p1, p2, p3 = init_particles(cfg, s.axes, s.gamma0, s.v0, s.T, fields[0], fields[1], fields[2])
-    else:
+    on_device = initial_state is None and injection_rng(cfg) == "device"
+    if initial_state is not None:
         x1, x2, x3, p1, p2, p3 = initial_state
+    elif not on_device:
+        x1, x2, x3, p1, p2, p3 = init_particles(cfg, s.axes, s.gamma0, s.v0, s.T, fields[0], fields[1], fields[2])
     pusher, storage = default_engine(cfg)
     own_ctx = ctx is None
     ctx = ctx or Context()
     own_field = field is None
     L = float(cfg["box_length_pc"]) * 3.085677581e16
     field = field or Field(ctx, fields, L, storage)
-    ens = Ensemble(ctx, field, (x1, x2, x3), (p1, p2, p3), pusher=pusher, periodic=s.boundary == "periodic")
+    periodic = s.boundary == "periodic"
+    if on_device:
+        ens = Ensemble.injected(ctx, field, n, inject_params(cfg, s.axes, s.gamma0, s.v0), pusher=pusher,
+                                periodic=periodic)
+    else:
+        ens = Ensemble(ctx, field, (x1, x2, x3), (p1, p2, p3), pusher=pusher, periodic=periodic)
     alive_fraction = np.empty(s.nsteps)
     stride = s.st.trajectory_time_stride
     nsave = len(s.save_idx)
@@ -434,7 +459,16 @@ def _common_meta(cfg, s, energy_GeV, alive_fraction):
         dt_gyroperiods=s.tt.dt_gyroperiods, integration_step_count=s.tt.n_integration_steps,
         analysis_sample_count=len(idx), trajectory_time_stride=s.st.trajectory_time_stride,
         exact_final_state_stored=False, boundary_mode=s.boundary,
-        trajectory_mode=str(cfg.get("trajectory_mode", "unknown")))
+        trajectory_mode=str(cfg.get("trajectory_mode", "unknown")), **injection_metadata(cfg))
+
+
+def injection_metadata(cfg):
+    """write_injection_metadata! (phase_space_gpu_runner.jl:136-143), plus where
+    the draws were made (injection_rng)."""
+    return dict(injection_mode=injection_mode(cfg), injection_mu0=[float(cfg.get("injection_mu0", 0.0))],
+                injection_position_mode=injection_position_mode(cfg),
+                injection_position=[float(v) for v in injection_position_tuple(cfg, np.float64)],
+                injection_position_unit=injection_position_unit(cfg), injection_rng=injection_rng(cfg))
 
 
 def run_gpu_ensemble(cfg, fields, *, energy_GeV, time_reference, ctx=None, field=None, initial_state=None,
