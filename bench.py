@@ -277,6 +277,44 @@ def stats_rates(ctx, F32, L, x, p, dt, log_fn):
     return out
 
 
+def injection_rates(ctx, F32, L, energy_gev, log_fn, n=4194304):
+    """Ensemble creation with the initial state drawn on the device
+    (pt_ens_create_injected) against the host path (numpy init_particles, then
+    pt_ens_create from host arrays), FP16 field, production pusher state."""
+    import numpy as np
+
+    from particle_transport_amd import injection
+    from particle_transport_amd.runner import Ensemble, Field
+    fld = Field(ctx, F32, L, "f16_brick")
+    nn = F32[0].shape[0]
+    axes = [dict(min=0.0, max=L, d=L / (nn - 1), n=nn)] * 3
+    gamma0 = injection.energy_to_gamma(energy_gev)
+    v0 = injection.energy_to_speed(energy_gev)
+    out = {}
+    for mode in ("isotropic", "fixed_mu"):
+        cfg = dict(seed=42, n_particles=n, injection_mode=mode, injection_mu0=0.5)
+        ip = injection.inject_params(cfg, axes, gamma0, v0)
+        for rep in range(2):  # the first call also grows the context scratch
+            t0 = time.perf_counter()
+            ens = Ensemble.injected(ctx, fld, n, ip, pusher="fast")
+            dev_s = time.perf_counter() - t0
+            ens.close()
+        # the host path interpolates FP64 field arrays, as the reference holds them
+        B = [np.asarray(b, dtype=np.float64) for b in F32[:3]] if mode == "fixed_mu" else [None] * 3
+        t0 = time.perf_counter()
+        st = injection.init_particles(cfg, axes, gamma0, v0, np.float64, *B)
+        ens = Ensemble(ctx, fld, st[:3], st[3:], pusher="fast")
+        host_s = time.perf_counter() - t0
+        ens.close()
+        del B, st
+        out[mode] = {"device_particles_per_s": n / dev_s, "host_particles_per_s": n / host_s,
+                     "device_s": dev_s, "host_s": host_s}
+        log_fn(f"[bench] injection {mode}: device {dev_s * 1e3:.1f} ms, host {host_s * 1e3:.1f} ms for {n} particles")
+    out["shape"] = f"{n} protons at {energy_gev:g} GeV into a {nn - 1}^3-cell FP16 field, FAST pusher state"
+    fld.close()
+    return out
+
+
 def pcie_inclusive(F32, L, x, p, energy_gev, log_fn, n=4194304, duration_gp=20.0):
     """The runner path end to end: run_mu_cache_ensemble with the mu cache in
     host memory (double-buffered device chunks, pinned staging, writer
@@ -490,6 +528,7 @@ def main():
         check = dmumu_check(ctx, F32, L, x, p, dt, 65536, 4001, stride)
 
     stats = None if args.no_stats else stats_rates(ctx, F32, L, x, p, dt, log)
+    inject = None if args.no_stats else injection_rates(ctx, F32, L, args.energy_gev, log)
     pcie = None if args.no_pcie else pcie_inclusive(F32, L, x, p, args.energy_gev, log)
 
     cpu = None
@@ -547,6 +586,7 @@ def main():
                       "speedup_of_bf16": timing["bf16_brick"]["value"] / plain_rate},
         "dmumu_check": check,
         "statistics": stats,
+        "injection": inject,
         "pcie_inclusive": pcie,
         "serving": serve,
         "modes_batch": modes,
