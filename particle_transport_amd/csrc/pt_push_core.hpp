@@ -47,10 +47,13 @@ struct Phys {
 // rotation's rounding drifts |p| by ~5e-8 per push, systematically.
 //
 // `renorm` (uniform across the wave) applies the rescale; between rescales
-// |U| is off by at most the rotation drift of the pushes since the last one,
-// while P2, gamma and the position update stay exact to FP64/FP32 rounding.
-__device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, double &P2, float &onep2, float bx,
-                                            float by, float bz, float ux, float uy, float uz, float &inv_gn,
+// |U| is off by at most the rotation drift of the pushes since the last one.
+// The Lorentz factors come from the FP32 vector (1/gamma- from 1 + |U-|^2,
+// 1/gamma+ from it and the second kick), so they are within ~2e-7 of the FP64
+// values and depend only on the particle's state, never on how the steps are
+// split into launches.  inv_gn returns 1/gamma of the new momentum.
+__device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, double &P2, float bx, float by,
+                                            float bz, float ux, float uy, float uz, float &inv_gn,
                                             bool renorm = true) {
     // E~ = -(u~ x B~)
     const float ex = -(uy * bz - uz * by);
@@ -60,10 +63,8 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
     const float ee = ex * ex + ey * ey + ez * ez;
     const float k1 = fmaf(2.0f, Ux * ex + Uy * ey + Uz * ez, ee);
     const float pmx = Ux + ex, pmy = Uy + ey, pmz = Uz + ez;
-    // 1/gamma- from the hardware reciprocal square root of 1 + |U-|^2, with
-    // onep2 = float(1 + P2) carried from the previous push: an error of an ulp
-    // changes the rotation angle by that relative amount and leaves |p| alone.
-    const float r = __builtin_amdgcn_rsqf(onep2 + k1);
+    // 1/gamma- from the hardware reciprocal square root of 1 + |U-|^2
+    const float r = __builtin_amdgcn_rsqf(fmaf(pmx, pmx, fmaf(pmy, pmy, fmaf(pmz, pmz, 1.0f))));
     const float tx = bx * r, ty = by * r, tz = bz * r;
     const float t2 = tx * tx + ty * ty + tz * tz;
     // s = 2t/(1+t^2): one Newton step on the hardware reciprocal (~0.5 ulp).
@@ -92,15 +93,15 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
         Uz *= sc;
     }
     P2 = P2n;
-    onep2 = static_cast<float>(1.0 + P2n);
-    inv_gn = __builtin_amdgcn_rsqf(onep2);
+    // The rotation keeps |U-|, so gamma+^2 = gamma-^2 + k2 and
+    // 1/gamma+ = r (1 + x)^(-1/2) with x = k2 r^2 ~ 1e-5: two Taylor terms.
+    const float x = k2 * (r * r);
+    inv_gn = r * fmaf(x, fmaf(0.375f, x, -0.5f), 1.0f);
 }
 
-// Advance one axis of the split position by u * step cells (one rounding);
-// `moved` is set when the cell changes.
-__device__ __forceinline__ void move_axis(int32_t &cell, float &off, float u, float step, int32_t nc,
-                                          bool periodic, bool &moved) {
-    off = fmaf(u, step, off);
+// Bring an offset that left [0, 1) back into its cell range (periodic wrap
+// of the cell index when `periodic`); `moved` is set when the cell changes.
+__device__ __forceinline__ void wrap_axis(int32_t &cell, float &off, int32_t nc, bool periodic, bool &moved) {
     if (!(off >= 0.0f && off < 1.0f)) {
         moved = true;
         const float fl = floorf(off);
@@ -115,6 +116,13 @@ __device__ __forceinline__ void move_axis(int32_t &cell, float &off, float u, fl
             if (cell < 0) cell += nc;
         }
     }
+}
+
+// Advance one axis of the split position by u * step cells (one rounding).
+__device__ __forceinline__ void move_axis(int32_t &cell, float &off, float u, float step, int32_t nc,
+                                          bool periodic, bool &moved) {
+    off = fmaf(u, step, off);
+    wrap_axis(cell, off, nc, periodic, moved);
 }
 
 // x > L or x < 0 kills (phase_space_gpu_runner.jl:385-387); x == L is cell nc, offset 0.
@@ -494,7 +502,6 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
         for (int q = 0; q < 8; ++q) a[c][q] = 0.0f;
     if (alive) lane_reload<STORE>(cell, fv, g, ph, a);
     bool reload = false;
-    float onep2 = static_cast<float>(1.0 + P2);
     const int32_t si_mod4 = static_cast<int32_t>(cad.si0 & 3);
     const ReloadLane rl = reload_lane(fv, ph);
     int32_t phase = cad.phase0;
@@ -525,15 +532,17 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
         // how the steps are split into launches
         const bool renorm = (((si_mod4 + k) & 3) == 0) | (phase == 0);
         float inv_gn;
-        boris_split(U[0], U[1], U[2], P2, onep2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn, renorm);
+        boris_split(U[0], U[1], U[2], P2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn, renorm);
         // all three offsets first; one rare branch for any cell change
 #pragma unroll
         for (int c = 0; c < 3; ++c) off[c] = fmaf(U[c], g.k[c] * inv_gn, off[c]);
-        const bool inside = (off[0] >= 0.0f) & (off[0] < 1.0f) & (off[1] >= 0.0f) & (off[1] < 1.0f) &
-                            (off[2] >= 0.0f) & (off[2] < 1.0f);
+        // every offset in [0, 1) <=> every bit pattern below that of 1.0f
+        // (negative values, -0 and NaN compare above it and take the slow path)
+        const uint32_t top = max(max(__float_as_uint(off[0]), __float_as_uint(off[1])), __float_as_uint(off[2]));
+        const bool inside = top < 0x3f800000u;
         if (!inside) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) move_axis(cell[c], off[c], 0.0f, 0.0f, g.nc[c], periodic, reload);
+            for (int c = 0; c < 3; ++c) wrap_axis(cell[c], off[c], g.nc[c], periodic, reload);
         }
         if (!periodic && (outside(cell[0], off[0], g.nc[0]) || outside(cell[1], off[1], g.nc[1]) ||
                           outside(cell[2], off[2], g.nc[2]))) {

@@ -84,9 +84,9 @@ k_push_plain(SplitState s, uint8_t *__restrict__ alive_arr, const float *__restr
         write_save(out_kind, out_f64, out_a, out_b, slot, n, col, alive, cell, off, U, v, g, ph);
     }
     if (!push) return;
-    float inv_gn, onep2 = static_cast<float>(1.0 + P2);
+    float inv_gn;
     bool moved = false;
-    boris_split(U[0], U[1], U[2], P2, onep2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn);
+    boris_split(U[0], U[1], U[2], P2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn);
     for (int c = 0; c < 3; ++c) move_axis(cell[c], off[c], U[c], g.k[c] * inv_gn, g.nc[c], periodic, moved);
     if (!periodic && (outside(cell[0], off[0], g.nc[0]) || outside(cell[1], off[1], g.nc[1]) ||
                       outside(cell[2], off[2], g.nc[2]))) {
