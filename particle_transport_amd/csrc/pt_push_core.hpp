@@ -66,18 +66,18 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
     // 1/gamma- from the hardware reciprocal square root of 1 + |U-|^2
     const float r = __builtin_amdgcn_rsqf(fmaf(pmx, pmx, fmaf(pmy, pmy, fmaf(pmz, pmz, 1.0f))));
     const float tx = bx * r, ty = by * r, tz = bz * r;
-    const float t2 = tx * tx + ty * ty + tz * tz;
     // s = 2t/(1+t^2): one Newton step on the hardware reciprocal (~0.5 ulp).
-    const float d = 1.0f + t2;
+    const float d = fmaf(tx, tx, fmaf(ty, ty, fmaf(tz, tz, 1.0f)));
     const float r0 = __builtin_amdgcn_rcpf(d);
     const float f = 2.0f * fmaf(r0, fmaf(-d, r0, 1.0f), r0);
     const float sx = tx * f, sy = ty * f, sz = tz * f;
-    const float ppx = pmx + (pmy * tz - pmz * ty);
-    const float ppy = pmy + (pmz * tx - pmx * tz);
-    const float ppz = pmz + (pmx * ty - pmy * tx);
-    const float qx = pmx + (ppy * sz - ppz * sy);
-    const float qy = pmy + (ppz * sx - ppx * sz);
-    const float qz = pmz + (ppx * sy - ppy * sx);
+    // p' = p- + p- x t and p+ = p- + p' x s, two fused multiply-adds per component
+    const float ppx = fmaf(pmy, tz, fmaf(-pmz, ty, pmx));
+    const float ppy = fmaf(pmz, tx, fmaf(-pmx, tz, pmy));
+    const float ppz = fmaf(pmx, ty, fmaf(-pmy, tx, pmz));
+    const float qx = fmaf(ppy, sz, fmaf(-ppz, sy, pmx));
+    const float qy = fmaf(ppz, sx, fmaf(-ppx, sz, pmy));
+    const float qz = fmaf(ppx, sy, fmaf(-ppy, sx, pmz));
     const float k2 = fmaf(2.0f, qx * ex + qy * ey + qz * ez, ee);
     const double P2n = P2 + static_cast<double>(k1 + k2);
     Ux = qx + ex;
@@ -410,13 +410,13 @@ __device__ __forceinline__ void lane_reload(const int32_t cell[3], const FieldVi
 }
 
 template <int STORE>
-__device__ __forceinline__ void coop_reload(bool &need, const int32_t cell[3], const FieldView &fv,
+__device__ __forceinline__ void coop_reload(int &need, const int32_t cell[3], const FieldView &fv,
                                             const SplitGeom &g, const ReloadLane &rl, ReloadLds *lds,
                                             float a[6][8]) {
-    uint64_t pending = __ballot(need);
+    uint64_t pending = __ballot(need != 0);
     while (pending) {
         const int rank = __popcll(pending & ((uint64_t{1} << rl.lane) - 1));
-        const bool mine = need && rank < kReloadBatch;
+        const bool mine = need != 0 && rank < kReloadBatch;
         const int m = __popcll(pending) < kReloadBatch ? __popcll(pending) : kReloadBatch;
         if (mine) {
             float b[3];
@@ -450,7 +450,7 @@ __device__ __forceinline__ void coop_reload(bool &need, const int32_t cell[3], c
                 d[2] = v.z;
                 d[3] = v.w;
             }
-            need = false;
+            need = 0;
         }
         pending &= ~__ballot(mine);
         wave_lds_sync();  // the next pass overwrites the slots and coefficients
@@ -501,7 +501,11 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
 #pragma unroll
         for (int q = 0; q < 8; ++q) a[c][q] = 0.0f;
     if (alive) lane_reload<STORE>(cell, fv, g, ph, a);
-    bool reload = false;
+    // Lane flags as integers, compared against zero where they are used:
+    // live = the particle is alive; reload = it changed cell in the previous
+    // push (never set on a dead lane).
+    int live = alive ? 1 : 0;
+    int reload = 0;
     const int32_t si_mod4 = static_cast<int32_t>(cad.si0 & 3);
     const ReloadLane rl = reload_lane(fv, ph);
     int32_t phase = cad.phase0;
@@ -512,8 +516,7 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
     for (int32_t k = 0; k < cad.K; ++k) {
         const bool do_save = cad.want_save && phase == 0;
         phase = phase + 1 == cad.stride ? 0 : phase + 1;
-        const bool push = alive && k < n_push;
-        reload = reload && alive;
+        const bool push = live != 0 && k < n_push;
         coop_reload<STORE>(reload, cell, fv, g, rl, lds, a);  // every lane of the wave, converged here
         if (!do_save && !push) continue;
         // dead lanes evaluate stale coefficients only to keep the loop
@@ -522,7 +525,7 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
 #pragma unroll
         for (int c = 0; c < 6; ++c) v[c] = eval_poly(a[c], off[0], off[1], off[2]);
         if (do_save) {
-            save(slot, alive, cell, off, U, P2, v);
+            save(slot, live != 0, cell, off, U, P2, v);
             ++slot;
         }
         if (!push) continue;
@@ -541,15 +544,18 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
         const uint32_t top = max(max(__float_as_uint(off[0]), __float_as_uint(off[1])), __float_as_uint(off[2]));
         const bool inside = top < 0x3f800000u;
         if (!inside) {
+            bool moved = false;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) wrap_axis(cell[c], off[c], g.nc[c], periodic, reload);
+            for (int c = 0; c < 3; ++c) wrap_axis(cell[c], off[c], g.nc[c], periodic, moved);
+            if (moved) reload = 1;
         }
-        if (!periodic && (outside(cell[0], off[0], g.nc[0]) || outside(cell[1], off[1], g.nc[1]) ||
-                          outside(cell[2], off[2], g.nc[2]))) {
-            alive = false;
-            die(cad.si0 + k);
-        }
+        const bool dies = !periodic && (outside(cell[0], off[0], g.nc[0]) | outside(cell[1], off[1], g.nc[1]) |
+                                        outside(cell[2], off[2], g.nc[2]));
+        if (dies) die(cad.si0 + k);
+        live = dies ? 0 : live;
+        reload = dies ? 0 : reload;
     }
+    alive = live != 0;
 }
 
 }  // namespace core
