@@ -139,19 +139,57 @@ def pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total):
     log("[probe] copies done")
 
 
-def cpu_baseline(F32, L, x, p, dt, n_cpu, steps_cpu):
+def cpu_threads():
+    """Host threads for the CPU baselines: the box's CPU share (16 per GPU) or fewer."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline(F32, L, x, p, dt, n_cpu, steps_cpu, threads=1):
+    """The oracle's FP64 step loop (C restatement of advance_particles_kernel!,
+    mu every 20 steps) on n_cpu particles, split into `threads` particle
+    chunks that run concurrently (the C calls release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     import numpy as np
 
     import oracle
     n_nodes = F32[0].shape[0]
     F = [np.asfortranarray(c, dtype=np.float64) for c in F32]
     g = oracle.make_geom(n_nodes, n_nodes, n_nodes, L)
-    xs = np.ascontiguousarray(x[:, :n_cpu])
-    ps = np.ascontiguousarray(p[:, :n_cpu])
+    bounds = np.linspace(0, n_cpu, threads + 1).astype(int)
+    parts = [(np.ascontiguousarray(x[:, a:b]), np.ascontiguousarray(p[:, a:b])) for a, b in zip(bounds, bounds[1:])]
+    oracle.lib()
     t0 = time.perf_counter()
-    oracle.run(F, g, xs, ps, dt, 1, steps_cpu, steps_cpu + 1, 20, True, 2)
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda xp: oracle.run(F, g, xp[0], xp[1], dt, 1, steps_cpu, steps_cpu + 1, 20, True, 2), parts))
     el = time.perf_counter() - t0
     return n_cpu * steps_cpu / el, el
+
+
+def cpu_dmumu_rate(mu_host, lags, edges, threads):
+    """Pair visits/s of the oracle's sliding D_mumu accumulator
+    (process_mu_chunk_full_pairs_cpu!, threads over particles as the
+    reference's Threads.@threads) on a host mu series (nsave, n)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    import oracle
+    from particle_transport_amd import dmumu
+    nsave, n = mu_host.shape
+    bounds = np.linspace(0, n, threads + 1).astype(int)
+    chunks = [np.ascontiguousarray(mu_host[:, a:b].T) for a, b in zip(bounds, bounds[1:])]
+
+    def one(c):
+        acc = oracle.new_dmumu_accumulators(len(lags), len(edges) - 1)
+        return oracle.dmumu(c, lags, edges, False, True, acc)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(one, chunks))
+    el = time.perf_counter() - t0
+    visits = dmumu.estimate_pair_visits(nsave, n, lags)
+    return visits / el, visits, el
 
 
 def dmumu_check(ctx, F32, L, x, p, dt, n_sub, steps, stride, groups=16):
@@ -222,7 +260,7 @@ def dmumu_check(ctx, F32, L, x, p, dt, n_sub, steps, stride, groups=16):
     return res
 
 
-def stats_rates(ctx, F32, L, x, p, dt, log_fn):
+def stats_rates(ctx, F32, L, x, p, dt, log_fn, cpu=True):
     """Pair-visit rates of the statistics kernels on series the production
     pusher writes into HBM (FP16 field): D_mumu over a mu cache of 262,144
     particles x 1001 samples (128 log-spaced lags, 40 |mu| bins, sliding
@@ -253,6 +291,15 @@ def stats_rates(ctx, F32, L, x, p, dt, log_fn):
     out["dmumu"] = {"pair_visits_per_s": visits / el, "pair_visits": visits, "seconds": el,
                     "shape": f"{n} particles x {nsave} samples, {len(lags)} lags, 40 bins, sliding"}
     log_fn(f"[bench] D_mumu: {visits / el:.4g} pair visits/s")
+    if cpu:
+        n_cpu = 16384
+        host = mu.to_host()[:, :n_cpu]
+        threads = cpu_threads()
+        rate, v_cpu, el_cpu = cpu_dmumu_rate(host, lags, edges, threads)
+        out["dmumu"]["cpu_baseline"] = {"pair_visits_per_s": rate, "cores": threads, "kind": "port",
+                                        "sample": f"{n_cpu} particles of the same series, {v_cpu} pair visits, "
+                                                  f"oracle accumulator in {threads} threads, {el_cpu:.1f} s"}
+        log_fn(f"[bench] D_mumu CPU oracle ({threads} threads): {rate:.4g} pair visits/s")
     mu.free()
     n = 131072
     mom = DeviceBuffer(ctx, (nsave, 3, n), np.float32)
@@ -422,7 +469,7 @@ def main():
     ap.add_argument("--energy-gev", type=float, default=100.0)
     ap.add_argument("--steps-per-gp", type=int, default=200)
     ap.add_argument("--plain-steps", type=int, default=40)
-    ap.add_argument("--cpu-particles", type=int, default=32768)
+    ap.add_argument("--cpu-particles", type=int, default=8192, help="CPU-baseline particles per host thread")
     ap.add_argument("--cpu-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -527,17 +574,24 @@ def main():
         log("[bench] D_mumu check on 65,536 particles")
         check = dmumu_check(ctx, F32, L, x, p, dt, 65536, 4001, stride)
 
-    stats = None if args.no_stats else stats_rates(ctx, F32, L, x, p, dt, log)
+    stats = None if args.no_stats else stats_rates(ctx, F32, L, x, p, dt, log, cpu=not args.no_cpu)
     inject = None if args.no_stats else injection_rates(ctx, F32, L, args.energy_gev, log)
     pcie = None if args.no_pcie else pcie_inclusive(F32, L, x, p, args.energy_gev, log)
 
     cpu = None
     if not args.no_cpu:
-        log(f"[bench] CPU oracle on {args.cpu_particles} particles x {args.cpu_steps} steps")
-        rate, el = cpu_baseline(F32, L, x, p, dt, args.cpu_particles, args.cpu_steps)
-        cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_particles} particles x {args.cpu_steps} pushes of the same field and dt, "
-                         f"FP64 oracle (C restatement of advance_particles_kernel!), mu every 20 steps, {el:.1f} s"}
+        threads = cpu_threads()
+        n_cpu = args.cpu_particles * threads
+        log(f"[bench] CPU oracle on {n_cpu} particles x {args.cpu_steps} steps, {threads} threads")
+        rate, el = cpu_baseline(F32, L, x, p, dt, n_cpu, args.cpu_steps, threads)
+        log(f"[bench] CPU oracle on {args.cpu_particles // 4} particles x {args.cpu_steps} steps, 1 thread")
+        rate1, el1 = cpu_baseline(F32, L, x, p, dt, args.cpu_particles // 4, args.cpu_steps, 1)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{n_cpu} particles x {args.cpu_steps} pushes of the same field and dt in {threads} host "
+                         f"threads, FP64 oracle (C restatement of advance_particles_kernel!), mu every 20 steps, "
+                         f"{el:.1f} s",
+               "single_core": {"value": rate1, "cores": 1,
+                               "sample": f"{args.cpu_particles // 4} particles x {args.cpu_steps} pushes, {el1:.1f} s"}}
     ctx.close()
     del F32
 
