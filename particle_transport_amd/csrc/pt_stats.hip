@@ -112,7 +112,8 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags,
         const float *__restrict__ lo_g, int nbins, float hi_last, int bin_abs, int injection,
         unsigned long long *__restrict__ g_cnt, double *__restrict__ g_s1, double *__restrict__ g_s2,
-        double *__restrict__ part_d, long long *__restrict__ part_i, int nbx, int ngy) {
+        double *__restrict__ part_d, long long *__restrict__ part_i, int nbx, int ngy,
+        const int32_t *__restrict__ fin, int lane32) {
     const TileGroup tg = tile_group(ngy);
     if (tg.tile >= nbx) return;  // whole block: before any barrier
     extern __shared__ unsigned char smem[];
@@ -158,10 +159,14 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     }
     int cur_bin = -1;
     float cur_lo = INFINITY, cur_hi = -INFINITY;
+    // starts of the current run visited on the finite fast path, where every
+    // lag's pair is valid: their count is added to each rc[g] at the flush
+    unsigned run = 0;
 
     auto flush = [&]() {
 #pragma unroll
         for (int g = 0; g < kG; ++g) {
+            rc[g] += run;
             if (cur_bin >= 0 && rc[g]) {
                 atomicAdd(&b_cnt[g * nbins + cur_bin], static_cast<unsigned long long>(rc[g]));
                 atomicAdd(&b_s1[g * nbins + cur_bin], r1[g]);
@@ -172,26 +177,46 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
             rc[g] = 0;
             r1[g] = r2[g] = 0.0;
         }
+        run = 0;
     };
+
+    int64_t lag_min = lag[0], lag_max = lag[0], off[kG];
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+        lag_min = lag[g] < lag_min ? lag[g] : lag_min;
+        lag_max = lag[g] > lag_max ? lag[g] : lag_max;
+        off[g] = lag[g] * ss;
+    }
+    const int64_t s_end = injection ? (lag_min < nsave ? 1 : 0) : (nsave - lag_min);
+    const int64_t s_full = nsave - lag_max;  // below it every lag of the group is in range
+    const int64_t s_main = s_end < s_full ? s_end : s_full;
+    // Finite fast path: every start below s_fin has a finite mu and finite
+    // partners at every lag of the group, so no pair needs a check.  s_fin is
+    // the smallest such bound over the wave's particles, so the walk below it
+    // advances the wave in step and addresses each sample row as one scalar
+    // pointer plus the lane's 32-bit byte offset (lane32: it fits).
+    int64_t s_fin = 0;
+    if (lane32) {
+        int64_t lim = s_main;
+        if (active) {
+            const int64_t f = static_cast<int64_t>(fin[j]) - lag_max;
+            lim = f < lim ? (f > 0 ? f : 0) : lim;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const int64_t t = __shfl_xor(lim, o, 64);
+            lim = t < lim ? t : lim;
+        }
+        s_fin = static_cast<int64_t>(__builtin_amdgcn_readfirstlane(static_cast<int32_t>(lim)));
+    }
 
     if (active) {
         const int64_t col = pidx ? pidx[j] : j;
         const float *row = mu + col * sp;  // sample s of this particle
-        int64_t lag_min = lag[0], lag_max = lag[0], off[kG];
-#pragma unroll
-        for (int g = 0; g < kG; ++g) {
-            lag_min = lag[g] < lag_min ? lag[g] : lag_min;
-            lag_max = lag[g] > lag_max ? lag[g] : lag_max;
-            off[g] = lag[g] * ss;
-        }
-        const int64_t s_end = injection ? (lag_min < nsave ? 1 : 0) : (nsave - lag_min);
-        const int64_t s_full = nsave - lag_max;  // below it every lag of the group is in range
 
         // One start sample: bin of mu[s] (a run boundary only when it
         // changes), then the pair with every lag, without branches.  `b[g]`
         // holds mu[s + lag[g]] (any value where `valid[g]` is false).
-        auto visit = [&](float a, const float *b, const bool *valid) {
-            if (!isfinite(a)) return;
+        auto rebin = [&](float a) {
             const float v = bin_abs ? fabsf(a) : a;
             if (!(v >= cur_lo && v < cur_hi)) {
                 flush();
@@ -217,6 +242,10 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
                     cur_hi = b0 + 1 < nbins ? lo[b0 + 1] : hi_last;
                 }
             }
+        };
+        auto visit = [&](float a, const float *b, const bool *valid) {
+            if (!isfinite(a)) return;
+            rebin(a);
             const double ad = static_cast<double>(a);
 #pragma unroll
             for (int g = 0; g < kG; ++g) {
@@ -227,12 +256,52 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
                 r2[g] = fma(d, d, r2[g]);
             }
         };
+        // a and every b[g] finite (start below the particle's first
+        // non-finite sample minus the group's largest lag)
+        auto visit_finite = [&](float a, const float *b) {
+            rebin(a);
+            const double ad = static_cast<double>(a);
+#pragma unroll
+            for (int g = 0; g < kG; ++g) {
+                const double d = static_cast<double>(b[g]) - ad;
+                r1[g] += d;
+                r2[g] = fma(d, d, r2[g]);
+            }
+            ++run;
+        };
         const bool all[kG] = {true, true, true, true, true, true, true, true};
         int64_t s = 0;
+        if (s_fin > 0) {
+            // three samples in flight in rotating registers
+            const uint32_t lane_bytes = static_cast<uint32_t>(col * sp * 4);
+            auto at = [&](const float *uniform_row) {
+                return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(uniform_row) + lane_bytes);
+            };
+            auto load = [&](int64_t t, float &a, float *b) {
+                const float *r = mu + t * ss;
+                a = at(r);
+#pragma unroll
+                for (int g = 0; g < kG; ++g) b[g] = at(r + off[g]);
+            };
+            float a0, a1, a2, b0[kG], b1[kG], b2[kG];
+            load(0, a0, b0);
+            if (s_fin > 1) load(1, a1, b1);
+            for (;;) {
+                if (s + 2 < s_fin) load(s + 2, a2, b2);
+                visit_finite(a0, b0);
+                if (++s == s_fin) break;
+                if (s + 2 < s_fin) load(s + 2, a0, b0);
+                visit_finite(a1, b1);
+                if (++s == s_fin) break;
+                if (s + 2 < s_fin) load(s + 2, a1, b1);
+                visit_finite(a2, b2);
+                if (++s == s_fin) break;
+            }
+            row += s * ss;
+        }
         // Every lag in range: the loads of sample s+1 are issued before the
         // pairs of sample s are accumulated, so their latency overlaps it.
-        const int64_t s_main = s_end < s_full ? s_end : s_full;
-        if (s_main > 0) {
+        if (s < s_main) {
             float an = row[0], bn[kG];
 #pragma unroll
             for (int g = 0; g < kG; ++g) bn[g] = row[off[g]];
@@ -296,6 +365,21 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     }
 }
 
+// fin[j] = index of the first non-finite sample of selected particle j (nsave
+// when the whole series is finite): the D_mumu walk needs no per-pair checks
+// below it.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_finite_prefix(const T *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, int64_t sp,
+                const int64_t *__restrict__ pidx, int32_t *__restrict__ fin) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n_sel) return;
+    const T *row = mu + (pidx ? pidx[j] : j) * sp;
+    int64_t s = 0;
+    while (s < nsave && isfinite(row[s * ss])) ++s;
+    fin[j] = static_cast<int32_t>(s);
+}
+
 // Merge per-workgroup partials of each lag in workgroup order.
 __global__ void k_merge_partials(const double *__restrict__ part_d, const long long *__restrict__ part_i,
                                  int nlags, int nbx, double *__restrict__ out_d,
@@ -349,7 +433,8 @@ __global__ void k_pmag(const float *__restrict__ mom, int64_t nsave, int64_t tot
 __global__ void __launch_bounds__(kTPB)
 k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
       const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags, double inv_p0,
-      double *__restrict__ part_d, long long *__restrict__ part_i, int nbx, int ngy) {
+      double *__restrict__ part_d, long long *__restrict__ part_i, int nbx, int ngy,
+      const int32_t *__restrict__ fin, int lane32) {
     const TileGroup tg = tile_group(ngy);
     if (tg.tile >= nbx) return;
     __shared__ double sh_d[kTPB / 64];
@@ -369,17 +454,32 @@ k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel
         cnt[g] = 0;
         s1[g] = s2[g] = s3[g] = s4[g] = 0.0;
     }
+    int64_t lag_min = lag[0], lag_max = lag[0], off[kG];
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+        lag_min = lag[g] < lag_min ? lag[g] : lag_min;
+        lag_max = lag[g] > lag_max ? lag[g] : lag_max;
+        off[g] = lag[g] * total;
+    }
+    const int64_t s_end = nsave - lag_min, s_full = nsave - lag_max;
+    const int64_t s_main = s_end < s_full ? s_end : s_full;
+    // Finite fast path below the wave's smallest s_fin, as in k_dmumu.
+    int64_t s_fin = 0;
+    if (lane32) {
+        int64_t lim = s_main;
+        if (j < n_sel) {
+            const int64_t f = static_cast<int64_t>(fin[j]) - lag_max;
+            lim = f < lim ? (f > 0 ? f : 0) : lim;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const int64_t t = __shfl_xor(lim, o, 64);
+            lim = t < lim ? t : lim;
+        }
+        s_fin = static_cast<int64_t>(__builtin_amdgcn_readfirstlane(static_cast<int32_t>(lim)));
+    }
     if (j < n_sel) {
         const int64_t col = pidx ? pidx[j] : j;
         const double *row = pm + col;
-        int64_t lag_min = lag[0], lag_max = lag[0], off[kG];
-#pragma unroll
-        for (int g = 0; g < kG; ++g) {
-            lag_min = lag[g] < lag_min ? lag[g] : lag_min;
-            lag_max = lag[g] > lag_max ? lag[g] : lag_max;
-            off[g] = lag[g] * total;
-        }
-        const int64_t s_end = nsave - lag_min, s_full = nsave - lag_max;
         // `pe[g]` = |p| at s + lag[g] (any value where `valid[g]` is false)
         auto visit = [&](double pst, const double *pe, const bool *valid) {
             if (!isfinite(pst)) return;
@@ -397,9 +497,49 @@ k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel
         };
         const bool all[kG] = {true, true, true, true, true, true, true, true};
         int64_t s = 0;
+        if (s_fin > 0) {
+            // every pair valid: the count is the number of starts
+            const uint32_t lane_bytes = static_cast<uint32_t>(col * 8);
+            auto at = [&](const double *uniform_row) {
+                return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(uniform_row) + lane_bytes);
+            };
+            auto load = [&](int64_t t, double &a, double *b) {
+                const double *r = pm + t * total;
+                a = at(r);
+#pragma unroll
+                for (int g = 0; g < kG; ++g) b[g] = at(r + off[g]);
+            };
+            auto visit_finite = [&](double pst, const double *pe) {
+#pragma unroll
+                for (int g = 0; g < kG; ++g) {
+                    const double d = pe[g] - pst;
+                    const double dn = d * inv_p0;
+                    s1[g] += d;
+                    s2[g] = fma(d, d, s2[g]);
+                    s3[g] += dn;
+                    s4[g] = fma(dn, dn, s4[g]);
+                }
+            };
+            double a0, a1, a2, b0[kG], b1[kG], b2[kG];
+            load(0, a0, b0);
+            if (s_fin > 1) load(1, a1, b1);
+            for (;;) {  // three samples in flight in rotating registers
+                if (s + 2 < s_fin) load(s + 2, a2, b2);
+                visit_finite(a0, b0);
+                if (++s == s_fin) break;
+                if (s + 2 < s_fin) load(s + 2, a0, b0);
+                visit_finite(a1, b1);
+                if (++s == s_fin) break;
+                if (s + 2 < s_fin) load(s + 2, a1, b1);
+                visit_finite(a2, b2);
+                if (++s == s_fin) break;
+            }
+#pragma unroll
+            for (int g = 0; g < kG; ++g) cnt[g] += static_cast<unsigned>(s);
+            row += s * total;
+        }
         // every lag in range: sample s+1 is loaded before sample s is used
-        const int64_t s_main = s_end < s_full ? s_end : s_full;
-        if (s_main > 0) {
+        if (s < s_main) {
             double an = row[0], bn[kG];
 #pragma unroll
             for (int g = 0; g < kG; ++g) bn[g] = row[off[g]];
@@ -746,10 +886,17 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
         auto *out_i = reinterpret_cast<long long *>(out_d + nl * 3);
         PT_HIP_CHECK(hipMemsetAsync(base, 0, nl * nb * 24, st));
 
+        DevBuf hfin;
+        auto *fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(n_sel) * 4));
+        k_finite_prefix<float><<<static_cast<unsigned>((n_sel + 255) / 256), 256, 0, st>>>(
+            dmu, nsave, n_sel, stride_s, stride_p, didx, fin);
+        PT_HIP_CHECK(hipGetLastError());
+        // the fast path's 32-bit lane offsets (bytes) cover every particle
+        const int lane32 = max_col * stride_p < (int64_t{1} << 30) && nsave < (int64_t{1} << 31) ? 1 : 0;
         const size_t shm = static_cast<size_t>(kG) * nb * 24 + nb * 4;
         k_dmumu<<<tile_group_blocks(nbx, ngy), kTPB, shm, st>>>(
             dmu, nsave, n_sel, stride_s, stride_p, didx, dlag, static_cast<int>(nlags), dlo, static_cast<int>(n_bins),
-            hi_last, bin_abs ? 1 : 0, start_mode, g_cnt, g_s1, g_s2, part_d, part_i, nbx, ngy);
+            hi_last, bin_abs ? 1 : 0, start_mode, g_cnt, g_s1, g_s2, part_d, part_i, nbx, ngy, fin, lane32);
         PT_HIP_CHECK(hipGetLastError());
         k_merge_partials<<<static_cast<unsigned>((nlags + 63) / 64), 64, 0, st>>>(part_d, part_i, static_cast<int>(nlags),
                                                                                   nbx, out_d, out_i);
@@ -823,11 +970,18 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         auto *out_i = reinterpret_cast<long long *>(out_d + nl * 4);
         std::vector<double> hd(nl * 4, 0.0), cd(nl * 4);
         std::vector<long long> hi(nl, 0), ci(nl);
+        DevBuf hfin;
+        auto *fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(std::max(chunk, n_sel)) * 4));
         auto run = [&](const double *pm, int64_t row_len, int64_t nsel_c, const int64_t *idx) {
             const int nbx = static_cast<int>((nsel_c + kTPB - 1) / kTPB);
+            k_finite_prefix<double><<<static_cast<unsigned>((nsel_c + 255) / 256), 256, 0, st>>>(
+                pm, nsave, nsel_c, row_len, 1, idx, fin);
+            PT_HIP_CHECK(hipGetLastError());
+            // the fast path's 32-bit lane offsets (bytes) cover every column
+            const int lane32 = row_len < (int64_t{1} << 29) && nsave < (int64_t{1} << 31) ? 1 : 0;
             k_dpp<<<tile_group_blocks(nbx, ngy), kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag,
                                                                 static_cast<int>(nlags), 1.0 / p0, part_d, part_i,
-                                                                nbx, ngy);
+                                                                nbx, ngy, fin, lane32);
             PT_HIP_CHECK(hipGetLastError());
             k_merge_dpp<<<static_cast<unsigned>((nlags + 63) / 64), 64, 0, st>>>(part_d, part_i,
                                                                                  static_cast<int>(nlags), nbx, out_d,

@@ -268,6 +268,54 @@ def test_dmumu_matches_oracle(ctx, injection, bin_abs):
         assert np.allclose(a, b, rtol=1e-11, atol=1e-12 * np.max(np.abs(b))), k
 
 
+@pytest.mark.parametrize("injection", [False, True])
+def test_dmumu_finite_walk_matches_oracle(ctx, injection):
+    """Mostly finite series, as a pusher writes them: most particles finite
+    throughout, some dead from a random sample on (NaN suffix), a few with
+    isolated NaN samples, mixed within each wavefront.  Host time-fastest
+    chunks, and a device particle-fastest cache with and without a selection
+    list, all against the oracle."""
+    from particle_transport_amd._lib import DeviceBuffer
+    rng = np.random.default_rng(21)
+    n, nsave = 700, 400
+    mu = _mu_series(n, nsave, 22, nan_frac=0.0)
+    dead = rng.random(n) < 0.1
+    for i in np.nonzero(dead)[0]:
+        mu[i, rng.integers(1, nsave):] = np.nan
+    gaps = (rng.random(n) < 0.05) & ~dead
+    for i in np.nonzero(gaps)[0]:
+        mu[i, rng.integers(0, nsave, 3)] = np.nan
+    lags = [1, 2, 3, 5, 8, 13, 21, 34, 55, 89, 144, 233, 377, 399, 400]
+    edges = np.linspace(0.0, 1.0, 21)
+    mode = "injection" if injection else "sliding"
+
+    def check(acc, oacc):
+        assert np.array_equal(acc.dmumu_counts, oacc["dmumu_counts"])
+        assert np.array_equal(acc.pair_counts, oacc["pair_counts"])
+        assert np.array_equal(acc.particle_counts, oacc["particle_counts"])
+        for k in ("dmumu_sum_delta", "dmumu_sum_delta2", "pair_sum_squares", "particle_means", "particle_m2s"):
+            a, b = getattr(acc, k), oacc[k]
+            assert np.allclose(a, b, rtol=1e-11, atol=1e-12 * np.max(np.abs(b))), k
+
+    acc = dmumu.new_accumulators(len(lags), 20)
+    dmumu.process_mu_chunk_dmumu(ctx, mu, lags, edges, acc, mode, True)
+    oacc = oracle.new_dmumu_accumulators(len(lags), 20)
+    oracle.dmumu(mu, lags, edges, injection, True, oacc)
+    check(acc, oacc)
+    dev = DeviceBuffer.from_host(ctx, np.ascontiguousarray(mu.T))
+    acc = dmumu.new_accumulators(len(lags), 20)
+    dmumu.process_mu_chunk_dmumu(ctx, dev, lags, edges, acc, mode, True, layout="particle_fastest")
+    check(acc, oacc)
+    sel = np.sort(rng.choice(n, 300, replace=False))
+    acc = dmumu.new_accumulators(len(lags), 20)
+    dmumu.process_mu_chunk_dmumu(ctx, dev, lags, edges, acc, mode, True, layout="particle_fastest",
+                                 particle_index=sel)
+    oacc = oracle.new_dmumu_accumulators(len(lags), 20)
+    oracle.dmumu(mu[sel], lags, edges, injection, True, oacc)
+    check(acc, oacc)
+    dev.free()
+
+
 def test_dmumu_device_cache_layout_and_selection(ctx):
     """mu cache [save][n] on the device with a particle selection list equals the
     reference's time-fastest chunk of the same particles."""
@@ -308,6 +356,27 @@ def test_dpp_and_energy_snapshots_match_oracle(ctx):
     oracle.energy_snapshots(mom, np.asarray(snaps) - 1, eo, 0)
     assert np.array_equal(np.isnan(e), np.isnan(eo))
     assert np.allclose(e[np.isfinite(eo)], eo[np.isfinite(eo)], rtol=1e-14)
+
+
+def test_dpp_finite_walk_matches_oracle(ctx):
+    """Mostly finite momenta with some particles dead from a random sample on
+    (NaN suffix), mixed within wavefronts, with and without a selection list."""
+    rng = np.random.default_rng(12)
+    nsave, n = 300, 900
+    mom = (rng.normal(0, 1e-18, (nsave, 3, n)) + 5e-17).astype(np.float32)
+    for i in np.nonzero(rng.random(n) < 0.1)[0]:
+        mom[rng.integers(1, nsave):, :, i] = np.nan
+    lags = [1, 2, 3, 5, 8, 13, 21, 34, 55, 89, 144, 233, 299, 300]
+    p0 = dpp.momentum_magnitude(*mom[0, :, 0])
+    for sel in (None, np.sort(rng.choice(n, 400, replace=False))):
+        acc = dpp.new_accumulators(len(lags))
+        sub = mom if sel is None else mom[:, :, sel]
+        dpp.process_momenta_chunk_dpp(ctx, mom, lags, p0, acc, particle_index=sel)
+        oacc = oracle.new_dpp_accumulators(len(lags))
+        oracle.dpp(sub, lags, p0, oacc)
+        assert np.array_equal(acc.counts, oacc["counts"])
+        for k in ("sum_delta_p", "sum_delta_p2", "sum_delta_p_norm", "sum_delta_p_norm2"):
+            assert np.allclose(getattr(acc, k), oacc[k], rtol=1e-11, atol=1e-12 * np.max(np.abs(oacc[k]))), k
 
 
 def test_dpp_from_fp64_pmag_cache_matches_oracle(ctx):
