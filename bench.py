@@ -584,14 +584,14 @@ def main():
         n_cpu = args.cpu_particles * threads
         log(f"[bench] CPU oracle on {n_cpu} particles x {args.cpu_steps} steps, {threads} threads")
         rate, el = cpu_baseline(F32, L, x, p, dt, n_cpu, args.cpu_steps, threads)
-        log(f"[bench] CPU oracle on {args.cpu_particles // 4} particles x {args.cpu_steps} steps, 1 thread")
-        rate1, el1 = cpu_baseline(F32, L, x, p, dt, args.cpu_particles // 4, args.cpu_steps, 1)
+        log(f"[bench] CPU oracle on {args.cpu_particles} particles x {args.cpu_steps} steps, 1 thread")
+        rate1, el1 = cpu_baseline(F32, L, x, p, dt, args.cpu_particles, args.cpu_steps, 1)
         cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": f"{n_cpu} particles x {args.cpu_steps} pushes of the same field and dt in {threads} host "
                          f"threads, FP64 oracle (C restatement of advance_particles_kernel!), mu every 20 steps, "
                          f"{el:.1f} s",
                "single_core": {"value": rate1, "cores": 1,
-                               "sample": f"{args.cpu_particles // 4} particles x {args.cpu_steps} pushes, {el1:.1f} s"}}
+                               "sample": f"{args.cpu_particles} particles x {args.cpu_steps} pushes, {el1:.1f} s"}}
     ctx.close()
     del F32
 
