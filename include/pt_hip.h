@@ -84,7 +84,8 @@ const char *pt_last_error(void);
  * default device and stream).  device: HIP ordinal. */
 int pt_ctx_create(int device, pt_ctx **out);
 int pt_ctx_destroy(pt_ctx *ctx);
-/* Device bytes currently held by the context's fields and ensembles. */
+/* Device bytes currently held by the context: fields, ensembles, buffers, its
+ * scratch, and the per-call temporaries it keeps for reuse (at most 64 MiB each). */
 int pt_ctx_bytes_in_use(const pt_ctx *ctx, int64_t *bytes);
 
 /* Device buffers owned by the context (cadence caches that stay in HBM and

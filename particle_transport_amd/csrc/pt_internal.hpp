@@ -63,6 +63,7 @@ struct pt_ctx {
     int64_t bytes_in_use = 0;
     void *scratch = nullptr;     // reusable device scratch (grown on demand)
     size_t scratch_bytes = 0;
+    void *pool = nullptr;        // cached per-call device buffers (pt_runtime.hip)
 };
 
 struct pt_field {
@@ -172,6 +173,12 @@ void launch_inject(const pt_field *f, const pt_inject_params &ip, int64_t first,
 
 // Device scratch owned by the context, grown on demand (never shrinks).
 void *ctx_scratch(pt_ctx *ctx, size_t bytes);
+// Per-call device buffers cached by the context: pool_get returns a buffer of
+// at least `bytes` (its size class in *granted), pool_put gives it back for a
+// later call once the context's stream no longer uses it.  Saves the
+// hipMalloc/hipFree pair, which synchronizes the device, on every call.
+void *pool_get(pt_ctx *ctx, size_t bytes, size_t *granted);
+void pool_put(pt_ctx *ctx, void *p, size_t granted);
 // Tracked device allocations (counted in pt_ctx_bytes_in_use).
 void *dev_alloc(pt_ctx *ctx, size_t bytes);
 void dev_free(pt_ctx *ctx, void *p, size_t bytes);

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 
 #include "pt_internal.hpp"
@@ -47,6 +48,53 @@ void dev_free(pt_ctx *ctx, void *p, size_t bytes) {
         (void)hipFree(p);
         ctx->bytes_in_use -= static_cast<int64_t>(bytes);
     }
+}
+
+// Size classes of 2^k bytes (at least 4 KiB); free buffers by class.
+struct BufferPool {
+    std::unordered_multimap<size_t, void *> free;
+};
+
+static size_t size_class(size_t bytes) {
+    size_t c = 4096;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
+void *pool_get(pt_ctx *ctx, size_t bytes, size_t *granted) {
+    if (!ctx->pool) ctx->pool = new BufferPool();
+    auto *pool = static_cast<BufferPool *>(ctx->pool);
+    const size_t c = size_class(bytes);
+    *granted = c;
+    auto it = pool->free.find(c);
+    if (it != pool->free.end()) {
+        void *p = it->second;
+        pool->free.erase(it);
+        return p;
+    }
+    return dev_alloc(ctx, c);
+}
+
+// Buffers above this size go back to the device at once: their allocation
+// cost is small next to the work they serve, and caching them would hold HBM.
+constexpr size_t kPoolMaxClass = size_t{64} << 20;
+
+void pool_put(pt_ctx *ctx, void *p, size_t granted) {
+    if (!p) return;
+    (void)hipStreamSynchronize(ctx->stream);  // idle unless an error cut the call short
+    if (granted > kPoolMaxClass) {
+        dev_free(ctx, p, granted);
+        return;
+    }
+    static_cast<BufferPool *>(ctx->pool)->free.emplace(granted, p);
+}
+
+static void pool_release(pt_ctx *ctx) {
+    auto *pool = static_cast<BufferPool *>(ctx->pool);
+    if (!pool) return;
+    for (auto &kv : pool->free) dev_free(ctx, kv.second, kv.first);
+    delete pool;
+    ctx->pool = nullptr;
 }
 
 void *ctx_scratch(pt_ctx *ctx, size_t bytes) {
@@ -450,6 +498,7 @@ int pt_ctx_destroy(pt_ctx *ctx) {
     return guarded([&] {
         if (!ctx) return;
         (void)hipStreamSynchronize(ctx->stream);
+        pool_release(ctx);
         if (ctx->scratch) (void)hipFree(ctx->scratch);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);

@@ -22,8 +22,9 @@ namespace pt {
 namespace {
 
 constexpr int kTPB = 256;
-constexpr int kG = 8;          // lags per workgroup
-constexpr int kMaxBins = 256;  // LDS budget: kG * bins * 24 B
+constexpr int kG = 8;          // lags per workgroup (D_pp)
+constexpr int kGmu = 4;        // lags per workgroup (D_mumu: its run accumulators take more registers)
+constexpr int kMaxBins = 256;  // LDS budget: kGmu * bins * 24 B
 
 struct Welford {
     double n, mean, m2;
@@ -118,14 +119,14 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     if (tg.tile >= nbx) return;  // whole block: before any barrier
     extern __shared__ unsigned char smem[];
     unsigned long long *b_cnt = reinterpret_cast<unsigned long long *>(smem);
-    double *b_s1 = reinterpret_cast<double *>(b_cnt + kG * nbins);
-    double *b_s2 = b_s1 + kG * nbins;
-    float *lo = reinterpret_cast<float *>(b_s2 + kG * nbins);
+    double *b_s1 = reinterpret_cast<double *>(b_cnt + kGmu * nbins);
+    double *b_s2 = b_s1 + kGmu * nbins;
+    float *lo = reinterpret_cast<float *>(b_s2 + kGmu * nbins);
     __shared__ Welford sh_w[kTPB / 64];
     __shared__ double sh_d[kTPB / 64];
     __shared__ long long sh_i[kTPB / 64];
 
-    for (int i = threadIdx.x; i < kG * nbins; i += kTPB) {
+    for (int i = threadIdx.x; i < kGmu * nbins; i += kTPB) {
         b_cnt[i] = 0;
         b_s1[i] = 0.0;
         b_s2[i] = 0.0;
@@ -136,11 +137,11 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     const int gy = tg.group;
     // The last group is padded with its first lag: every load stays in range
     // and the padded slots are never written out (li < nlags below).
-    int64_t lag[kG];
+    int64_t lag[kGmu];
 #pragma unroll
-    for (int g = 0; g < kG; ++g) {
-        const int li = gy * kG + g;
-        lag[g] = lags[li < nlags ? li : gy * kG];
+    for (int g = 0; g < kGmu; ++g) {
+        const int li = gy * kGmu + g;
+        lag[g] = lags[li < nlags ? li : gy * kGmu];
     }
     const int64_t j = static_cast<int64_t>(tg.tile) * kTPB + threadIdx.x;
     const bool active = j < n_sel;
@@ -149,10 +150,10 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     // whose start was outside the binned range is dropped at the flush) and
     // the particle's totals over all valid pairs (sum dmu^2, count), folded
     // in at each flush.
-    unsigned rc[kG], pc[kG];
-    double r1[kG], r2[kG], ps[kG];
+    unsigned rc[kGmu], pc[kGmu];
+    double r1[kGmu], r2[kGmu], ps[kGmu];
 #pragma unroll
-    for (int g = 0; g < kG; ++g) {
+    for (int g = 0; g < kGmu; ++g) {
         rc[g] = 0;
         pc[g] = 0;
         r1[g] = r2[g] = ps[g] = 0.0;
@@ -165,7 +166,7 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
 
     auto flush = [&]() {
 #pragma unroll
-        for (int g = 0; g < kG; ++g) {
+        for (int g = 0; g < kGmu; ++g) {
             rc[g] += run;
             if (cur_bin >= 0 && rc[g]) {
                 atomicAdd(&b_cnt[g * nbins + cur_bin], static_cast<unsigned long long>(rc[g]));
@@ -180,9 +181,9 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         run = 0;
     };
 
-    int64_t lag_min = lag[0], lag_max = lag[0], off[kG];
+    int64_t lag_min = lag[0], lag_max = lag[0], off[kGmu];
 #pragma unroll
-    for (int g = 0; g < kG; ++g) {
+    for (int g = 0; g < kGmu; ++g) {
         lag_min = lag[g] < lag_min ? lag[g] : lag_min;
         lag_max = lag[g] > lag_max ? lag[g] : lag_max;
         off[g] = lag[g] * ss;
@@ -248,7 +249,7 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
             rebin(a);
             const double ad = static_cast<double>(a);
 #pragma unroll
-            for (int g = 0; g < kG; ++g) {
+            for (int g = 0; g < kGmu; ++g) {
                 const bool ok = valid[g] && isfinite(b[g]);
                 const double d = ok ? static_cast<double>(b[g]) - ad : 0.0;
                 rc[g] += ok ? 1u : 0u;
@@ -262,14 +263,16 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
             rebin(a);
             const double ad = static_cast<double>(a);
 #pragma unroll
-            for (int g = 0; g < kG; ++g) {
+            for (int g = 0; g < kGmu; ++g) {
                 const double d = static_cast<double>(b[g]) - ad;
                 r1[g] += d;
                 r2[g] = fma(d, d, r2[g]);
             }
             ++run;
         };
-        const bool all[kG] = {true, true, true, true, true, true, true, true};
+        bool all[kGmu];
+#pragma unroll
+        for (int g = 0; g < kGmu; ++g) all[g] = true;
         int64_t s = 0;
         if (s_fin > 0) {
             // three samples in flight in rotating registers
@@ -281,9 +284,9 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
                 const float *r = mu + t * ss;
                 a = at(r);
 #pragma unroll
-                for (int g = 0; g < kG; ++g) b[g] = at(r + off[g]);
+                for (int g = 0; g < kGmu; ++g) b[g] = at(r + off[g]);
             };
-            float a0, a1, a2, b0[kG], b1[kG], b2[kG];
+            float a0, a1, a2, b0[kGmu], b1[kGmu], b2[kGmu];
             load(0, a0, b0);
             if (s_fin > 1) load(1, a1, b1);
             for (;;) {
@@ -302,28 +305,28 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         // Every lag in range: the loads of sample s+1 are issued before the
         // pairs of sample s are accumulated, so their latency overlaps it.
         if (s < s_main) {
-            float an = row[0], bn[kG];
+            float an = row[0], bn[kGmu];
 #pragma unroll
-            for (int g = 0; g < kG; ++g) bn[g] = row[off[g]];
+            for (int g = 0; g < kGmu; ++g) bn[g] = row[off[g]];
             for (; s < s_main; ++s) {
                 const float a = an;
-                float b[kG];
+                float b[kGmu];
 #pragma unroll
-                for (int g = 0; g < kG; ++g) b[g] = bn[g];
+                for (int g = 0; g < kGmu; ++g) b[g] = bn[g];
                 row += ss;
                 if (s + 1 < s_main) {
                     an = row[0];
 #pragma unroll
-                    for (int g = 0; g < kG; ++g) bn[g] = row[off[g]];
+                    for (int g = 0; g < kGmu; ++g) bn[g] = row[off[g]];
                 }
                 visit(a, b, all);
             }
         }
         for (; s < s_end; ++s, row += ss) {
-            float b[kG];
-            bool valid[kG];
+            float b[kGmu];
+            bool valid[kGmu];
 #pragma unroll
-            for (int g = 0; g < kG; ++g) {
+            for (int g = 0; g < kGmu; ++g) {
                 valid[g] = s + lag[g] < nsave;  // uniform
                 b[g] = 0.0f;
                 if (valid[g]) b[g] = row[off[g]];
@@ -334,8 +337,8 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     }
 
 #pragma unroll
-    for (int g = 0; g < kG; ++g) {
-        const int li = gy * kG + g;
+    for (int g = 0; g < kGmu; ++g) {
+        const int li = gy * kGmu + g;
         Welford w{0, 0, 0};
         if (active && pc[g] > 0) {
             w.n = 1.0;
@@ -354,9 +357,9 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kG * nbins; i += kTPB) {
+    for (int i = threadIdx.x; i < kGmu * nbins; i += kTPB) {
         const int g = i / nbins, b = i % nbins;
-        const int li = gy * kG + g;
+        const int li = gy * kGmu + g;
         if (li < nlags && b_cnt[i]) {
             atomicAdd(&g_cnt[static_cast<int64_t>(li) * nbins + b], b_cnt[i]);
             atomicAdd(&g_s1[static_cast<int64_t>(li) * nbins + b], b_s1[i]);
@@ -495,7 +498,9 @@ k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel
                 s4[g] = fma(dn, dn, s4[g]);
             }
         };
-        const bool all[kG] = {true, true, true, true, true, true, true, true};
+        bool all[kG];
+#pragma unroll
+        for (int g = 0; g < kG; ++g) all[g] = true;
         int64_t s = 0;
         if (s_fin > 0) {
             // every pair valid: the count is the number of starts
@@ -705,17 +710,24 @@ int guarded(F &&fn) {
     }
 }
 
-// Bring a host or device array into device memory (owned copy in `hold`).
+// A device buffer for the duration of one call, from the context's pool.
 struct DevBuf {
+    pt_ctx *ctx;
     void *p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
+    size_t granted = 0;
+    explicit DevBuf(pt_ctx *c) : ctx(c) {}
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { pool_put(ctx, p, granted); }
     void *alloc(size_t bytes) {
-        PT_HIP_CHECK(hipMalloc(&p, bytes ? bytes : 1));
+        pool_put(ctx, p, granted);
+        p = nullptr;
+        p = pool_get(ctx, bytes ? bytes : 1, &granted);
         return p;
     }
 };
+
+// Bring a host or device array into device memory (owned copy in `hold`).
 
 template <class T>
 const T *to_device(const T *src, int on_device, size_t count, DevBuf &hold, hipStream_t s) {
@@ -774,7 +786,7 @@ int pt_mu_from_phase(pt_ctx *ctx, const pt_field *field, int64_t chunk, int64_t 
         PT_HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t st = ctx->stream;
         const size_t cnt = static_cast<size_t>(chunk * 3 * nsave);
-        DevBuf hp, hm, ho;
+        DevBuf hp(ctx), hm(ctx), ho(ctx);
         const size_t esz = src_dtype == PT_F64 ? 8 : 4;
         const void *dp = positions, *dm = momenta;
         if (!src_on_device) {
@@ -864,14 +876,14 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
             }
         }
         const size_t mu_count = static_cast<size_t>((nsave - 1) * stride_s + max_col * stride_p + 1);
-        DevBuf hmu, hidx, hlag, hlo, hout;
+        DevBuf hmu(ctx), hidx(ctx), hlag(ctx), hlo(ctx), hout(ctx);
         const float *dmu = to_device(mu, mu_on_device, mu_count, hmu, st);
         const int64_t *didx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
         const int64_t *dlag = to_device(lags.data(), 0, lags.size(), hlag, st);
         const float *dlo = to_device(lo.data(), 0, lo.size(), hlo, st);
 
         const int nbx = static_cast<int>((n_sel + kTPB - 1) / kTPB);
-        const int ngy = static_cast<int>((nlags + kG - 1) / kG);
+        const int ngy = static_cast<int>((nlags + kGmu - 1) / kGmu);
         const size_t nb = static_cast<size_t>(n_bins), nl = static_cast<size_t>(nlags);
         const size_t parts = nl * static_cast<size_t>(nbx);
         // device layout: counts u64 [nlags][nbins], s1, s2, part_d, part_i, out_d, out_i
@@ -886,14 +898,14 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
         auto *out_i = reinterpret_cast<long long *>(out_d + nl * 3);
         PT_HIP_CHECK(hipMemsetAsync(base, 0, nl * nb * 24, st));
 
-        DevBuf hfin;
+        DevBuf hfin(ctx);
         auto *fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(n_sel) * 4));
         k_finite_prefix<float><<<static_cast<unsigned>((n_sel + 255) / 256), 256, 0, st>>>(
             dmu, nsave, n_sel, stride_s, stride_p, didx, fin);
         PT_HIP_CHECK(hipGetLastError());
         // the fast path's 32-bit lane offsets (bytes) cover every particle
         const int lane32 = max_col * stride_p < (int64_t{1} << 30) && nsave < (int64_t{1} << 31) ? 1 : 0;
-        const size_t shm = static_cast<size_t>(kG) * nb * 24 + nb * 4;
+        const size_t shm = static_cast<size_t>(kGmu) * nb * 24 + nb * 4;
         k_dmumu<<<tile_group_blocks(nbx, ngy), kTPB, shm, st>>>(
             dmu, nsave, n_sel, stride_s, stride_p, didx, dlag, static_cast<int>(nlags), dlo, static_cast<int>(n_bins),
             hi_last, bin_abs ? 1 : 0, start_mode, g_cnt, g_s1, g_s2, part_d, part_i, nbx, ngy, fin, lane32);
@@ -949,7 +961,7 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         if (n_sel == 0) return;
         PT_HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t st = ctx->stream;
-        DevBuf hm, hidx, hlag, hout, hpm;
+        DevBuf hm(ctx), hidx(ctx), hlag(ctx), hout(ctx), hpm(ctx);
         const Src *dm = to_device(momenta, on_device, static_cast<size_t>(nsave * comps * total), hm, st);
         const int64_t *didx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
         const int64_t *dlag = to_device(lags.data(), 0, lags.size(), hlag, st);
@@ -970,7 +982,7 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         auto *out_i = reinterpret_cast<long long *>(out_d + nl * 4);
         std::vector<double> hd(nl * 4, 0.0), cd(nl * 4);
         std::vector<long long> hi(nl, 0), ci(nl);
-        DevBuf hfin;
+        DevBuf hfin(ctx);
         auto *fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(std::max(chunk, n_sel)) * 4));
         auto run = [&](const double *pm, int64_t row_len, int64_t nsel_c, const int64_t *idx) {
             const int nbx = static_cast<int>((nsel_c + kTPB - 1) / kTPB);
@@ -1055,7 +1067,7 @@ int pt_energy_snapshots(pt_ctx *ctx, const float *momenta, int on_device, int64_
         if (n_sel == 0) return;
         PT_HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t st = ctx->stream;
-        DevBuf hm, hidx, hs, ho;
+        DevBuf hm(ctx), hidx(ctx), hs(ctx), ho(ctx);
         const float *dm = to_device(momenta, on_device, static_cast<size_t>(nsave * 3 * total), hm, st);
         const int64_t *didx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
         const int64_t *dsn = to_device(snapshot_idx, 0, static_cast<size_t>(k), hs, st);
