@@ -369,41 +369,88 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
 }
 
 // fin[j] = index of the first non-finite sample of selected particle j (nsave
-// when the whole series is finite): the D_mumu walk needs no per-pair checks
-// below it.
+// when the whole series is finite): the pair walks need no per-pair checks
+// below it.  Threads cover (particle, run of samples) tiles, so the read is
+// coalesced for the [save][n] layout and many loads are in flight;
+// non-finite samples (rare) lower fin[j] with an atomic min.  fin must hold
+// nsave beforehand.
+__global__ void __launch_bounds__(256) k_fill_i32(int32_t *__restrict__ a, int64_t n, int32_t v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+constexpr int kPrefixRun = 16;  // samples per thread of k_finite_prefix
+
 template <class T>
 __global__ void __launch_bounds__(256)
 k_finite_prefix(const T *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, int64_t sp,
                 const int64_t *__restrict__ pidx, int32_t *__restrict__ fin) {
+    // x: particle (coalesced for the [save][n] layout); y: a run of kPrefixRun
+    // samples, all loaded before any is tested so the loads overlap
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= n_sel) return;
     const T *row = mu + (pidx ? pidx[j] : j) * sp;
-    int64_t s = 0;
-    while (s < nsave && isfinite(row[s * ss])) ++s;
-    fin[j] = static_cast<int32_t>(s);
+    for (int64_t s0 = static_cast<int64_t>(blockIdx.y) * kPrefixRun; s0 < nsave;
+         s0 += static_cast<int64_t>(gridDim.y) * kPrefixRun) {
+        T v[kPrefixRun];
+#pragma unroll
+        for (int q = 0; q < kPrefixRun; ++q) v[q] = s0 + q < nsave ? row[(s0 + q) * ss] : T(0);
+        int first = -1;
+#pragma unroll
+        for (int q = kPrefixRun - 1; q >= 0; --q)
+            if (!isfinite(v[q])) first = q;
+        if (first >= 0) {
+            atomicMin(&fin[j], static_cast<int32_t>(s0 + first));
+            return;  // later runs can only hold larger indices
+        }
+    }
+}
+
+// Fill fin (n_sel values) for the walks; see k_finite_prefix.
+template <class T>
+void launch_finite_prefix(const T *mu, int64_t nsave, int64_t n_sel, int64_t ss, int64_t sp, const int64_t *pidx,
+                          int32_t *fin, hipStream_t st) {
+    k_fill_i32<<<static_cast<unsigned>((n_sel + 255) / 256), 256, 0, st>>>(fin, n_sel, static_cast<int32_t>(nsave));
+    PT_HIP_CHECK(hipGetLastError());
+    const int64_t runs = (nsave + kPrefixRun - 1) / kPrefixRun;
+    const dim3 grid(static_cast<unsigned>((n_sel + 255) / 256), static_cast<unsigned>(runs < 65535 ? runs : 65535));
+    k_finite_prefix<T><<<grid, 256, 0, st>>>(mu, nsave, n_sel, ss, sp, pidx, fin);
+    PT_HIP_CHECK(hipGetLastError());
 }
 
 // Merge per-workgroup partials of each lag in workgroup order.
-__global__ void k_merge_partials(const double *__restrict__ part_d, const long long *__restrict__ part_i,
-                                 int nlags, int nbx, double *__restrict__ out_d,
-                                 long long *__restrict__ out_i) {
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= nlags) return;
+// One workgroup per lag: thread t merges a contiguous run of partials in
+// workgroup order, then the fixed tree of block_welford/block_sum_*, so the
+// result is deterministic.
+__global__ void __launch_bounds__(kTPB)
+k_merge_partials(const double *__restrict__ part_d, const long long *__restrict__ part_i, int nlags, int nbx,
+                 double *__restrict__ out_d, long long *__restrict__ out_i) {
+    __shared__ Welford sh_w[kTPB / 64];
+    __shared__ double sh_d[kTPB / 64];
+    __shared__ long long sh_i[kTPB / 64];
+    const int li = blockIdx.x;
+    const int per = (nbx + kTPB - 1) / kTPB;
+    const int b0 = threadIdx.x * per, b1 = b0 + per < nbx ? b0 + per : nbx;
     Welford w{0, 0, 0};
     double pss = 0.0;
     long long pcs = 0;
-    for (int b = 0; b < nbx; ++b) {
+    for (int b = b0; b < b1; ++b) {
         const int64_t o = static_cast<int64_t>(li) * nbx + b;
         Welford x{static_cast<double>(part_i[o * 2]), part_d[o * 3], part_d[o * 3 + 1]};
         w = chan(w, x);
         pss += part_d[o * 3 + 2];
         pcs += part_i[o * 2 + 1];
     }
-    out_d[li * 3 + 0] = w.mean;
-    out_d[li * 3 + 1] = w.m2;
-    out_d[li * 3 + 2] = pss;
-    out_i[li * 2 + 0] = static_cast<long long>(w.n);
-    out_i[li * 2 + 1] = pcs;
+    const Welford bw = block_welford(w, sh_w);
+    const double ps = block_sum_d(pss, sh_d);
+    const long long pc = block_sum_i(pcs, sh_i);
+    if (threadIdx.x == 0 && li < nlags) {
+        out_d[li * 3 + 0] = bw.mean;
+        out_d[li * 3 + 1] = bw.m2;
+        out_d[li * 3 + 2] = ps;
+        out_i[li * 2 + 0] = static_cast<long long>(bw.n);
+        out_i[li * 2 + 1] = pc;
+    }
 }
 
 // ------------------------------------------------------------------ D_pp ---
@@ -593,13 +640,17 @@ k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel
     }
 }
 
-__global__ void k_merge_dpp(const double *__restrict__ part_d, const long long *__restrict__ part_i,
-                            int nlags, int nbx, double *__restrict__ out_d, long long *__restrict__ out_i) {
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= nlags) return;
+__global__ void __launch_bounds__(kTPB)
+k_merge_dpp(const double *__restrict__ part_d, const long long *__restrict__ part_i, int nlags, int nbx,
+            double *__restrict__ out_d, long long *__restrict__ out_i) {
+    __shared__ double sh_d[kTPB / 64];
+    __shared__ long long sh_i[kTPB / 64];
+    const int li = blockIdx.x;
+    const int per = (nbx + kTPB - 1) / kTPB;
+    const int b0 = threadIdx.x * per, b1 = b0 + per < nbx ? b0 + per : nbx;
     double a = 0, b = 0, c = 0, d = 0;
     long long n = 0;
-    for (int k = 0; k < nbx; ++k) {
+    for (int k = b0; k < b1; ++k) {
         const int64_t o = static_cast<int64_t>(li) * nbx + k;
         n += part_i[o];
         a += part_d[o * 4 + 0];
@@ -607,11 +658,18 @@ __global__ void k_merge_dpp(const double *__restrict__ part_d, const long long *
         c += part_d[o * 4 + 2];
         d += part_d[o * 4 + 3];
     }
-    out_i[li] = n;
-    out_d[li * 4 + 0] = a;
-    out_d[li * 4 + 1] = b;
-    out_d[li * 4 + 2] = c;
-    out_d[li * 4 + 3] = d;
+    a = block_sum_d(a, sh_d);
+    b = block_sum_d(b, sh_d);
+    c = block_sum_d(c, sh_d);
+    d = block_sum_d(d, sh_d);
+    n = block_sum_i(n, sh_i);
+    if (threadIdx.x == 0 && li < nlags) {
+        out_i[li] = n;
+        out_d[li * 4 + 0] = a;
+        out_d[li * 4 + 1] = b;
+        out_d[li * 4 + 2] = c;
+        out_d[li * 4 + 3] = d;
+    }
 }
 
 // kinetic_energy_gev_from_pmag (compute_dpp_full.jl:236-240) at snapshot saves.
@@ -900,9 +958,7 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
 
         DevBuf hfin(ctx);
         auto *fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(n_sel) * 4));
-        k_finite_prefix<float><<<static_cast<unsigned>((n_sel + 255) / 256), 256, 0, st>>>(
-            dmu, nsave, n_sel, stride_s, stride_p, didx, fin);
-        PT_HIP_CHECK(hipGetLastError());
+        launch_finite_prefix(dmu, nsave, n_sel, stride_s, stride_p, didx, fin, st);
         // the fast path's 32-bit lane offsets (bytes) cover every particle
         const int lane32 = max_col * stride_p < (int64_t{1} << 30) && nsave < (int64_t{1} << 31) ? 1 : 0;
         const size_t shm = static_cast<size_t>(kGmu) * nb * 24 + nb * 4;
@@ -910,8 +966,8 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
             dmu, nsave, n_sel, stride_s, stride_p, didx, dlag, static_cast<int>(nlags), dlo, static_cast<int>(n_bins),
             hi_last, bin_abs ? 1 : 0, start_mode, g_cnt, g_s1, g_s2, part_d, part_i, nbx, ngy, fin, lane32);
         PT_HIP_CHECK(hipGetLastError());
-        k_merge_partials<<<static_cast<unsigned>((nlags + 63) / 64), 64, 0, st>>>(part_d, part_i, static_cast<int>(nlags),
-                                                                                  nbx, out_d, out_i);
+        k_merge_partials<<<static_cast<unsigned>(nlags), kTPB, 0, st>>>(part_d, part_i, static_cast<int>(nlags), nbx,
+                                                                        out_d, out_i);
         PT_HIP_CHECK(hipGetLastError());
         std::vector<unsigned long long> hc(nl * nb);
         std::vector<double> h1(nl * nb), h2(nl * nb), hd(nl * 3);
@@ -986,18 +1042,15 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         auto *fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(std::max(chunk, n_sel)) * 4));
         auto run = [&](const double *pm, int64_t row_len, int64_t nsel_c, const int64_t *idx) {
             const int nbx = static_cast<int>((nsel_c + kTPB - 1) / kTPB);
-            k_finite_prefix<double><<<static_cast<unsigned>((nsel_c + 255) / 256), 256, 0, st>>>(
-                pm, nsave, nsel_c, row_len, 1, idx, fin);
-            PT_HIP_CHECK(hipGetLastError());
+            launch_finite_prefix(pm, nsave, nsel_c, row_len, int64_t{1}, idx, fin, st);
             // the fast path's 32-bit lane offsets (bytes) cover every column
             const int lane32 = row_len < (int64_t{1} << 29) && nsave < (int64_t{1} << 31) ? 1 : 0;
             k_dpp<<<tile_group_blocks(nbx, ngy), kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag,
                                                                 static_cast<int>(nlags), 1.0 / p0, part_d, part_i,
                                                                 nbx, ngy, fin, lane32);
             PT_HIP_CHECK(hipGetLastError());
-            k_merge_dpp<<<static_cast<unsigned>((nlags + 63) / 64), 64, 0, st>>>(part_d, part_i,
-                                                                                 static_cast<int>(nlags), nbx, out_d,
-                                                                                 out_i);
+            k_merge_dpp<<<static_cast<unsigned>(nlags), kTPB, 0, st>>>(part_d, part_i, static_cast<int>(nlags), nbx,
+                                                                       out_d, out_i);
             PT_HIP_CHECK(hipGetLastError());
             PT_HIP_CHECK(hipMemcpyAsync(cd.data(), out_d, nl * 32, hipMemcpyDeviceToHost, st));
             PT_HIP_CHECK(hipMemcpyAsync(ci.data(), out_i, nl * 8, hipMemcpyDeviceToHost, st));
