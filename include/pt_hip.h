@@ -87,6 +87,9 @@ int pt_ctx_destroy(pt_ctx *ctx);
 /* Device bytes currently held by the context: fields, ensembles, buffers, its
  * scratch, and the per-call temporaries it keeps for reuse (at most 64 MiB each). */
 int pt_ctx_bytes_in_use(const pt_ctx *ctx, int64_t *bytes);
+/* Free and total memory of the context's device (hipMemGetInfo), for
+ * monitoring a serving process.  No reference counterpart. */
+int pt_device_memory(const pt_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
 
 /* Device buffers owned by the context (cadence caches that stay in HBM and
  * feed the statistics entry points with on_device = 1).  No reference

@@ -23,7 +23,7 @@ OUT = {"none": 0, "phase": 1, "phase-space": 1, "mu": 2}
 
 # Every symbol include/pt_hip.h declares.
 EXPORTS = (
-    "pt_abi_version", "pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_ctx_bytes_in_use",
+    "pt_abi_version", "pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_ctx_bytes_in_use", "pt_device_memory",
     "pt_field_create", "pt_field_destroy", "pt_field_mean_abs_b", "pt_mean_abs_b",
     "pt_ens_create", "pt_ens_create_injected", "pt_inject", "pt_ens_destroy", "pt_ens_advance",
     "pt_ens_get_state", "pt_ens_last_timing", "pt_ens_sort",
@@ -79,6 +79,7 @@ def load():
         "pt_ctx_create": (_int, [_int, ctypes.POINTER(_vp)]),
         "pt_ctx_destroy": (_int, [_vp]),
         "pt_ctx_bytes_in_use": (_int, [_vp, _i64p]),
+        "pt_device_memory": (_int, [_vp, _i64p, _i64p]),
         "pt_field_create": (_int, [_vp, _i64, _i64, _i64, ctypes.c_double, _int, _int, _int,
                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
         "pt_field_destroy": (_int, [_vp]),

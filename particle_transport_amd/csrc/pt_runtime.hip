@@ -514,6 +514,17 @@ int pt_ctx_bytes_in_use(const pt_ctx *ctx, int64_t *bytes) {
     });
 }
 
+int pt_device_memory(const pt_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && free_bytes && total_bytes, "pt_device_memory: null argument");
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        size_t f = 0, t = 0;
+        PT_HIP_CHECK(hipMemGetInfo(&f, &t));
+        *free_bytes = static_cast<int64_t>(f);
+        *total_bytes = static_cast<int64_t>(t);
+    });
+}
+
 int pt_buffer_alloc(pt_ctx *ctx, int64_t bytes, void **dev_ptr) {
     return guarded([&] {
         PT_REQUIRE(ctx && dev_ptr, "pt_buffer_alloc: null argument");

@@ -256,6 +256,15 @@ struct pt_serve {
     std::deque<int> inflight;  // slots of enqueued, not yet completed ticks, oldest first
     int64_t particle_steps = 0;
     double kernel_ms = 0.0;
+    // Arenas come from the device's stream-ordered pool (hipMallocAsync),
+    // which keeps freed memory reserved: with requests of every size, and
+    // frees that race the next ticks, that reserve grows without bound, and
+    // the pool gives it back only after a device-wide synchronisation.  So
+    // once kTrimSlack bytes of arenas have been released, step()
+    // synchronises the device and trims the pool; admission does the same
+    // before retrying an allocation that failed.
+    hipMemPool_t mempool = nullptr;
+    int64_t freed_since_trim = 0;
 };
 
 namespace pt {
@@ -327,10 +336,25 @@ void launch_init(pt_serve *srv, SReq &r, const double *dinit) {
     PT_HIP_CHECK(hipGetLastError());
 }
 
+// Synchronise the device (the pool releases freed arenas only then) and give
+// the pool's unused memory back to the device.
+constexpr int64_t kTrimSlack = int64_t{8} << 30;
+
+void trim_pool(pt_serve *srv) {
+    PT_HIP_CHECK(hipDeviceSynchronize());
+    PT_HIP_CHECK(hipMemPoolTrimTo(srv->mempool, 0));
+    srv->freed_since_trim = 0;
+}
+
 void admit(pt_serve *srv, SReq &r) {
     hipStream_t st = srv->ctx->stream;
     void *mem = nullptr;
     hipError_t e = hipMallocAsync(&mem, r.bytes, st);
+    if (e == hipErrorOutOfMemory) {  // freed arenas may still sit in the pool
+        (void)hipGetLastError();
+        trim_pool(srv);
+        e = hipMallocAsync(&mem, r.bytes, st);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         throw Error{PT_ERR_MEMORY, std::string("hipMallocAsync failed: ") + hipGetErrorString(e)};
@@ -470,6 +494,7 @@ int pt_serve_create(pt_ctx *ctx, int64_t hbm_budget_bytes, int32_t steps_per_tic
             PT_HIP_CHECK(hipEventCreate(&t.ev0));
             PT_HIP_CHECK(hipEventCreate(&t.ev1));
         }
+        PT_HIP_CHECK(hipDeviceGetDefaultMemPool(&s->mempool, ctx->device));
         *out = s;
     });
 }
@@ -488,6 +513,7 @@ int pt_serve_destroy(pt_serve *srv) {
                 srv->ctx->bytes_in_use -= static_cast<int64_t>(kv.second.bytes);
             }
         (void)hipStreamSynchronize(st);
+        if (srv->mempool) (void)hipMemPoolTrimTo(srv->mempool, 0);
         for (auto &t : srv->tick) {
             if (t.d_table) (void)hipFree(t.d_table);
             if (t.h_table) (void)hipHostFree(t.h_table);
@@ -576,6 +602,7 @@ int pt_serve_step(pt_serve *srv, int64_t *n_running, int64_t *n_queued, int64_t 
         PT_HIP_CHECK(hipSetDevice(srv->ctx->device));
         // keep at most one tick in flight while this call enqueues the next
         while (srv->inflight.size() > 1) complete_oldest(srv);
+        if (srv->freed_since_trim > kTrimSlack) trim_pool(srv);
         while (!srv->queue.empty() && static_cast<int32_t>(srv->active.size()) < srv->max_active) {
             SReq &r = srv->reqs.at(srv->queue.front());
             if (srv->reserved + static_cast<int64_t>(r.bytes) > srv->budget) break;  // FIFO: wait for memory
@@ -658,6 +685,7 @@ int pt_serve_release(pt_serve *srv, int64_t request_id) {
             PT_HIP_CHECK(hipFreeAsync(r.mem, srv->ctx->stream));
             srv->ctx->bytes_in_use -= static_cast<int64_t>(r.bytes);
             srv->reserved -= static_cast<int64_t>(r.bytes);
+            srv->freed_since_trim += static_cast<int64_t>(r.bytes);
             r.mem = nullptr;
         }
         srv->reqs.erase(it);

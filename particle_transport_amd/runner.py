@@ -79,6 +79,12 @@ class Context:
         check(_lib.load().pt_ctx_bytes_in_use(self.h, ctypes.byref(v)))
         return v.value
 
+    def device_memory(self):
+        """(free, total) bytes of the device (pt_device_memory)."""
+        f, t = ctypes.c_int64(), ctypes.c_int64()
+        check(_lib.load().pt_device_memory(self.h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
+
     def mean_abs_b(self, Bx, By, Bz) -> float:
         """reference_B0_T (time_units.jl:18-20) as an FP64 device reduction."""
         keep = []

@@ -131,6 +131,44 @@ def test_fixed_mu_injection_starts_at_mu0(setup):
     assert np.max(np.abs(mu[0] + 0.6)) <= 2e-5
 
 
+def test_released_arenas_return_to_the_device(setup):
+    """Released requests give their memory back to the device while the server
+    keeps running, as in the serving loop: 60 requests arrive together under a
+    3 GiB budget (several run at once), each finished request's D_mumu product
+    runs on the statistics context in a worker thread beside the next ticks,
+    and the request is released when its product is done.  About 25 GB of
+    arenas pass through the stream-ordered pool; device free memory must stay
+    within the budget plus the server's 8 GiB trim slack of where it started."""
+    from concurrent.futures import ThreadPoolExecutor
+    ctx, stats, fld, F = setup
+    srv = serving.Server(ctx, 3 << 30, 200)
+    srv.add_field(fld)
+    start, _ = ctx.device_memory()
+    low = start
+    rng = np.random.default_rng(8)
+    pending = {}
+    for i in range(60):
+        n = int(rng.integers(40000, 200000))  # cache: 1001 samples x n x 4 B
+        r = req(i, n, 10.0, 1001, serving.OUT_MU, fld.B0, seed=200 + i)
+        r.stride = 1
+        pending[serving.submit_request(srv, r, L_BOX)] = r
+    finishing = {}
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        while pending or finishing:
+            srv.step()
+            for rid in [k for k in pending if srv.poll(k) == serving.DONE]:
+                r = pending.pop(rid)
+                cp, nsave, n, _ = srv.fetch(rid)
+                finishing[rid] = pool.submit(serving.request_product, stats, cp, nsave, n, r)
+            for rid in [k for k, f in finishing.items() if f.done()]:
+                finishing.pop(rid).result()
+                srv.release(rid)
+            low = min(low, ctx.device_memory()[0])
+    assert srv.stats().bytes_reserved == 0
+    assert start - low < (3 << 30) + (8 << 30) + (1 << 30), (start - low) / 2 ** 30
+    srv.close()
+
+
 def test_oversized_request_rejected(setup):
     ctx, stats, fld, F = setup
     srv = serving.Server(ctx, 1 << 20, 50)
