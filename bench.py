@@ -362,11 +362,13 @@ def injection_rates(ctx, F32, L, energy_gev, log_fn, n=4194304):
     return out
 
 
-def pcie_inclusive(F32, L, x, p, energy_gev, log_fn, n=4194304, duration_gp=20.0):
+def pcie_inclusive(F32, L, x, p, energy_gev, log_fn, n=4194304, duration_gp=20.0, output_path=None):
     """The runner path end to end: run_mu_cache_ensemble with the mu cache in
     host memory (double-buffered device chunks, pinned staging, writer
     thread), FP16 field, 200 steps/T_g0, mu every 0.1 T_g0.  Rate = particle
-    pushes / the runner's loop time, downloads included."""
+    pushes / the runner's loop time, downloads included.  With output_path
+    (a .h5 file) the cache goes to an HDF5 file in the reference's layout
+    instead, file writes included; the file is removed afterwards."""
     from particle_transport_amd import runner
     cfg = dict(box_length_pc=10.0, boundary="periodic", precision="Float32", integration_steps_per_gyroperiod=200.0,
                trajectory_duration_gyroperiods=duration_gp, trajectory_save_interval_gyroperiods=0.1, use_cfl=False,
@@ -376,13 +378,25 @@ def pcie_inclusive(F32, L, x, p, energy_gev, log_fn, n=4194304, duration_gp=20.0
         fld = runner.Field(ctx, F32, L, "f16_brick")
         tr = runner.resolve_total_field_time_reference(cfg, F32, energy_GeV=energy_gev, ctx=ctx)
         state = (x[0, :n], x[1, :n], x[2, :n], p[0, :n], p[1, :n], p[2, :n])
-        res = runner.run_mu_cache_ensemble(cfg, F32, None, energy_GeV=energy_gev, time_reference=tr, ctx=ctx,
-                                           field=fld, initial_state=state, chunk_saves=10)
+        res = runner.run_mu_cache_ensemble(cfg, F32, output_path, energy_GeV=energy_gev, time_reference=tr,
+                                           ctx=ctx, field=fld, initial_state=state, chunk_saves=10)
         fld.close()
     pushes = n * (res.trajectory_time.n_integration_steps - 1)
+    if output_path:
+        res.cache.close()
+        os.remove(output_path)
+        log_fn(f"[bench] runner with an HDF5 mu cache: {pushes / res.elapsed:.4g} particle-steps/s "
+               f"({res.cache_gib:.2f} GiB file)")
+        return {"value": pushes / res.elapsed, "unit": "particle-steps/s", "elapsed_s": res.elapsed,
+                "kernel_s": res.kernel_ms / 1e3, "cache_gib": res.cache_gib,
+                "file_GB_per_s": res.cache_gib * 1024 ** 3 / res.elapsed / 1e9,
+                "writer_download_s": res.io_s["download_s"], "writer_file_write_s": res.io_s["write_s"],
+                "path": "run_mu_cache_ensemble: FP16 field, mu cache to an HDF5 file (libpt_h5, reference layout) "
+                        "through double-buffered device chunks of 10 samples, pinned staging and one hyperslab "
+                        "write per chunk on the writer thread"}
     log_fn(f"[bench] runner with host mu cache: {pushes / res.elapsed:.4g} particle-steps/s")
     return {"value": pushes / res.elapsed, "unit": "particle-steps/s", "elapsed_s": res.elapsed,
-            "kernel_s": res.kernel_ms / 1e3, "cache_gib": res.cache_gib,
+            "kernel_s": res.kernel_ms / 1e3, "cache_gib": res.cache_gib, "writer_download_s": res.io_s["download_s"],
             "path": "run_mu_cache_ensemble: FP16 field, mu cache to host memory through double-buffered device "
                     "chunks of 10 samples and pinned staging"}
 
@@ -475,6 +489,7 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-stats", action="store_true", help="skip the D_mumu / D_pp pair-visit rates")
     ap.add_argument("--no-pcie", action="store_true", help="skip the runner path with a host mu cache")
+    ap.add_argument("--no-h5", action="store_true", help="skip the runner path with an HDF5 mu cache file")
     ap.add_argument("--serving-requests", type=int, default=100,
                     help="requests of the serving stream to replay (0 = skip; 2000 = the full stream)")
     ap.add_argument("--modes", action="store_true", help="also run the 96-campaign mode-decomposed batch")
@@ -577,6 +592,11 @@ def main():
     stats = None if args.no_stats else stats_rates(ctx, F32, L, x, p, dt, log, cpu=not args.no_cpu)
     inject = None if args.no_stats else injection_rates(ctx, F32, L, args.energy_gev, log)
     pcie = None if args.no_pcie else pcie_inclusive(F32, L, x, p, args.energy_gev, log)
+    h5_file = None
+    if not args.no_pcie and not args.no_h5:
+        tmp = os.environ.get("TMPDIR", "/tmp")
+        h5_file = pcie_inclusive(F32, L, x, p, args.energy_gev, log,
+                                 output_path=os.path.join(tmp, f"pt_bench_mu_cache_{os.getpid()}.h5"))
 
     cpu = None
     if not args.no_cpu:
@@ -642,6 +662,7 @@ def main():
         "statistics": stats,
         "injection": inject,
         "pcie_inclusive": pcie,
+        "hdf5_cache_inclusive": h5_file,
         "serving": serve,
         "modes_batch": modes,
         "cpu_baseline": cpu,

@@ -9,6 +9,11 @@ Per-file flags:
   pt_stats.hip      -munsafe-fp-atomics (hardware FP64 atomics on coarse-grained
                                         hipMalloc memory)
 All files: packed FP32 instructions disabled (-target-feature -packed-fp32-ops).
+
+libpt_h5.so (csrc/pt_h5.cpp, host C++ over the HDF5 C library) is built beside
+it with g++: HDF5 from $PT_HDF5_DIR (default /opt/conda), found at run time
+through an rpath; libstdc++ is linked statically so that loading it never
+pulls a second C++ runtime into the process.
 """
 from __future__ import annotations
 
@@ -54,6 +59,28 @@ def _compile(src: str, extra: list[str]) -> str:
     return obj
 
 
+H5_LIB = os.path.join(HERE, "libpt_h5.so")
+HDF5_DIR = os.environ.get("PT_HDF5_DIR", "/opt/conda")
+
+
+def build_h5(verbose: bool = False) -> str:
+    inc = os.path.join(HDF5_DIR, "include")
+    lib = os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        raise RuntimeError(f"hdf5.h not found under {inc}: set PT_HDF5_DIR to an HDF5 1.10+ installation")
+    tmp = H5_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", "-Wextra",
+           "-I", os.path.join(os.path.dirname(HERE), "include"), "-I", inc, os.path.join(CSRC, "pt_h5.cpp"),
+           "-o", tmp, "-static-libstdc++", "-static-libgcc", "-L", lib, "-lhdf5", f"-Wl,-rpath,{lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for pt_h5.cpp:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, H5_LIB)
+    if verbose:
+        print("built", H5_LIB)
+    return H5_LIB
+
+
 def build_library(verbose: bool = False) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
@@ -71,4 +98,5 @@ def build_library(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_h5(verbose=True)
     sys.exit(0)

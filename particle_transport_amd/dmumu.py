@@ -16,7 +16,7 @@ from types import SimpleNamespace
 
 import numpy as np
 
-from . import _lib
+from . import _lib, h5
 from ._lib import check, ptr
 from .julia_compat import range_int_length, range_length, round_int, unique_stable
 from .time_units import PipelineError, resolve_lag_grid, t_gyroperiods_from_axes, tOmega0_to_gyroperiods, \
@@ -322,18 +322,52 @@ def _setup_pass(cfg, t_s, t_norm, t_gyroperiods, nsteps, total, key_name):
                            acc=new_accumulators(len(lag_steps), len(centers)))
 
 
+def device_particle_groups(ctx, sources, pidx0, chunk, dtypes, budget_bytes=256 << 20):
+    """h5.iter_particle_groups over host sources (numpy arrays, memory maps
+    or h5.H5Datasets), each group uploaded once into reusable device buffers:
+    yields (device views, chunks).  The next group is read from the host
+    source while the GPU works on the current one."""
+    srcs = [s if hasattr(s, "read_box") else h5.ArrayDataset(np.asarray(s)) for s in sources]
+    bufs = [None] * len(srcs)
+    try:
+        for arrays, chunks in h5.iter_particle_groups(srcs, pidx0, chunk, budget_bytes):
+            views = []
+            for k, (a, dt) in enumerate(zip(arrays, dtypes)):
+                a = np.ascontiguousarray(a, dtype=dt)
+                if bufs[k] is None or bufs[k].nbytes < a.nbytes:
+                    if bufs[k] is not None:
+                        bufs[k].free()
+                    bufs[k] = _lib.DeviceBuffer(ctx, (max(1, a.size),), dt)
+                v = bufs[k].view(a.shape)
+                v.upload(a)
+                views.append(v)
+            yield views, chunks
+    finally:
+        for b in bufs:
+            if b is not None:
+                b.free()
+
+
 def run_combined_from_mu_cache(cfg, ctx, mu_cache, t_s, t_norm, t_gyroperiods=None):
     """run_combined_from_mu_cache (multimode_multienergy_cache_pipeline.jl:1633-1812)
-    over a mu cache shaped (nsave, n) (Julia (n, nsave)), numpy or a
-    DeviceBuffer already on the device.  Returns the "delta_mu2" and "dmumu"
-    product groups of delta_mu2_dmumu_full.h5 as dicts."""
+    over a mu cache shaped (nsave, n) (Julia (n, nsave)): a DeviceBuffer
+    already on the device, or a host source (numpy, a memory map, or the
+    h5.H5Dataset "mu" of a cache file) streamed to the device in groups of
+    chunks, the next group read while the GPU works on the current one.  Returns the "delta_mu2" and
+    "dmumu" product groups of delta_mu2_dmumu_full.h5 as dicts."""
     nsteps, total = mu_cache.shape
     u = _setup_pass(cfg, t_s, t_norm, t_gyroperiods, nsteps, total, "Mu-cache time axes")
     start_mode, bin_abs = dmumu_start_mode(cfg), mu_bin_abs(cfg)
     # Chunks are merged with Chan's formula in order, as the reference merges them.
-    for c0 in range(0, len(u.pidx), u.chunk):
-        process_mu_chunk_dmumu(ctx, mu_cache, u.lag_steps, u.edges, u.acc, start_mode, bin_abs,
-                               layout="particle_fastest", particle_index=u.pidx[c0:c0 + u.chunk] - 1)
+    if isinstance(mu_cache, _lib.DeviceBuffer):
+        for c0 in range(0, len(u.pidx), u.chunk):
+            process_mu_chunk_dmumu(ctx, mu_cache, u.lag_steps, u.edges, u.acc, start_mode, bin_abs,
+                                   layout="particle_fastest", particle_index=u.pidx[c0:c0 + u.chunk] - 1)
+    else:
+        for (block,), chunks in device_particle_groups(ctx, [mu_cache], u.pidx - 1, u.chunk, [np.float32]):
+            for _, local in chunks:
+                process_mu_chunk_dmumu(ctx, block, u.lag_steps, u.edges, u.acc, start_mode, bin_abs,
+                                       layout="particle_fastest", particle_index=local)
     return _finish_products(cfg, u.acc, u.lag_steps, u.grid, u.tau_s, u.tau_norm, u.t_s, u.t_norm, u.edges,
                             u.centers, u.visits, u.pidx)
 
@@ -382,7 +416,8 @@ def run_combined_full(cfg, ctx, field, positions, momenta, t_s, t_norm, t_gyrope
     directly, so mu never travels to the host.  positions/momenta: numpy
     (nsave, 3, n) as the phase-space cache stores them (Julia (n, 3, nsave)),
     e.g. memory maps; each chunk's columns are gathered on the host like
-    read_particle_batch (:411-437).  field: a planar Field with B (the
+    read_particle_batch (:411-437), or h5.H5Datasets of a phase-space cache
+    file, streamed in groups of chunks.  field: a planar Field with B (the
     reference loads B in compute_precision, Float32 by default).  Returns the
     product groups like run_combined_from_mu_cache."""
     nsteps, three, total = tuple(positions.shape)
@@ -394,14 +429,21 @@ def run_combined_full(cfg, ctx, field, positions, momenta, t_s, t_norm, t_gyrope
     start_mode, bin_abs = dmumu_start_mode(cfg), mu_bin_abs(cfg)
     T = np.float64 if field.storage == "f64_planar" else np.float32
     buf = _lib.DeviceBuffer(ctx, (u.chunk, nsteps), np.float32)
+
+    def one_chunk(pos, mom):
+        mu = reconstruct_mu_chunk(ctx, field, np.ascontiguousarray(pos, dtype=T), np.ascontiguousarray(mom, dtype=T),
+                                  out=buf.view((pos.shape[-1], nsteps)))
+        process_mu_chunk_dmumu(ctx, mu, u.lag_steps, u.edges, u.acc, start_mode, bin_abs, layout="time_fastest")
+
     try:
-        for c0 in range(0, len(u.pidx), u.chunk):
-            cols = u.pidx[c0:c0 + u.chunk] - 1
-            pos = np.ascontiguousarray(np.asarray(positions)[:, :, cols], dtype=T)
-            mom = np.ascontiguousarray(np.asarray(momenta)[:, :, cols], dtype=T)
-            mu = reconstruct_mu_chunk(ctx, field, pos, mom, out=buf.view((len(cols), nsteps)))
-            process_mu_chunk_dmumu(ctx, mu, u.lag_steps, u.edges, u.acc, start_mode, bin_abs,
-                                   layout="time_fastest")
+        if isinstance(positions, h5.H5Dataset):
+            for (P, M), chunks in h5.iter_particle_groups([positions, momenta], u.pidx - 1, u.chunk):
+                for _, local in chunks:
+                    one_chunk(P[:, :, local], M[:, :, local])
+        else:
+            for c0 in range(0, len(u.pidx), u.chunk):
+                cols = u.pidx[c0:c0 + u.chunk] - 1
+                one_chunk(np.asarray(positions)[:, :, cols], np.asarray(momenta)[:, :, cols])
     finally:
         buf.free()
     return _finish_products(cfg, u.acc, u.lag_steps, u.grid, u.tau_s, u.tau_norm, u.t_s, u.t_norm, u.edges,

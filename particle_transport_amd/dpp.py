@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, ptr
-from .dmumu import build_particle_indices
+from .dmumu import build_particle_indices, device_particle_groups
 from .injection import C_LIGHT, GEV_TO_J, M_P
 from .julia_compat import range_int_length, round_int, unique_stable
 from .time_units import PipelineError, resolve_lag_grid, t_gyroperiods_from_axes, validate_time_axes
@@ -121,7 +121,8 @@ def compute_dpp_arrays(acc, tau_s, tau_norm):
 
 def run_dpp_full(cfg, ctx, momenta, t_s, t_norm, t_gyroperiods=None):
     """run_dpp_full (compute_dpp_full.jl:456-541) over a momenta cache shaped
-    (nsave, 3, n).  Returns the "dpp" and "energy_snapshots" groups as dicts."""
+    (nsave, 3, n): a DeviceBuffer, numpy / a memory map, or the h5.H5Dataset
+    "momenta" of a phase-space cache file.  Returns the "dpp" and "energy_snapshots" groups as dicts."""
     t_s = np.asarray(t_s, dtype=np.float64)
     t_norm = np.asarray(t_norm, dtype=np.float64)
     t_gp = t_gyroperiods_from_axes(t_s, t_norm) if t_gyroperiods is None else np.asarray(t_gyroperiods)
@@ -134,10 +135,13 @@ def run_dpp_full(cfg, ctx, momenta, t_s, t_norm, t_gyroperiods=None):
     lag_steps = grid.lag_steps
     tau_s = np.array([t_s[l] - t_s[0] for l in lag_steps])
     tau_norm = grid.tau_norm
+    i0 = int(pidx[0]) - 1
     if isinstance(momenta, _lib.DeviceBuffer):
-        m0 = momenta.to_host()[0, :, int(pidx[0]) - 1]
+        m0 = momenta.to_host()[0, :, i0]
+    elif hasattr(momenta, "read_box"):
+        m0 = momenta.read_box((0, 0, i0), (1, 3, 1)).reshape(3)
     else:
-        m0 = np.asarray(momenta[0, :, int(pidx[0]) - 1])
+        m0 = np.asarray(momenta[0, :, i0])
     p0 = momentum_magnitude(*m0)
     if not (math.isfinite(p0) and p0 > 0.0):
         raise PipelineError("Cannot compute D_pp normalization: first selected particle has invalid initial momentum.")
@@ -147,10 +151,16 @@ def run_dpp_full(cfg, ctx, momenta, t_s, t_norm, t_gyroperiods=None):
     chunk = min(int(cfg.get("particle_chunk_size", len(pidx))), len(pidx))
     if chunk <= 0:
         raise PipelineError("particle_chunk_size must be positive.")
-    for c0 in range(0, len(pidx), chunk):
-        cols = pidx[c0:c0 + chunk] - 1
-        process_momenta_chunk_dpp(ctx, momenta, lag_steps, p0, acc, particle_index=cols)
-        fill_energy_snapshots(ctx, energy, momenta, snaps, c0 + 1, particle_index=cols)
+    if isinstance(momenta, _lib.DeviceBuffer):
+        for c0 in range(0, len(pidx), chunk):
+            cols = pidx[c0:c0 + chunk] - 1
+            process_momenta_chunk_dpp(ctx, momenta, lag_steps, p0, acc, particle_index=cols)
+            fill_energy_snapshots(ctx, energy, momenta, snaps, c0 + 1, particle_index=cols)
+    else:  # host source (numpy, memory map, h5.H5Dataset): streamed to the device in groups of chunks
+        for (block,), chunks in device_particle_groups(ctx, [momenta], pidx - 1, chunk, [np.float32]):
+            for c0, local in chunks:
+                process_momenta_chunk_dpp(ctx, block, lag_steps, p0, acc, particle_index=local)
+                fill_energy_snapshots(ctx, energy, block, snaps, c0 + 1, particle_index=local)
     arrays = compute_dpp_arrays(acc, tau_s, tau_norm)
     dpp = dict(p0_kg_m_per_s=p0, counts=acc.counts, sum_delta_p=acc.sum_delta_p, sum_delta_p2=acc.sum_delta_p2,
                sum_delta_p_norm=acc.sum_delta_p_norm, sum_delta_p_norm2=acc.sum_delta_p_norm2, lag_steps=lag_steps,

@@ -22,7 +22,7 @@ from types import SimpleNamespace
 
 import numpy as np
 
-from . import _lib
+from . import _lib, cache_h5, h5
 from ._lib import PT_F32, PT_F64, PUSHER, STORAGE, check, ptr
 from .fields import build_uniform_coords, estimate_min_dx
 from .injection import (M_P, Q_E, energy_to_gamma, energy_to_speed, init_particles, inject_params,
@@ -248,16 +248,28 @@ def _saves_in_range(first, count, stride):
 class CacheWriter:
     """Cadence cache in the reference's dataset layout (HDF5.jl writes a Julia
     (n,3,nsave) array, seen from C as (nsave,3,n); numpy uses the C view).
-    Without an output directory the datasets stay in memory, page-locked so
-    that device chunks are copied straight into them; with one they are .npy
-    memory maps next to a metadata.json (h5py is not part of this build)."""
+
+    output path ending in ".h5": an HDF5 file in the reference's cache layout
+      (create_phase_space_writer / create_mu_cache_writer,
+      phase_space_gpu_runner.jl:406-446, multimode_multienergy_cache_pipeline.jl:
+      1312-1347), written through libpt_h5.so, one hyperslab per drained chunk;
+    any other path: .npy memory maps next to a metadata.json;
+    no path: in memory, page-locked so that device chunks are copied straight
+      into them."""
 
     def __init__(self, path, datasets: dict, dtype=np.float32):
         self.path = path
         self.dtype = dtype
         self.data = {}
         self.meta = {}
+        self.file = None
         self.pinned = not path
+        if path and str(path).endswith(".h5"):
+            os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+            self.file = h5.H5File(path, "w")
+            for name, shape in datasets.items():
+                self.data[name] = self.file.create_dataset(name, shape, dtype)
+            return
         if path:
             os.makedirs(path, exist_ok=True)
         for name, shape in datasets.items():
@@ -271,17 +283,48 @@ class CacheWriter:
                     self.pinned = False
                     self.data[name] = np.empty(shape, dtype=dtype)
 
+    @property
+    def is_h5(self):
+        return self.file is not None or (bool(self.path) and str(self.path).endswith(".h5"))
+
     def __getitem__(self, k):
         return self.data[k]
 
-    def finalize(self, meta: dict):
+    def write(self, name, offset, block):
+        """Samples offset .. offset + len(block) - 1 of dataset `name`."""
+        if self.file is not None:
+            self.file.write_slab(name, offset, block)
+        else:
+            self.data[name][offset:offset + len(block)] = block
+
+    def finalize(self, meta: dict, h5_meta=None):
+        """Record the metadata; an HDF5 cache gets it written by h5_meta(file)
+        in the reference's forms, is closed, and is reopened read-only so the
+        datasets stay readable through this writer."""
         self.meta.update(meta)
-        if self.path:
+        if self.file is not None:
+            if h5_meta is not None:
+                h5_meta(self.file)
+            self.file.close()
+            self.file = None
+            reader = h5.H5File(self.path, "r")
+            self.data = {k: reader.dataset(k) for k in self.data}
+            self._reader = reader
+        elif self.path:
             for a in self.data.values():
                 a.flush()
             with open(os.path.join(self.path, "metadata.json"), "w") as f:
                 json.dump({k: (v.tolist() if isinstance(v, np.ndarray) else v) for k, v in self.meta.items()}, f,
                           default=str)
+
+    def close(self):
+        if self.file is not None:
+            self.file.close()
+            self.file = None
+        r = getattr(self, "_reader", None)
+        if r is not None:
+            r.close()
+            self._reader = None
 
 
 def _precision_dtype(cfg):
@@ -397,17 +440,24 @@ def _run_loop(cfg, fields, setup, out_kind, writer, ctx, field, initial_state, c
     busy = [None, None]
     kernel_ms = 0.0
 
+    io = {"download_s": 0.0, "write_s": 0.0}
+
     def drain(buf, offset, count):
         for k, name in enumerate(names):
             nbytes = count * int(np.prod(shape[1:])) * np.dtype(out_T).itemsize
+            t0 = time.perf_counter()
             if pinned is None:
                 dst = writer[name][offset:offset + count]
                 check(_lib.load().pt_memcpy(copy_ctx.h, ctypes.c_void_p(dst.ctypes.data),
                                             ctypes.c_void_p(dev[buf][k].ptr), nbytes, 1))
+                io["download_s"] += time.perf_counter() - t0
             else:
                 check(_lib.load().pt_memcpy(copy_ctx.h, ctypes.c_void_p(pinned[buf][k].ptr),
                                             ctypes.c_void_p(dev[buf][k].ptr), nbytes, 1))
-                writer[name][offset:offset + count] = pinned[buf][k].array[:count]
+                t1 = time.perf_counter()
+                writer.write(name, offset, pinned[buf][k].array[:count])
+                io["download_s"] += t1 - t0
+                io["write_s"] += time.perf_counter() - t1
 
     try:
         t0 = time.perf_counter()
@@ -451,7 +501,7 @@ def _run_loop(cfg, fields, setup, out_kind, writer, ctx, field, initial_state, c
             field.close()
         if own_ctx:
             ctx.close()
-    return alive_fraction, elapsed, kernel_ms
+    return alive_fraction, elapsed, kernel_ms, io
 
 
 def _common_meta(cfg, s, energy_GeV, alive_fraction):
@@ -486,16 +536,21 @@ def run_gpu_ensemble(cfg, fields, *, energy_GeV, time_reference, ctx=None, field
     n = int(cfg["n_particles"])
     writer = CacheWriter(output_path, {"positions": (len(s.save_idx), 3, n), "momenta": (len(s.save_idx), 3, n)},
                          out_T)
-    alive, elapsed, kms = _run_loop(cfg, fields, s, "phase", writer, ctx, field, initial_state, chunk_saves)
+    try:
+        alive, elapsed, kms, io = _run_loop(cfg, fields, s, "phase", writer, ctx, field, initial_state,
+                                            chunk_saves)
+    except BaseException:
+        writer.close()
+        raise
     meta = _common_meta(cfg, s, energy_GeV, alive)
     meta.update(trajectory_output_precision="Float64" if out_T == np.float64 else "Float32", position_unit="m",
                 momentum_unit="kg*m/s")
-    writer.finalize(meta)
+    writer.finalize(meta, lambda f: cache_h5.finalize_cache(f, cfg, s, energy_GeV, alive, "phase", out_T))
     return SimpleNamespace(
         t_norm=s.t_norm.astype(np.float64), t_s=s.t_s.astype(np.float64), t_gyroperiods=s.t_gp.astype(np.float64),
         alive_fraction=alive, nsave=len(s.save_idx), phase_space_path=output_path,
         phase_space_gib=6 * n * len(s.save_idx) * np.dtype(out_T).itemsize / 1024.0 ** 3, elapsed=elapsed,
-        kernel_ms=kms, Omega0=float(s.Omega0), dt=float(s.dt), trajectory_time=s.tt, save_time=s.st,
+        kernel_ms=kms, io_s=io, Omega0=float(s.Omega0), dt=float(s.dt), trajectory_time=s.tt, save_time=s.st,
         B0=float(s.time_reference.B0_reference_T), time_reference=s.time_reference, cache=writer)
 
 
@@ -514,13 +569,18 @@ def run_mu_cache_ensemble(cfg, fields, output_path=None, *, energy_GeV, time_ref
     out_T = np.float64 if str(cfg.get("mu_cache_output_precision", "Float32")) == "Float64" else np.float32
     n = int(cfg["n_particles"])
     writer = CacheWriter(output_path, {"mu": (len(s.save_idx), n)}, out_T)
-    alive, elapsed, kms = _run_loop(cfg, fields, s, "mu", writer, ctx, field, initial_state, chunk_saves)
+    try:
+        alive, elapsed, kms, io = _run_loop(cfg, fields, s, "mu", writer, ctx, field, initial_state,
+                                            chunk_saves)
+    except BaseException:
+        writer.close()
+        raise
     meta = _common_meta(cfg, s, energy_GeV, alive)
     meta.update(cache_mode="mu", cache_output_precision="Float64" if out_T == np.float64 else "Float32",
                 mu_unit="dimensionless")
-    writer.finalize(meta)
+    writer.finalize(meta, lambda f: cache_h5.finalize_cache(f, cfg, s, energy_GeV, alive, "mu", out_T))
     return SimpleNamespace(cache_path=output_path, cache_gib=n * len(s.save_idx) * np.dtype(out_T).itemsize / 1024.0 ** 3,
-                           elapsed=elapsed, kernel_ms=kms, Omega0=float(s.Omega0), dt=float(s.dt),
+                           elapsed=elapsed, kernel_ms=kms, io_s=io, Omega0=float(s.Omega0), dt=float(s.dt),
                            trajectory_time=s.tt, save_time=s.st, B0=float(s.time_reference.B0_reference_T),
                            time_reference=s.time_reference, cache=writer, alive_fraction=alive,
                            t_s=s.t_s, t_norm=s.t_norm, t_gyroperiods=s.t_gp)
