@@ -322,6 +322,20 @@ def _setup_pass(cfg, t_s, t_norm, t_gyroperiods, nsteps, total, key_name):
                            acc=new_accumulators(len(lag_steps), len(centers)))
 
 
+def gpu_batch_particles(cfg, chunk, bytes_per_particle, budget_bytes=256 << 20):
+    """Particles per GPU pass: whole analysis chunks (particle_chunk_size),
+    up to cfg["gpu_batch_particles"] (default 65536) and at most budget_bytes
+    of input.  The reference's chunk of 128 particles would leave most of the
+    GPU idle (a 128-particle D_mumu call fills 32 workgroups); batches of
+    chunks merge with Chan's formula in order, as the reference merges chunks
+    and threads, so only the rounding of the Welford merge tree differs."""
+    want = int(cfg.get("gpu_batch_particles", 65536))
+    if want <= 0:
+        raise PipelineError("gpu_batch_particles must be positive.")
+    cap = max(chunk, int(budget_bytes) // max(1, int(bytes_per_particle)))
+    return max(chunk, (min(max(want, chunk), cap) // chunk) * chunk)
+
+
 def device_particle_groups(ctx, sources, pidx0, chunk, dtypes, budget_bytes=256 << 20):
     """h5.iter_particle_groups over host sources (numpy arrays, memory maps
     or h5.H5Datasets), each group uploaded once into reusable device buffers:
@@ -353,19 +367,22 @@ def run_combined_from_mu_cache(cfg, ctx, mu_cache, t_s, t_norm, t_gyroperiods=No
     over a mu cache shaped (nsave, n) (Julia (n, nsave)): a DeviceBuffer
     already on the device, or a host source (numpy, a memory map, or the
     h5.H5Dataset "mu" of a cache file) streamed to the device in groups of
-    chunks, the next group read while the GPU works on the current one.  Returns the "delta_mu2" and
-    "dmumu" product groups of delta_mu2_dmumu_full.h5 as dicts."""
+    chunks, the next group read while the GPU works on the current one.
+    Each GPU pass covers a batch of whole chunks (gpu_batch_particles).
+    Returns the "delta_mu2" and "dmumu" product groups of
+    delta_mu2_dmumu_full.h5 as dicts."""
     nsteps, total = mu_cache.shape
     u = _setup_pass(cfg, t_s, t_norm, t_gyroperiods, nsteps, total, "Mu-cache time axes")
     start_mode, bin_abs = dmumu_start_mode(cfg), mu_bin_abs(cfg)
-    # Chunks are merged with Chan's formula in order, as the reference merges them.
+    # Batches of whole chunks, merged with Chan's formula in order.
+    batch = gpu_batch_particles(cfg, u.chunk, nsteps * 4)
     if isinstance(mu_cache, _lib.DeviceBuffer):
-        for c0 in range(0, len(u.pidx), u.chunk):
+        for b0 in range(0, len(u.pidx), batch):
             process_mu_chunk_dmumu(ctx, mu_cache, u.lag_steps, u.edges, u.acc, start_mode, bin_abs,
-                                   layout="particle_fastest", particle_index=u.pidx[c0:c0 + u.chunk] - 1)
+                                   layout="particle_fastest", particle_index=u.pidx[b0:b0 + batch] - 1)
     else:
-        for (block,), chunks in device_particle_groups(ctx, [mu_cache], u.pidx - 1, u.chunk, [np.float32]):
-            for _, local in chunks:
+        for (block,), batches in device_particle_groups(ctx, [mu_cache], u.pidx - 1, batch, [np.float32]):
+            for _, local in batches:
                 process_mu_chunk_dmumu(ctx, block, u.lag_steps, u.edges, u.acc, start_mode, bin_abs,
                                        layout="particle_fastest", particle_index=local)
     return _finish_products(cfg, u.acc, u.lag_steps, u.grid, u.tau_s, u.tau_norm, u.t_s, u.t_norm, u.edges,
@@ -411,7 +428,8 @@ def reconstruct_mu_chunk(ctx, field, positions, momenta, out=None):
 
 def run_combined_full(cfg, ctx, field, positions, momenta, t_s, t_norm, t_gyroperiods=None):
     """run_combined_full (compute_delta_mu2_dmumu_full.jl:1012-1259) over a
-    phase-space cache: per particle chunk, mu is reconstructed on the device
+    phase-space cache: per batch of particle chunks (gpu_batch_particles), mu
+    is reconstructed on the device
     (reconstruct_mu_chunk) into a device buffer that the pair pass reads
     directly, so mu never travels to the host.  positions/momenta: numpy
     (nsave, 3, n) as the phase-space cache stores them (Julia (n, 3, nsave)),
@@ -428,22 +446,19 @@ def run_combined_full(cfg, ctx, field, positions, momenta, t_s, t_norm, t_gyrope
     u = _setup_pass(cfg, t_s, t_norm, t_gyroperiods, nsteps, total, "Trajectory HDF5 time axes")
     start_mode, bin_abs = dmumu_start_mode(cfg), mu_bin_abs(cfg)
     T = np.float64 if field.storage == "f64_planar" else np.float32
-    buf = _lib.DeviceBuffer(ctx, (u.chunk, nsteps), np.float32)
+    batch = gpu_batch_particles(cfg, u.chunk, nsteps * 6 * np.dtype(T).itemsize)
+    buf = _lib.DeviceBuffer(ctx, (min(batch, len(u.pidx)), nsteps), np.float32)
 
-    def one_chunk(pos, mom):
+    def one_batch(pos, mom):
         mu = reconstruct_mu_chunk(ctx, field, np.ascontiguousarray(pos, dtype=T), np.ascontiguousarray(mom, dtype=T),
                                   out=buf.view((pos.shape[-1], nsteps)))
         process_mu_chunk_dmumu(ctx, mu, u.lag_steps, u.edges, u.acc, start_mode, bin_abs, layout="time_fastest")
 
+    srcs = [a if hasattr(a, "read_box") else h5.ArrayDataset(np.asarray(a)) for a in (positions, momenta)]
     try:
-        if isinstance(positions, h5.H5Dataset):
-            for (P, M), chunks in h5.iter_particle_groups([positions, momenta], u.pidx - 1, u.chunk):
-                for _, local in chunks:
-                    one_chunk(P[:, :, local], M[:, :, local])
-        else:
-            for c0 in range(0, len(u.pidx), u.chunk):
-                cols = u.pidx[c0:c0 + u.chunk] - 1
-                one_chunk(np.asarray(positions)[:, :, cols], np.asarray(momenta)[:, :, cols])
+        for (P, M), batches in h5.iter_particle_groups(srcs, u.pidx - 1, batch):
+            for _, local in batches:
+                one_batch(P[:, :, local], M[:, :, local])
     finally:
         buf.free()
     return _finish_products(cfg, u.acc, u.lag_steps, u.grid, u.tau_s, u.tau_norm, u.t_s, u.t_norm, u.edges,

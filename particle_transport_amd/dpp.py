@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, ptr
-from .dmumu import build_particle_indices, device_particle_groups
+from .dmumu import build_particle_indices, device_particle_groups, gpu_batch_particles
 from .injection import C_LIGHT, GEV_TO_J, M_P
 from .julia_compat import range_int_length, round_int, unique_stable
 from .time_units import PipelineError, resolve_lag_grid, t_gyroperiods_from_axes, validate_time_axes
@@ -151,16 +151,17 @@ def run_dpp_full(cfg, ctx, momenta, t_s, t_norm, t_gyroperiods=None):
     chunk = min(int(cfg.get("particle_chunk_size", len(pidx))), len(pidx))
     if chunk <= 0:
         raise PipelineError("particle_chunk_size must be positive.")
+    batch = gpu_batch_particles(cfg, chunk, nsteps * 12)  # GPU passes over whole chunks
     if isinstance(momenta, _lib.DeviceBuffer):
-        for c0 in range(0, len(pidx), chunk):
-            cols = pidx[c0:c0 + chunk] - 1
+        for b0 in range(0, len(pidx), batch):
+            cols = pidx[b0:b0 + batch] - 1
             process_momenta_chunk_dpp(ctx, momenta, lag_steps, p0, acc, particle_index=cols)
-            fill_energy_snapshots(ctx, energy, momenta, snaps, c0 + 1, particle_index=cols)
-    else:  # host source (numpy, memory map, h5.H5Dataset): streamed to the device in groups of chunks
-        for (block,), chunks in device_particle_groups(ctx, [momenta], pidx - 1, chunk, [np.float32]):
-            for c0, local in chunks:
+            fill_energy_snapshots(ctx, energy, momenta, snaps, b0 + 1, particle_index=cols)
+    else:  # host source (numpy, memory map, h5.H5Dataset): streamed to the device in groups of batches
+        for (block,), batches in device_particle_groups(ctx, [momenta], pidx - 1, batch, [np.float32]):
+            for b0, local in batches:
                 process_momenta_chunk_dpp(ctx, block, lag_steps, p0, acc, particle_index=local)
-                fill_energy_snapshots(ctx, energy, block, snaps, c0 + 1, particle_index=local)
+                fill_energy_snapshots(ctx, energy, block, snaps, b0 + 1, particle_index=local)
     arrays = compute_dpp_arrays(acc, tau_s, tau_norm)
     dpp = dict(p0_kg_m_per_s=p0, counts=acc.counts, sum_delta_p=acc.sum_delta_p, sum_delta_p2=acc.sum_delta_p2,
                sum_delta_p_norm=acc.sum_delta_p_norm, sum_delta_p_norm2=acc.sum_delta_p_norm2, lag_steps=lag_steps,

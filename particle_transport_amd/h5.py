@@ -145,20 +145,24 @@ class H5Dataset:
         Julia first axis) for every other index, one hyperslab per run of
         consecutive indices, as read_particle_batch does
         (compute_delta_mu2_dmumu_full.jl:411-437)."""
-        cols = np.asarray(cols, dtype=np.int64)
-        if cols.size == 0:
-            raise H5Error("Cannot read an empty particle batch.")
-        if cols.min() < 0 or cols.max() >= self.shape[-1]:
-            raise H5Error("particle index out of range")
-        dt = np.dtype(dtype or self.dtype)
-        out = np.empty(tuple(self.shape[:-1]) + (len(cols),), dtype=dt)
-        breaks = np.flatnonzero(np.diff(cols) != 1) + 1
-        starts = np.concatenate([[0], breaks])
-        stops = np.concatenate([breaks, [len(cols)]])
-        for s, e in zip(starts, stops):
-            lead = (0,) * (self.ndim - 1)
-            out[..., s:e] = self.read_box(lead + (int(cols[s]),), tuple(self.shape[:-1]) + (int(e - s),), dt)
-        return out
+        return _read_columns(self, cols, dtype)
+
+
+def _read_columns(ds, cols, dtype=None):
+    cols = np.asarray(cols, dtype=np.int64)
+    if cols.size == 0:
+        raise H5Error("Cannot read an empty particle batch.")
+    if cols.min() < 0 or cols.max() >= ds.shape[-1]:
+        raise H5Error("particle index out of range")
+    dt = np.dtype(dtype or ds.dtype)
+    out = np.empty(tuple(ds.shape[:-1]) + (len(cols),), dtype=dt)
+    breaks = np.flatnonzero(np.diff(cols) != 1) + 1
+    starts = np.concatenate([[0], breaks])
+    stops = np.concatenate([breaks, [len(cols)]])
+    lead = (0,) * (ds.ndim - 1)
+    for s, e in zip(starts, stops):
+        out[..., s:e] = ds.read_box(lead + (int(cols[s]),), tuple(ds.shape[:-1]) + (int(e - s),), dt)
+    return out
 
 
 class ArrayDataset:
@@ -176,7 +180,7 @@ class ArrayDataset:
         return np.ascontiguousarray(self.a[sl], dtype=dtype or self.dtype)
 
     def read_columns(self, cols, dtype=None):
-        return np.ascontiguousarray(self.a[..., np.asarray(cols, dtype=np.int64)], dtype=dtype or self.dtype)
+        return _read_columns(self, cols, dtype)
 
 
 class H5File:
