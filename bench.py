@@ -370,7 +370,10 @@ def pcie_inclusive(F32, L, x, p, energy_gev, log_fn, n=4194304, duration_gp=20.0
     (a .h5 file) the cache goes to an HDF5 file in the reference's layout
     instead, file writes included; the file is removed afterwards."""
     from particle_transport_amd import runner
-    cfg = dict(box_length_pc=10.0, boundary="periodic", precision="Float32", integration_steps_per_gyroperiod=200.0,
+    # FP64 time axes (a Float32 run's axes fail the reference's 1e-10 uniformity check, so no product could be
+    # computed from its cache); the production pusher and the FP16 field are chosen explicitly
+    cfg = dict(box_length_pc=10.0, boundary="periodic", precision="Float64", gpu_pusher="fast",
+               field_storage="f16_brick", integration_steps_per_gyroperiod=200.0,
                trajectory_duration_gyroperiods=duration_gp, trajectory_save_interval_gyroperiods=0.1, use_cfl=False,
                cfl=0.2, eta=0.1, tOmega0_max=1.0, trajectory_time_stride=1, seed=1, n_particles=n,
                trajectory_mode="total", mu_cache_output_precision="Float32", progress_every=0, steps_per_launch=200)
@@ -381,9 +384,29 @@ def pcie_inclusive(F32, L, x, p, energy_gev, log_fn, n=4194304, duration_gp=20.0
         res = runner.run_mu_cache_ensemble(cfg, F32, output_path, energy_GeV=energy_gev, time_reference=tr,
                                            ctx=ctx, field=fld, initial_state=state, chunk_saves=10)
         fld.close()
+        product = None
+        if output_path:
+            # the D_mumu product read back from the file: 64 log-spaced lags, 20 |mu| bins, all particles
+            from particle_transport_amd import cache_h5, dmumu
+            res.cache.close()
+            c = cache_h5.open_cache(output_path)
+            pcfg = dict(mu_min=0.0, mu_max=1.0, n_mu_bins=20, mu_bin_abs=True, dmumu_start_mode="sliding",
+                        first_particle=1, n_particles_to_use=None, particle_selection="range", particle_chunk_size=128,
+                        lag_mode="log_samples", n_lag_samples=64, lag_step_stride=1, min_lag_steps=1,
+                        max_lag_steps=None, min_count_per_cell=20, allow_huge=True)
+            t0 = time.perf_counter()
+            prod = dmumu.run_combined_from_mu_cache(pcfg, ctx, c.mu, c.t_s, c.t_norm)
+            el = time.perf_counter() - t0
+            c.file.close()
+            v = int(prod["estimated_pair_visits"])
+            product = {"seconds": el, "pair_visits": v, "pair_visits_per_s": v / el,
+                       "file_GB_per_s": res.cache_gib * 1024 ** 3 / el / 1e9,
+                       "what": "dmumu.run_combined_from_mu_cache over the file: all particles, "
+                               f"{len(prod['dmumu']['lag_steps'])} log-spaced lags, 20 |mu| bins, sliding starts, "
+                               "chunk 128, file reads included"}
+            log_fn(f"[bench] D_mumu product from the HDF5 cache: {el:.3f} s, {v / el:.4g} pair visits/s")
     pushes = n * (res.trajectory_time.n_integration_steps - 1)
     if output_path:
-        res.cache.close()
         os.remove(output_path)
         log_fn(f"[bench] runner with an HDF5 mu cache: {pushes / res.elapsed:.4g} particle-steps/s "
                f"({res.cache_gib:.2f} GiB file)")
@@ -391,6 +414,7 @@ def pcie_inclusive(F32, L, x, p, energy_gev, log_fn, n=4194304, duration_gp=20.0
                 "kernel_s": res.kernel_ms / 1e3, "cache_gib": res.cache_gib,
                 "file_GB_per_s": res.cache_gib * 1024 ** 3 / res.elapsed / 1e9,
                 "writer_download_s": res.io_s["download_s"], "writer_file_write_s": res.io_s["write_s"],
+                "dmumu_product_from_file": product,
                 "path": "run_mu_cache_ensemble: FP16 field, mu cache to an HDF5 file (libpt_h5, reference layout) "
                         "through double-buffered device chunks of 10 samples, pinned staging and one hyperslab "
                         "write per chunk on the writer thread"}
