@@ -475,3 +475,79 @@ def test_morton_sort_leaves_results_bit_identical(ctx, periodic):
     assert np.array_equal(ra.out_a[~np.isnan(ra.out_a)], rb.out_a[~np.isnan(rb.out_a)])
     assert np.array_equal(ra.alive_counts, rb.alive_counts)
     assert np.array_equal(aa, ab) and np.array_equal(xa[:, aa], xb[:, ab]) and np.array_equal(pa, pb)
+
+
+def _dmumu_gpu(ctx, mu, lags, edges, injection, bin_abs):
+    acc = dmumu.new_accumulators(len(lags), len(edges) - 1)
+    dmumu.process_mu_chunk_dmumu(ctx, mu, lags, edges, acc, "injection" if injection else "sliding", bin_abs)
+    return acc
+
+
+def _check_dmumu(acc, oacc):
+    assert np.array_equal(acc.dmumu_counts, oacc["dmumu_counts"])
+    assert np.array_equal(acc.pair_counts, oacc["pair_counts"])
+    assert np.array_equal(acc.particle_counts, oacc["particle_counts"])
+    for k in ("dmumu_sum_delta", "dmumu_sum_delta2", "pair_sum_squares", "particle_means", "particle_m2s"):
+        a, b = getattr(acc, k), oacc[k]
+        scale = np.max(np.abs(b)) if b.size and np.max(np.abs(b)) > 0 else 1.0
+        assert np.allclose(a, b, rtol=1e-11, atol=1e-12 * scale), k
+
+
+@pytest.mark.parametrize("case", ["one_particle_two_samples", "max_bins", "one_bin", "all_nan", "lags_past_end"])
+def test_dmumu_edge_shapes(ctx, case):
+    """Shapes at the edges the reference's loops allow: a single particle
+    with two samples, 256 bins (the kernel's maximum) and one bin, particles
+    whose series are all NaN, and lags at or past the series end (skipped)."""
+    if case == "one_particle_two_samples":
+        mu, lags, edges = np.array([[0.3, 0.7]], np.float32), [1, 2, 5], np.linspace(0.0, 1.0, 21)
+    elif case == "max_bins":
+        mu, lags, edges = _mu_series(300, 200, 3), [1, 7, 50, 199], np.linspace(0.0, 1.0, 257)
+    elif case == "one_bin":
+        mu, lags, edges = _mu_series(300, 200, 4), [1, 7, 50, 199], np.array([0.0, 1.0])
+    elif case == "all_nan":
+        mu, lags, edges = np.full((70, 50), np.nan, np.float32), [1, 2, 3], np.linspace(0.0, 1.0, 11)
+    else:
+        mu, lags, edges = _mu_series(130, 40, 5), [39, 40, 41, 1000], np.linspace(0.0, 1.0, 11)
+    for injection in (False, True):
+        for bin_abs in (True, False):
+            e = edges if bin_abs else np.linspace(-1.0, 1.0, len(edges))
+            acc = _dmumu_gpu(ctx, mu, lags, e, injection, bin_abs)
+            oacc = oracle.new_dmumu_accumulators(len(lags), len(e) - 1)
+            oracle.dmumu(mu, lags, e, injection, bin_abs, oacc)
+            _check_dmumu(acc, oacc)
+    if case == "all_nan":
+        assert acc.dmumu_counts.sum() == 0 and acc.particle_counts.sum() == 0
+
+
+def test_dmumu_offsets_beyond_32_bits(ctx):
+    """A time-fastest cache of 2,097,664 particles x 512 samples (4.3 GB):
+    particles selected near its end sit more than 2^32 bytes from its start,
+    so the kernel takes its 64-bit addressing walk.  Only the selected
+    particles' series are written; the oracle sees the same series."""
+    import ctypes
+
+    from particle_transport_amd import _lib
+    from particle_transport_amd._lib import DeviceBuffer
+    total, nsave = (1 << 21) + 512, 512
+    assert (total - 1) * nsave * 4 > (1 << 32)
+    rng = np.random.default_rng(8)
+    sel = np.sort(np.concatenate([rng.choice(1000, 40, replace=False),
+                                  total - 1 - rng.choice(2000, 60, replace=False)])).astype(np.int64)
+    series = _mu_series(len(sel), nsave, 12, nan_frac=0.002)
+    dev = DeviceBuffer(ctx, (total, nsave), np.float32)
+    try:
+        for k, j in enumerate(sel):
+            row = np.ascontiguousarray(series[k])
+            _lib.check(_lib.load().pt_memcpy(ctx.h, ctypes.c_void_p(dev.ptr + int(j) * nsave * 4),
+                                             row.ctypes.data_as(ctypes.c_void_p), row.nbytes, 0))
+        lags = [1, 3, 10, 64, 300, 511]
+        edges = np.linspace(0.0, 1.0, 21)
+        for injection in (False, True):
+            acc = dmumu.new_accumulators(len(lags), 20)
+            dmumu.process_mu_chunk_dmumu(ctx, dev, lags, edges, acc, "injection" if injection else "sliding", True,
+                                         layout="time_fastest", particle_index=sel)
+            oacc = oracle.new_dmumu_accumulators(len(lags), 20)
+            oracle.dmumu(series, lags, edges, injection, True, oacc)
+            _check_dmumu(acc, oacc)
+    finally:
+        dev.free()
