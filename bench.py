@@ -444,15 +444,18 @@ def serving_run(args, log_fn):
         del F
     B0s = [f.B0 for f in flds]
     reqs = serving.make_stream(args.serving_requests, 4, B0s, rate_hz=20.0, seed=77)
-    res = serving.run_stream(push_ctx, stats_ctx, flds, reqs, budget_bytes=200 * 10 ** 9, steps_per_tick=200,
-                             box_length_m=L, log=log_fn)
+    res = serving.run_stream(push_ctx, stats_ctx, flds, reqs, budget_bytes=200 * 10 ** 9,
+                             steps_per_tick=args.serving_tick_steps, box_length_m=L, log=log_fn)
     for f in flds:
         f.close()
     stats_ctx.close()
     push_ctx.close()
     return {"requests": res.requests, "latency_p50_s": res.latency_p50_s, "latency_p99_s": res.latency_p99_s,
             "particle_steps_per_s": res.particle_steps_per_s, "particle_steps": res.particle_steps,
-            "wall_s": res.wall_s, "stream": "Poisson 20/s, 4 resident BF16 fields, 200 GB budget, "
+            "wall_s": res.wall_s, "tick_kernel_s": res.kernel_ms / 1e3,
+            "tick_kernel_particle_steps_per_s": res.particle_steps / max(res.kernel_ms / 1e3, 1e-12),
+            "product_thread_busy_s": res.product_busy_s, "tick_steps": args.serving_tick_steps,
+            "failed_requests": len(res.failed), "stream": "Poisson 20/s, 4 resident BF16 fields, 200 GB budget, "
             "energy log-U[1 GeV,100 TeV], n log-U[1e4,1e6], duration U[10,500] T_g0, mu+D_mumu(64x20) or D_pp"}
 
 
@@ -519,6 +522,8 @@ def main():
     ap.add_argument("--no-h5", action="store_true", help="skip the runner path with an HDF5 mu cache file")
     ap.add_argument("--serving-requests", type=int, default=100,
                     help="requests of the serving stream to replay (0 = skip; 2000 = the full stream)")
+    ap.add_argument("--serving-tick-steps", type=int, default=200,
+                    help="pushes per serving tick (one fused launch over all running requests)")
     ap.add_argument("--modes", action="store_true", help="also run the 96-campaign mode-decomposed batch")
     ap.add_argument("--mode-cells", type=int, default=256)
     ap.add_argument("--mode-gp", type=int, default=10)

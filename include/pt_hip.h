@@ -311,9 +311,16 @@ typedef struct {
                                 phase_space_gpu_runner.jl:170-194) on the device */
     double mu0;              /* injection_mu0 for inject_fixed_mu */
     double p_magnitude;      /* |p| (kg m/s) for inject_fixed_mu */
-    const double *x[3];      /* host positions (m) */
-    const double *p[3];      /* host momenta (kg m/s); unused when inject_fixed_mu */
+    const double *x[3];      /* host positions (m); unused when inject_on_device */
+    const double *p[3];      /* host momenta (kg m/s); unused when inject_fixed_mu or inject_on_device */
     const double *phase;     /* host gyrophases 2*pi*rand for inject_fixed_mu */
+    int32_t inject_on_device; /* 1: the initial state is drawn on the device at admission, as pt_inject
+                                 draws particles inject_first_index .. + n - 1 with `inject` (init_particles,
+                                 phase_space_gpu_runner.jl:196-245); no host arrays, no host O(n) work.
+                                 inject_fixed_mu, mu0, p_magnitude and the arrays above are then unused. */
+    int32_t reserved0;
+    int64_t inject_first_index;
+    pt_inject_params inject;
 } pt_serve_request;
 
 int pt_serve_create(pt_ctx *ctx, int64_t hbm_budget_bytes, int32_t steps_per_tick,
@@ -328,6 +335,10 @@ int pt_serve_submit(pt_serve *srv, const pt_serve_request *req, int64_t *request
 int pt_serve_step(pt_serve *srv, int64_t *n_running, int64_t *n_queued, int64_t *n_done);
 /* Waits for the enqueued tick, if any, and completes it. */
 int pt_serve_drain(pt_serve *srv);
+/* Ids of the requests that became DONE since the previous call, oldest first
+ * (at most `capacity`; the rest stay for the next call), so an event loop
+ * need not poll every outstanding request. */
+int pt_serve_take_done(pt_serve *srv, int64_t *ids, int64_t capacity, int64_t *count);
 int pt_serve_poll(pt_serve *srv, int64_t request_id, int32_t *state);
 /* For a DONE request: the device cadence cache [save][n] (float32 mu or
  * float64 |p|), its shape and the number of particles alive at the end. */

@@ -170,6 +170,8 @@ void launch_split_to_si(pt_ens *e, double *dx, double *dp, uint8_t *alive_out);
 // ([x1 x2 x3 p1 p2 p3][count], FP64 SI); *bad = 1 where fixed-mu meets zero |B|.
 void launch_inject(const pt_field *f, const pt_inject_params &ip, int64_t first, int64_t count, double *out,
                    int *bad, hipStream_t s);
+// The reference's checks on injection settings (pt_runtime.hip); throws.
+void check_inject_params(const pt_inject_params &ip, const pt_field *field);
 
 // Device scratch owned by the context, grown on demand (never shrinks).
 void *ctx_scratch(pt_ctx *ctx, size_t bytes);

@@ -419,7 +419,7 @@ static void check_pusher_field(int pusher, const pt_field *field, const char *wh
 
 // The reference's checks on the injection settings (init_particles,
 // phase_space_gpu_runner.jl:215-219; fixed_injection_position :164-166).
-static void check_inject_params(const pt_inject_params &ip, const pt_field *field) {
+void check_inject_params(const pt_inject_params &ip, const pt_field *field) {
     PT_REQUIRE(ip.fixed_position == 0 || ip.fixed_position == 1, "pt_inject_params.fixed_position must be 0 or 1");
     PT_REQUIRE(ip.fixed_mu == 0 || ip.fixed_mu == 1, "pt_inject_params.fixed_mu must be 0 or 1");
     PT_REQUIRE(std::isfinite(ip.p_magnitude), "pt_inject_params.p_magnitude must be finite");

@@ -249,6 +249,7 @@ struct pt_serve {
     std::deque<int64_t> queue;
     std::vector<int64_t> active;
     int64_t reserved = 0, next_id = 1, n_done = 0;
+    std::vector<int64_t> newly_done;  // DONE since the last pt_serve_take_done
     // Two ticks in flight: step() enqueues the next tick before waiting for
     // the previous one, so the device always has a tick queued while the
     // host admits, completes and hands out requests.
@@ -328,10 +329,9 @@ void fill_dev(pt_serve *srv, SReq &r) {
 }
 
 template <int STORE>
-void launch_init(pt_serve *srv, SReq &r, const double *dinit) {
+void launch_init(pt_serve *srv, SReq &r, const double *dinit, int fixed_mu) {
     const unsigned blocks = static_cast<unsigned>((r.d.n + 255) / 256);
-    k_serve_init<STORE><<<blocks, 256, 0, srv->ctx->stream>>>(r.dev, dinit, r.d.inject_fixed_mu,
-                                                               static_cast<float>(r.d.mu0),
+    k_serve_init<STORE><<<blocks, 256, 0, srv->ctx->stream>>>(r.dev, dinit, fixed_mu, static_cast<float>(r.d.mu0),
                                                                r.d.mass_kg * kC, r.d.p_magnitude);
     PT_HIP_CHECK(hipGetLastError());
 }
@@ -362,17 +362,29 @@ void admit(pt_serve *srv, SReq &r) {
     r.mem = static_cast<char *>(mem);
     srv->ctx->bytes_in_use += static_cast<int64_t>(r.bytes);
     fill_dev(srv, r);
-    double *dinit = static_cast<double *>(ctx_scratch(srv->ctx, r.init.size() * sizeof(double)));
-    PT_HIP_CHECK(hipMemcpyAsync(dinit, r.init.data(), r.init.size() * sizeof(double), hipMemcpyHostToDevice, st));
     PT_HIP_CHECK(hipMemsetAsync(r.dev.alive, 1, static_cast<size_t>(r.d.n), st));
-    PT_HIP_CHECK(hipMemsetAsync(r.dev.deaths, 0, kAlign, st));
+    PT_HIP_CHECK(hipMemsetAsync(r.dev.deaths, 0, kAlign, st));  // deaths and the injection flag
+    double *dinit;
+    int fixed_mu = r.d.inject_fixed_mu;
+    if (r.d.inject_on_device) {
+        // the initial state as pt_inject draws it, then converted like explicit
+        // host momenta; a fixed-mu failure is read when the request completes
+        dinit = static_cast<double *>(ctx_scratch(srv->ctx, static_cast<size_t>(r.d.n) * 6 * sizeof(double)));
+        launch_inject(srv->fields[static_cast<size_t>(r.d.field_id)], r.d.inject, r.d.inject_first_index, r.d.n,
+                      dinit, r.dev.bad, st);
+        fixed_mu = 0;
+    } else {
+        dinit = static_cast<double *>(ctx_scratch(srv->ctx, r.init.size() * sizeof(double)));
+        PT_HIP_CHECK(hipMemcpyAsync(dinit, r.init.data(), r.init.size() * sizeof(double), hipMemcpyHostToDevice,
+                                    st));
+    }
     switch (srv->storage) {
-        case PT_STORE_F32_BRICK: launch_init<PT_STORE_F32_BRICK>(srv, r, dinit); break;
-        case PT_STORE_BF16_BRICK: launch_init<PT_STORE_BF16_BRICK>(srv, r, dinit); break;
-        case PT_STORE_F16_BRICK: launch_init<PT_STORE_F16_BRICK>(srv, r, dinit); break;
+        case PT_STORE_F32_BRICK: launch_init<PT_STORE_F32_BRICK>(srv, r, dinit, fixed_mu); break;
+        case PT_STORE_BF16_BRICK: launch_init<PT_STORE_BF16_BRICK>(srv, r, dinit, fixed_mu); break;
+        case PT_STORE_F16_BRICK: launch_init<PT_STORE_F16_BRICK>(srv, r, dinit, fixed_mu); break;
         default: throw Error{PT_ERR_STATE, "server storage not set"};
     }
-    if (r.d.inject_fixed_mu) {
+    if (fixed_mu) {
         int bad = 0;
         PT_HIP_CHECK(hipMemcpyAsync(&bad, r.dev.bad, sizeof(int), hipMemcpyDeviceToHost, st));
         PT_HIP_CHECK(hipStreamSynchronize(st));
@@ -400,11 +412,14 @@ void complete_oldest(pt_serve *srv) {
         srv->particle_steps += pushes * r.d.n;
         r.si = si0 + K;
         if (r.si > r.d.total_steps) {
-            unsigned long long deaths = 0;
-            PT_HIP_CHECK(hipMemcpy(&deaths, r.dev.deaths, 8, hipMemcpyDeviceToHost));
+            unsigned long long flags[2] = {0, 0};  // deaths, then the injection flag (its int in the low half)
+            PT_HIP_CHECK(hipMemcpy(flags, r.dev.deaths, 16, hipMemcpyDeviceToHost));
+            const unsigned long long deaths = flags[0];
+            if (static_cast<uint32_t>(flags[1]) != 0) r.failed = true;
             r.alive_final = r.d.n - static_cast<int64_t>(deaths);
             r.state = PT_REQ_DONE;
             srv->n_done += 1;
+            srv->newly_done.push_back(r.id);
             srv->active.erase(std::remove(srv->active.begin(), srv->active.end(), r.id), srv->active.end());
         }
     }
@@ -551,15 +566,22 @@ int pt_serve_submit(pt_serve *srv, const pt_serve_request *req, int64_t *request
         PT_REQUIRE(d.total_steps >= 1 && d.save_stride >= 1, "pt_serve_submit: total_steps and save_stride must be >= 1");
         PT_REQUIRE(d.out_kind == PT_OUT_NONE || d.out_kind == PT_OUT_MU || d.out_kind == PT_OUT_PMAG,
                    "pt_serve_submit: out_kind must be PT_OUT_NONE, PT_OUT_MU or PT_OUT_PMAG");
-        for (int c = 0; c < 3; ++c) PT_REQUIRE(d.x[c], "pt_serve_submit: null position array");
-        if (d.inject_fixed_mu) {
-            PT_REQUIRE(d.phase, "pt_serve_submit: fixed-mu injection needs gyrophases");
-            PT_REQUIRE(d.mu0 >= -1.0 && d.mu0 <= 1.0, "injection_mu0 must be in [-1, 1]");
-            PT_REQUIRE(std::isfinite(d.p_magnitude) && d.p_magnitude > 0, "pt_serve_submit: p_magnitude must be positive");
-        } else {
-            for (int c = 0; c < 3; ++c) PT_REQUIRE(d.p[c], "pt_serve_submit: null momentum array");
-        }
+        PT_REQUIRE(d.inject_on_device == 0 || d.inject_on_device == 1, "pt_serve_submit: inject_on_device must be 0 or 1");
         const pt_field *f = srv->fields[static_cast<size_t>(d.field_id)];
+        if (d.inject_on_device) {
+            PT_REQUIRE(d.inject_first_index >= 0, "pt_serve_submit: inject_first_index must be >= 0");
+            check_inject_params(d.inject, f);
+        } else {
+            for (int c = 0; c < 3; ++c) PT_REQUIRE(d.x[c], "pt_serve_submit: null position array");
+            if (d.inject_fixed_mu) {
+                PT_REQUIRE(d.phase, "pt_serve_submit: fixed-mu injection needs gyrophases");
+                PT_REQUIRE(d.mu0 >= -1.0 && d.mu0 <= 1.0, "injection_mu0 must be in [-1, 1]");
+                PT_REQUIRE(std::isfinite(d.p_magnitude) && d.p_magnitude > 0,
+                           "pt_serve_submit: p_magnitude must be positive");
+            } else {
+                for (int c = 0; c < 3; ++c) PT_REQUIRE(d.p[c], "pt_serve_submit: null momentum array");
+            }
+        }
         const size_t n = static_cast<size_t>(d.n);
         SReq r;
         r.id = srv->next_id++;
@@ -568,22 +590,24 @@ int pt_serve_submit(pt_serve *srv, const pt_serve_request *req, int64_t *request
         r.bytes = request_bytes(d, r.nsave);
         PT_REQUIRE(static_cast<int64_t>(r.bytes) <= srv->budget,
                    "pt_serve_submit: request needs " + std::to_string(r.bytes) + " bytes, more than the whole budget");
-        r.init.resize(d.inject_fixed_mu ? 4 * n : 6 * n);
-        for (int c = 0; c < 3; ++c)
-            for (size_t i = 0; i < n; ++i) {
-                const double v = d.x[c][i];
-                PT_REQUIRE(std::isfinite(v) && v >= f->g.lo[c] && v <= f->g.hi[c],
-                           "pt_serve_submit: initial position outside the simulation box");
-                r.init[c * n + i] = v;
-            }
-        if (d.inject_fixed_mu) {
-            std::memcpy(r.init.data() + 3 * n, d.phase, n * sizeof(double));
-        } else {
+        if (!d.inject_on_device) {  // copy the host state (device injection draws it at admission)
+            r.init.resize(d.inject_fixed_mu ? 4 * n : 6 * n);
             for (int c = 0; c < 3; ++c)
                 for (size_t i = 0; i < n; ++i) {
-                    PT_REQUIRE(std::isfinite(d.p[c][i]), "pt_serve_submit: non-finite momentum");
-                    r.init[(3 + c) * n + i] = d.p[c][i];
+                    const double v = d.x[c][i];
+                    PT_REQUIRE(std::isfinite(v) && v >= f->g.lo[c] && v <= f->g.hi[c],
+                               "pt_serve_submit: initial position outside the simulation box");
+                    r.init[c * n + i] = v;
                 }
+            if (d.inject_fixed_mu) {
+                std::memcpy(r.init.data() + 3 * n, d.phase, n * sizeof(double));
+            } else {
+                for (int c = 0; c < 3; ++c)
+                    for (size_t i = 0; i < n; ++i) {
+                        PT_REQUIRE(std::isfinite(d.p[c][i]), "pt_serve_submit: non-finite momentum");
+                        r.init[(3 + c) * n + i] = d.p[c][i];
+                    }
+            }
         }
         for (int c = 0; c < 3; ++c) {
             r.d.x[c] = nullptr;
@@ -611,6 +635,7 @@ int pt_serve_step(pt_serve *srv, int64_t *n_running, int64_t *n_queued, int64_t 
             if (r.failed) {
                 r.state = PT_REQ_DONE;
                 srv->n_done += 1;
+                srv->newly_done.push_back(r.id);
             } else {
                 srv->active.push_back(r.id);
             }
@@ -622,6 +647,16 @@ int pt_serve_step(pt_serve *srv, int64_t *n_running, int64_t *n_queued, int64_t 
         if (n_running) *n_running = static_cast<int64_t>(srv->active.size());
         if (n_queued) *n_queued = static_cast<int64_t>(srv->queue.size());
         if (n_done) *n_done = srv->n_done;
+    });
+}
+
+int pt_serve_take_done(pt_serve *srv, int64_t *ids, int64_t capacity, int64_t *count) {
+    return guarded([&] {
+        PT_REQUIRE(srv && count && (ids || capacity == 0) && capacity >= 0, "pt_serve_take_done: bad argument");
+        const int64_t m = std::min<int64_t>(capacity, static_cast<int64_t>(srv->newly_done.size()));
+        std::copy(srv->newly_done.begin(), srv->newly_done.begin() + m, ids);
+        srv->newly_done.erase(srv->newly_done.begin(), srv->newly_done.begin() + m);
+        *count = m;
     });
 }
 

@@ -37,7 +37,9 @@ class ServeRequest(ctypes.Structure):
                 ("total_steps", ctypes.c_int64), ("save_stride", ctypes.c_int64), ("periodic", ctypes.c_int32),
                 ("inject_fixed_mu", ctypes.c_int32), ("mu0", ctypes.c_double), ("p_magnitude", ctypes.c_double),
                 ("x", ctypes.POINTER(ctypes.c_double) * 3), ("p", ctypes.POINTER(ctypes.c_double) * 3),
-                ("phase", ctypes.POINTER(ctypes.c_double))]
+                ("phase", ctypes.POINTER(ctypes.c_double)), ("inject_on_device", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32), ("inject_first_index", ctypes.c_int64),
+                ("inject", _lib.InjectParams)]
 
 
 def _bind(L):
@@ -52,6 +54,7 @@ def _bind(L):
         "pt_serve_submit": [vp, ctypes.POINTER(ServeRequest), i64p],
         "pt_serve_step": [vp, i64p, i64p, i64p],
         "pt_serve_drain": [vp],
+        "pt_serve_take_done": [vp, i64p, i64, i64p],
         "pt_serve_poll": [vp, i64, ctypes.POINTER(i32)],
         "pt_serve_fetch": [vp, i64, ctypes.POINTER(vp), i64p, i64p, i64p],
         "pt_serve_copy_cache": [vp, i64, vp, i64],
@@ -82,14 +85,23 @@ class Server:
         check(self.L.pt_serve_add_field(self.h, field.h, ctypes.byref(fid)))
         return fid.value
 
-    def submit(self, field_id, x, p=None, *, dt, total_steps, stride, out_kind=OUT_MU, charge=Q_E, mass=M_P,
-               periodic=True, mu0=None, phase=None, p_magnitude=None) -> int:
-        n = len(x[0])
-        keep = [np.ascontiguousarray(v, dtype=np.float64) for v in x]
+    def submit(self, field_id, x=None, p=None, *, dt, total_steps, stride, out_kind=OUT_MU, charge=Q_E, mass=M_P,
+               periodic=True, mu0=None, phase=None, p_magnitude=None, n=None, inject=None, first_index=0) -> int:
+        """Host state: x (and p, or mu0 + phase + p_magnitude for fixed-mu).
+        Device state: n particles and `inject` (a _lib.InjectParams), drawn at
+        admission as pt_inject draws particles first_index .. first_index+n-1."""
         r = ServeRequest()
-        r.field_id, r.out_kind, r.n = int(field_id), int(out_kind), n
+        r.field_id, r.out_kind = int(field_id), int(out_kind)
         r.dt_s, r.charge_C, r.mass_kg = float(dt), float(charge), float(mass)
         r.total_steps, r.save_stride, r.periodic = int(total_steps), int(stride), int(bool(periodic))
+        if inject is not None:
+            r.n, r.inject_on_device, r.inject_first_index, r.inject = int(n), 1, int(first_index), inject
+            rid = ctypes.c_int64()
+            check(self.L.pt_serve_submit(self.h, ctypes.byref(r), ctypes.byref(rid)))
+            return rid.value
+        n = len(x[0])
+        r.n = n
+        keep = [np.ascontiguousarray(v, dtype=np.float64) for v in x]
         for c in range(3):
             r.x[c] = keep[c].ctypes.data_as(_lib._dp)
         if mu0 is not None:
@@ -113,6 +125,13 @@ class Server:
 
     def drain(self):
         check(self.L.pt_serve_drain(self.h))
+
+    def take_done(self, capacity=4096):
+        """Ids of requests that became DONE since the previous call."""
+        buf = (ctypes.c_int64 * capacity)()
+        cnt = ctypes.c_int64()
+        check(self.L.pt_serve_take_done(self.h, buf, capacity, ctypes.byref(cnt)))
+        return list(buf[:cnt.value])
 
     def poll(self, rid) -> int:
         s = ctypes.c_int32()
@@ -208,8 +227,22 @@ def make_stream(n_requests, n_fields, B0s, *, rate_hz=20.0, seed=7, e_range=(1.0
     return reqs
 
 
+def request_inject_params(req):
+    """The request's device injection: uniform positions in the box, fixed-mu
+    (req.mu0) or isotropic directions, |p| = gamma m_p v (init_particles,
+    phase_space_gpu_runner.jl:196-245), Philox4x64-10 keyed by req.seed."""
+    ip = _lib.InjectParams()
+    ip.seed = int(req.seed) & ((1 << 64) - 1)
+    ip.fixed_position = 0
+    ip.fixed_mu = int(req.mu0 is not None)
+    ip.mu0 = float(req.mu0) if req.mu0 is not None else 0.0
+    ip.p_magnitude = req.gamma * M_P * C_LIGHT * math.sqrt(1 - 1 / req.gamma ** 2)
+    return ip
+
+
 def request_state(req, L):
-    """Initial positions (uniform) and momenta/phases of one request."""
+    """Initial positions (uniform) and momenta/phases of one request, drawn
+    on the host (numpy) for submissions that pass host arrays."""
     rng = np.random.default_rng(req.seed)
     x = rng.random((3, req.n)) * L
     pmag = req.gamma * M_P * C_LIGHT * math.sqrt(1 - 1 / req.gamma ** 2)
@@ -221,7 +254,13 @@ def request_state(req, L):
     return x, pmag * np.stack([r * np.cos(phi), r * np.sin(phi), z]), None, pmag
 
 
-def submit_request(server, req, L):
+def submit_request(server, req, L, on_device=True):
+    """Submit one stream request.  on_device: its initial state is drawn at
+    admission (request_inject_params), so the host does no O(n) work per
+    request; otherwise request_state draws it on the host."""
+    if on_device:
+        return server.submit(req.field_id, n=req.n, inject=request_inject_params(req), dt=req.dt,
+                             total_steps=req.total_steps, stride=req.stride, out_kind=req.out_kind)
     x, p, phase, pmag = request_state(req, L)
     return server.submit(req.field_id, x, p, dt=req.dt, total_steps=req.total_steps, stride=req.stride,
                          out_kind=req.out_kind, mu0=req.mu0, phase=phase, p_magnitude=pmag)
@@ -256,6 +295,16 @@ def run_stream(push_ctx, stats_ctx, fields, reqs, *, budget_bytes, steps_per_tic
 
     srv = Server(push_ctx, budget_bytes, steps_per_tick, max_active)
     fids = [srv.add_field(f) for f in fields]
+    busy = [0.0]
+    failed = []
+
+    def timed_product(*a):
+        t = time.perf_counter()
+        try:
+            return request_product(*a)
+        finally:
+            busy[0] += time.perf_counter() - t
+
     assert fids == list(range(len(fields)))
     pending = sorted(reqs, key=lambda r: r.arrival)
     outstanding = {}
@@ -274,10 +323,15 @@ def run_stream(push_ctx, stats_ctx, fields, reqs, *, budget_bytes, steps_per_tic
                 outstanding[submit_request(srv, rq, box_length_m)] = rq
                 k += 1
             running, queued, done = srv.step()
-            for rid in [r for r in outstanding if srv.poll(r) == DONE]:
+            for rid in srv.take_done():
                 rq = outstanding.pop(rid)
-                cp, nsave, n, _ = srv.fetch(rid)
-                finishing[rid] = (pool.submit(request_product, stats_ctx, cp, nsave, n, rq), rq)
+                try:
+                    cp, nsave, n, _ = srv.fetch(rid)
+                except _lib.PTError as e:  # e.g. fixed-mu injection where the local B is zero
+                    failed.append((rq.i, str(e)))
+                    srv.release(rid)
+                    continue
+                finishing[rid] = (pool.submit(timed_product, stats_ctx, cp, nsave, n, rq), rq)
             for rid in [r for r, (f, _) in finishing.items() if f.done()]:
                 fut, rq = finishing.pop(rid)
                 res = fut.result()
@@ -302,5 +356,5 @@ def run_stream(push_ctx, stats_ctx, fields, reqs, *, budget_bytes, steps_per_tic
     lat = np.asarray(lat)
     return SimpleNamespace(latency_p50_s=float(np.percentile(lat, 50)), latency_p99_s=float(np.percentile(lat, 99)),
                            requests=len(lat), wall_s=wall, particle_steps=st.particle_steps,
-                           particle_steps_per_s=st.particle_steps / wall, kernel_ms=st.kernel_ms, results=results,
-                           latencies=lat)
+                           particle_steps_per_s=st.particle_steps / wall, kernel_ms=st.kernel_ms,
+                           product_busy_s=busy[0], failed=failed, results=results, latencies=lat)

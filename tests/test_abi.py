@@ -43,3 +43,37 @@ def test_h5_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(h5.LIB_PATH)
     assert not [s for s in syms if not hasattr(lib, s)]
     assert set(syms) == set(h5.EXPORTS)
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """sizeof and every field offset of the structs the bindings mirror, as a
+    C compiler lays them out from include/pt_hip.h."""
+    import shutil
+    import subprocess
+
+    import pytest
+
+    from particle_transport_amd import serving
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    mirrors = {"pt_inject_params": _lib.InjectParams, "pt_advance_params": _lib.AdvanceParams,
+               "pt_serve_request": serving.ServeRequest}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pt_hip.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

@@ -37,10 +37,10 @@ def req(i, n, E, dur_steps, kind, B0, mu0=None, seed=0):
                            total_steps=dur_steps, stride=10, gamma=gamma, seed=seed)
 
 
-def run_alone(ctx, fld, r, steps_per_tick=64):
+def run_alone(ctx, fld, r, steps_per_tick=64, on_device=True):
     srv = serving.Server(ctx, 1 << 32, steps_per_tick)
     srv.add_field(fld)
-    rid = serving.submit_request(srv, r, L_BOX)
+    rid = serving.submit_request(srv, r, L_BOX, on_device=on_device)
     while srv.poll(rid) != serving.DONE:
         srv.step()
     out = srv.copy_cache(rid, np.float32 if r.out_kind == serving.OUT_MU else np.float64)
@@ -50,21 +50,22 @@ def run_alone(ctx, fld, r, steps_per_tick=64):
     return out, alive
 
 
-def test_alone_equals_mix_and_single_campaign_pusher(setup):
+@pytest.mark.parametrize("on_device", [True, False], ids=["device_injection", "host_state"])
+def test_alone_equals_mix_and_single_campaign_pusher(setup, on_device):
     ctx, stats, fld, F = setup
     B0 = fld.B0
     A = req(0, 3000, 10.0, 401, serving.OUT_MU, B0, seed=1)
     B = req(1, 777, 1e4, 251, serving.OUT_PMAG, B0, seed=2)
     C = req(2, 5000, 3.0, 333, serving.OUT_MU, B0, mu0=0.4, seed=3)
-    alone = {r.i: run_alone(ctx, fld, r) for r in (A, B, C)}
+    alone = {r.i: run_alone(ctx, fld, r, on_device=on_device) for r in (A, B, C)}
     srv = serving.Server(ctx, 1 << 32, 64)
     srv.add_field(fld)
-    ids = {serving.submit_request(srv, A, L_BOX): A}
+    ids = {serving.submit_request(srv, A, L_BOX, on_device=on_device): A}
     srv.step()
     srv.step()  # B and C join while A is running
-    ids[serving.submit_request(srv, B, L_BOX)] = B
+    ids[serving.submit_request(srv, B, L_BOX, on_device=on_device)] = B
     srv.step()
-    ids[serving.submit_request(srv, C, L_BOX)] = C
+    ids[serving.submit_request(srv, C, L_BOX, on_device=on_device)] = C
     mixed = {}
     while len(mixed) < 3:
         srv.step()
@@ -90,13 +91,20 @@ def test_alone_equals_mix_and_single_campaign_pusher(setup):
     buf.free()
     assert np.array_equal(acc_alone.dmumu_counts, mix_acc.dmumu_counts)
     assert np.allclose(acc_alone.dmumu_sum_delta2, mix_acc.dmumu_sum_delta2, rtol=1e-12, atol=0)
-    # the same request through the single-campaign production pusher
-    x, p, _, _ = serving.request_state(A, L_BOX)
-    ens = Ensemble(ctx, fld, x, p, pusher="fast")
-    r = ens.advance(dt=A.dt, first_step=1, n_steps=A.total_steps, total_steps=A.total_steps, stride=A.stride,
-                    out_kind="mu", steps_per_launch=100)
-    ens.close()
-    assert np.array_equal(r.out_a, alone[0][0])
+    # the same requests through the single-campaign production pusher (the
+    # device-injected ones from pt_ens_create_injected with the same draws)
+    for R in (A, C):
+        if on_device:
+            ens = Ensemble.injected(ctx, fld, R.n, serving.request_inject_params(R), pusher="fast")
+        elif R.mu0 is None:
+            x, p, _, _ = serving.request_state(R, L_BOX)
+            ens = Ensemble(ctx, fld, x, p, pusher="fast")
+        else:
+            continue  # host fixed-mu states are built inside the server from host gyrophases
+        r = ens.advance(dt=R.dt, first_step=1, n_steps=R.total_steps, total_steps=R.total_steps, stride=R.stride,
+                        out_kind="mu", steps_per_launch=100)
+        ens.close()
+        assert np.array_equal(r.out_a, alone[R.i][0]), R.i
 
 
 def test_budget_admission_fifo(setup):
@@ -124,11 +132,37 @@ def test_budget_admission_fifo(setup):
     srv.close()
 
 
-def test_fixed_mu_injection_starts_at_mu0(setup):
+@pytest.mark.parametrize("on_device", [True, False], ids=["device_injection", "host_state"])
+def test_fixed_mu_injection_starts_at_mu0(setup, on_device):
     ctx, stats, fld, F = setup
     r = req(0, 4096, 100.0, 11, serving.OUT_MU, fld.B0, mu0=-0.6, seed=9)
-    mu, _ = run_alone(ctx, fld, r)
+    mu, _ = run_alone(ctx, fld, r, on_device=on_device)
     assert np.max(np.abs(mu[0] + 0.6)) <= 2e-5
+
+
+def test_device_injection_on_a_zero_field_fails_at_fetch(setup):
+    """A fixed-mu request where the local B is zero: admission does not wait
+    for the injection kernel, so the error (the reference's "cannot use
+    fixed-mu injection where local B is zero") is reported when the request's
+    result is fetched; a request beside it is unaffected."""
+    ctx, stats, fld, F = setup
+    zero = Field(ctx, [np.zeros((9, 9, 9), order="F") for _ in range(6)], L_BOX, "f32_brick")
+    srv = serving.Server(ctx, 1 << 30, 50)
+    srv.add_field(zero)
+    bad = req(0, 1000, 10.0, 101, serving.OUT_MU, 1e-10, mu0=0.3, seed=5)
+    good = req(1, 1000, 10.0, 101, serving.OUT_MU, 1e-10, seed=6)
+    ib = serving.submit_request(srv, bad, L_BOX)
+    ig = serving.submit_request(srv, good, L_BOX)
+    while srv.poll(ib) != serving.DONE or srv.poll(ig) != serving.DONE:
+        srv.step()
+    from particle_transport_amd._lib import PTError
+    with pytest.raises(PTError, match="local B is zero"):
+        srv.fetch(ib)
+    assert srv.fetch(ig)[3] == 1000
+    srv.release(ib)
+    srv.release(ig)
+    srv.close()
+    zero.close()
 
 
 def test_released_arenas_return_to_the_device(setup):
