@@ -332,18 +332,34 @@ __device__ __forceinline__ void load_component_corners(const FieldView &fv, int3
         const int32_t sx = (gi & 7) == 7 ? 512 - 7 : 1;
         const int32_t sy = (gj & 7) == 7 ? nb0 * 512 - 56 : 8;
         const int32_t sz = (gk & 7) == 7 ? nb0 * nb1 * 512 - 448 : 64;
+        // This component's half of a 32-bit word, taken per lane without a
+        // select between the two halves (with a select the compiler waited
+        // on each of the eight loads in turn): FP16 by a shift, BF16 by one
+        // byte permute that moves the half to the top and zeroes the rest.
+        const uint32_t sh = (comp & 1) ? 16u : 0u;
+        const uint32_t sel = (comp & 1) ? 0x03020c0cu : 0x01000c0cu;
+        if constexpr (STORE == PT_STORE_BF16_BRICK) {
+            // the eight loads first, then the decodes: the scheduler keeps
+            // them apart, so all eight are in flight together
+            uint32_t w[8];
 #pragma unroll
-        for (int ci = 0; ci < 8; ++ci) {
-            const int64_t rec = base + ((ci & 1) ? sx : 0) + ((ci & 2) ? sy : 0) + ((ci & 4) ? sz : 0);
-            if constexpr (STORE == PT_STORE_F32_BRICK) {
-                s[ci] = static_cast<const float *>(fv.data)[rec * 8 + comp];
-            } else {
-                const uint32_t w = static_cast<const uint32_t *>(fv.data)[rec * 4 + (comp >> 1)];
-                const uint32_t h = (comp & 1) ? (w >> 16) : (w & 0xffffu);
-                if constexpr (STORE == PT_STORE_BF16_BRICK)
-                    s[ci] = __uint_as_float(h << 16);
-                else
-                    s[ci] = f16_bits_to_float(static_cast<uint16_t>(h));
+            for (int ci = 0; ci < 8; ++ci) {
+                const int64_t rec = base + ((ci & 1) ? sx : 0) + ((ci & 2) ? sy : 0) + ((ci & 4) ? sz : 0);
+                w[ci] = static_cast<const uint32_t *>(fv.data)[rec * 4 + (comp >> 1)];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int ci = 0; ci < 8; ++ci) s[ci] = __uint_as_float(__builtin_amdgcn_perm(w[ci], w[ci], sel));
+        } else {
+#pragma unroll
+            for (int ci = 0; ci < 8; ++ci) {
+                const int64_t rec = base + ((ci & 1) ? sx : 0) + ((ci & 2) ? sy : 0) + ((ci & 4) ? sz : 0);
+                if constexpr (STORE == PT_STORE_F32_BRICK) {
+                    s[ci] = static_cast<const float *>(fv.data)[rec * 8 + comp];
+                } else {
+                    const uint32_t w = static_cast<const uint32_t *>(fv.data)[rec * 4 + (comp >> 1)];
+                    s[ci] = f16_bits_to_float(static_cast<uint16_t>((w >> sh) & 0xffffu));
+                }
             }
         }
     }
