@@ -323,6 +323,9 @@ typedef struct {
     pt_inject_params inject;
 } pt_serve_request;
 
+/* Reserves one device arena of the budget plus 12.5 % (for best-fit
+ * fragmentation) for the server's lifetime; requests' state and caches are
+ * placed in it at admission and returned at pt_serve_release. */
 int pt_serve_create(pt_ctx *ctx, int64_t hbm_budget_bytes, int32_t steps_per_tick,
                     int32_t max_active, pt_serve **out);
 int pt_serve_destroy(pt_serve *srv);
@@ -347,7 +350,8 @@ int pt_serve_fetch(pt_serve *srv, int64_t request_id, const void **cache_device,
 /* Copies a DONE request's cadence cache to host memory (bytes must equal
  * nsave * n * element size). */
 int pt_serve_copy_cache(pt_serve *srv, int64_t request_id, void *host_dst, int64_t bytes);
-/* Frees a DONE request's device memory (or drops a QUEUED one). */
+/* Returns a DONE request's memory to the arena (or drops a QUEUED one).
+ * Call it after the last use of the request's cache has completed. */
 int pt_serve_release(pt_serve *srv, int64_t request_id);
 /* Budget bytes held by admitted requests, particle-steps pushed so far and
  * the summed kernel time of all ticks (HIP events on the server's stream). */

@@ -211,6 +211,65 @@ int guarded(F &&fn) {
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
+// Best-fit allocator over the server's one device arena (offsets in bytes,
+// multiples of kAlign).  Free blocks are kept by offset, to merge a released
+// block with its neighbours, and by length, to place a request in the
+// smallest block that holds it.
+class Arena {
+  public:
+    void reset(size_t bytes) {
+        by_off_.clear();
+        by_len_.clear();
+        if (bytes) insert(0, bytes);
+    }
+    bool alloc(size_t n, size_t &off) {
+        auto it = by_len_.lower_bound(n);
+        if (it == by_len_.end()) return false;
+        const size_t len = it->first;
+        off = it->second;
+        erase(off, len);
+        if (len > n) insert(off + n, len - n);
+        return true;
+    }
+    void release(size_t off, size_t n) {
+        auto next = by_off_.lower_bound(off);
+        if (next != by_off_.end() && off + n == next->first) {
+            const size_t o = next->first, l = next->second;
+            erase(o, l);
+            n += l;
+        }
+        auto after = by_off_.lower_bound(off);
+        if (after != by_off_.begin()) {
+            auto prev = std::prev(after);
+            if (prev->first + prev->second == off) {
+                const size_t o = prev->first, l = prev->second;
+                erase(o, l);
+                off = o;
+                n += l;
+            }
+        }
+        insert(off, n);
+    }
+    size_t largest() const { return by_len_.empty() ? 0 : by_len_.rbegin()->first; }
+
+  private:
+    void insert(size_t off, size_t len) {
+        by_off_.emplace(off, len);
+        by_len_.emplace(len, off);
+    }
+    void erase(size_t off, size_t len) {
+        by_off_.erase(off);
+        auto range = by_len_.equal_range(len);
+        for (auto it = range.first; it != range.second; ++it)
+            if (it->second == off) {
+                by_len_.erase(it);
+                break;
+            }
+    }
+    std::map<size_t, size_t> by_off_;
+    std::multimap<size_t, size_t> by_len_;
+};
+
 // One launch of k_serve: its request table (pinned host copy and device
 // copy), events and the work it carries (request id, first step, steps).
 struct Tick {
@@ -257,15 +316,14 @@ struct pt_serve {
     std::deque<int> inflight;  // slots of enqueued, not yet completed ticks, oldest first
     int64_t particle_steps = 0;
     double kernel_ms = 0.0;
-    // Arenas come from the device's stream-ordered pool (hipMallocAsync),
-    // which keeps freed memory reserved: with requests of every size, and
-    // frees that race the next ticks, that reserve grows without bound, and
-    // the pool gives it back only after a device-wide synchronisation.  So
-    // once kTrimSlack bytes of arenas have been released, step()
-    // synchronises the device and trims the pool; admission does the same
-    // before retrying an allocation that failed.
-    hipMemPool_t mempool = nullptr;
-    int64_t freed_since_trim = 0;
+    // Requests' memory comes from one device arena reserved when the server
+    // is created (the budget plus kArenaSlack for fragmentation), placed by
+    // best fit.  Admission and release are host bookkeeping: no allocation,
+    // free or device-wide synchronisation while the server runs, so a tick
+    // never waits for work on other streams (a statistics product, say).
+    char *arena_base = nullptr;
+    size_t arena_bytes = 0;
+    pt::Arena arena;
 };
 
 namespace pt {
@@ -336,31 +394,16 @@ void launch_init(pt_serve *srv, SReq &r, const double *dinit, int fixed_mu) {
     PT_HIP_CHECK(hipGetLastError());
 }
 
-// Synchronise the device (the pool releases freed arenas only then) and give
-// the pool's unused memory back to the device.
-constexpr int64_t kTrimSlack = int64_t{8} << 30;
+// The arena is the budget plus this fraction of it, so that best-fit
+// placement of requests of every size still finds room for the budget.
+constexpr double kArenaSlack = 0.125;
 
-void trim_pool(pt_serve *srv) {
-    PT_HIP_CHECK(hipDeviceSynchronize());
-    PT_HIP_CHECK(hipMemPoolTrimTo(srv->mempool, 0));
-    srv->freed_since_trim = 0;
-}
-
-void admit(pt_serve *srv, SReq &r) {
+// Places the request in the arena (false: no free block holds it yet).
+bool admit(pt_serve *srv, SReq &r) {
     hipStream_t st = srv->ctx->stream;
-    void *mem = nullptr;
-    hipError_t e = hipMallocAsync(&mem, r.bytes, st);
-    if (e == hipErrorOutOfMemory) {  // freed arenas may still sit in the pool
-        (void)hipGetLastError();
-        trim_pool(srv);
-        e = hipMallocAsync(&mem, r.bytes, st);
-    }
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error{PT_ERR_MEMORY, std::string("hipMallocAsync failed: ") + hipGetErrorString(e)};
-    }
-    r.mem = static_cast<char *>(mem);
-    srv->ctx->bytes_in_use += static_cast<int64_t>(r.bytes);
+    size_t off = 0;
+    if (!srv->arena.alloc(r.bytes, off)) return false;
+    r.mem = srv->arena_base + off;
     fill_dev(srv, r);
     PT_HIP_CHECK(hipMemsetAsync(r.dev.alive, 1, static_cast<size_t>(r.d.n), st));
     PT_HIP_CHECK(hipMemsetAsync(r.dev.deaths, 0, kAlign, st));  // deaths and the injection flag
@@ -393,6 +436,7 @@ void admit(pt_serve *srv, SReq &r) {
     std::vector<double>().swap(r.init);
     srv->reserved += static_cast<int64_t>(r.bytes);
     r.state = PT_REQ_RUNNING;
+    return true;
 }
 
 // Wait for the oldest in-flight tick and retire its work: requests whose
@@ -509,7 +553,23 @@ int pt_serve_create(pt_ctx *ctx, int64_t hbm_budget_bytes, int32_t steps_per_tic
             PT_HIP_CHECK(hipEventCreate(&t.ev0));
             PT_HIP_CHECK(hipEventCreate(&t.ev1));
         }
-        PT_HIP_CHECK(hipDeviceGetDefaultMemPool(&s->mempool, ctx->device));
+        s->arena_bytes = align_up(static_cast<size_t>(static_cast<double>(hbm_budget_bytes) * (1.0 + kArenaSlack)));
+        void *mem = nullptr;
+        const hipError_t e = hipMalloc(&mem, s->arena_bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            for (auto &t : s->tick) {
+                (void)hipEventDestroy(t.ev0);
+                (void)hipEventDestroy(t.ev1);
+            }
+            const size_t want = s->arena_bytes;
+            delete s;
+            throw Error{PT_ERR_MEMORY, "pt_serve_create: cannot reserve the " + std::to_string(want) +
+                                           "-byte arena for the HBM budget: " + hipGetErrorString(e)};
+        }
+        s->arena_base = static_cast<char *>(mem);
+        s->arena.reset(s->arena_bytes);
+        ctx->bytes_in_use += static_cast<int64_t>(s->arena_bytes);
         *out = s;
     });
 }
@@ -521,14 +581,11 @@ int pt_serve_destroy(pt_serve *srv) {
             complete_all(srv);
         } catch (...) {
         }
-        hipStream_t st = srv->ctx->stream;
-        for (auto &kv : srv->reqs)
-            if (kv.second.mem) {
-                (void)hipFreeAsync(kv.second.mem, st);
-                srv->ctx->bytes_in_use -= static_cast<int64_t>(kv.second.bytes);
-            }
-        (void)hipStreamSynchronize(st);
-        if (srv->mempool) (void)hipMemPoolTrimTo(srv->mempool, 0);
+        (void)hipStreamSynchronize(srv->ctx->stream);
+        if (srv->arena_base) {
+            (void)hipFree(srv->arena_base);
+            srv->ctx->bytes_in_use -= static_cast<int64_t>(srv->arena_bytes);
+        }
         for (auto &t : srv->tick) {
             if (t.d_table) (void)hipFree(t.d_table);
             if (t.h_table) (void)hipHostFree(t.h_table);
@@ -626,12 +683,11 @@ int pt_serve_step(pt_serve *srv, int64_t *n_running, int64_t *n_queued, int64_t 
         PT_HIP_CHECK(hipSetDevice(srv->ctx->device));
         // keep at most one tick in flight while this call enqueues the next
         while (srv->inflight.size() > 1) complete_oldest(srv);
-        if (srv->freed_since_trim > kTrimSlack) trim_pool(srv);
         while (!srv->queue.empty() && static_cast<int32_t>(srv->active.size()) < srv->max_active) {
             SReq &r = srv->reqs.at(srv->queue.front());
             if (srv->reserved + static_cast<int64_t>(r.bytes) > srv->budget) break;  // FIFO: wait for memory
+            if (!admit(srv, r)) break;  // no free block holds it yet: wait for releases
             srv->queue.pop_front();
-            admit(srv, r);
             if (r.failed) {
                 r.state = PT_REQ_DONE;
                 srv->n_done += 1;
@@ -717,10 +773,10 @@ int pt_serve_release(pt_serve *srv, int64_t request_id) {
             srv->queue.erase(std::remove(srv->queue.begin(), srv->queue.end(), request_id), srv->queue.end());
         }
         if (r.mem) {
-            PT_HIP_CHECK(hipFreeAsync(r.mem, srv->ctx->stream));
-            srv->ctx->bytes_in_use -= static_cast<int64_t>(r.bytes);
+            // a DONE request's ticks have completed; its cache's other readers
+            // finished before the caller releases it
+            srv->arena.release(static_cast<size_t>(r.mem - srv->arena_base), r.bytes);
             srv->reserved -= static_cast<int64_t>(r.bytes);
-            srv->freed_since_trim += static_cast<int64_t>(r.bytes);
             r.mem = nullptr;
         }
         srv->reqs.erase(it);
