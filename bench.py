@@ -520,11 +520,12 @@ def main():
     ap.add_argument("--no-stats", action="store_true", help="skip the D_mumu / D_pp pair-visit rates")
     ap.add_argument("--no-pcie", action="store_true", help="skip the runner path with a host mu cache")
     ap.add_argument("--no-h5", action="store_true", help="skip the runner path with an HDF5 mu cache file")
-    ap.add_argument("--serving-requests", type=int, default=100,
-                    help="requests of the serving stream to replay (0 = skip; 2000 = the full stream)")
+    ap.add_argument("--serving-requests", type=int, default=2000,
+                    help="requests of the serving stream to replay in real time (0 = skip; the default 2000 is the "
+                         "whole stream, about 105 s)")
     ap.add_argument("--serving-tick-steps", type=int, default=200,
                     help="pushes per serving tick (one fused launch over all running requests)")
-    ap.add_argument("--modes", action="store_true", help="also run the 96-campaign mode-decomposed batch")
+    ap.add_argument("--no-modes", action="store_true", help="skip the 96-campaign mode-decomposed batch")
     ap.add_argument("--mode-cells", type=int, default=256)
     ap.add_argument("--mode-gp", type=int, default=10)
     ap.add_argument("--pmc-probe", action="store_true",
@@ -648,7 +649,7 @@ def main():
     del F32
 
     serve = serving_run(args, log) if args.serving_requests > 0 else None
-    modes = modes_run(args, log) if args.modes else None
+    modes = None if args.no_modes else modes_run(args, log)
 
     # Headline: the fastest field format whose D_mumu passes the check against
     # the FP64 pusher (the fastest format when the check is skipped).
