@@ -274,6 +274,18 @@ int pt_dpp_accumulate_pmag(pt_ctx *ctx, const double *pmag, int on_device, int64
                            double *sum_delta_p, double *sum_delta_p2, double *sum_delta_p_norm,
                            double *sum_delta_p_norm2);
 
+/* The same pairs over a relative cache delta = |p|/p_ref - 1 (float32,
+ * [save][total]), as a serving request with PT_OUT_DPREL writes it: Delta p =
+ * p_ref (delta(t+tau) - delta(t)), the difference taken in FP64.  Half the
+ * bytes of the FP64 |p| cache; the rounding of delta to float32 is |delta|
+ * 6e-8 relative, |delta| times smaller than the rounding of a Float32
+ * phase-space cache that the reference's D_pp reads (its |p| carries 6e-8
+ * relative).  Same accumulators (kg m/s and p0-normalised) and selection. */
+int pt_dpp_accumulate_dprel(pt_ctx *ctx, const float *dprel, int on_device, int64_t nsave, int64_t total,
+                            int64_t n_sel, const int64_t *particle_index, const int64_t *lag_steps, int64_t nlags,
+                            double p_ref, double p0, int64_t *counts, double *sum_delta_p, double *sum_delta_p2,
+                            double *sum_delta_p_norm, double *sum_delta_p_norm2);
+
 /* Replaces fill_energy_snapshots! (compute_dpp_full.jl:290-299): kinetic energy
  * in GeV of each selected particle at each snapshot save index (0-based),
  * written into out, Julia (k, out_cols) column-major, at column col0 + j. */
@@ -296,12 +308,14 @@ typedef struct pt_serve pt_serve;
 
 enum pt_serve_state { PT_REQ_QUEUED = 0, PT_REQ_RUNNING = 1, PT_REQ_DONE = 2, PT_REQ_RELEASED = 3 };
 
-/* Extra cadence output for serving: |p| (kg m/s, FP64) for D_pp. */
+/* Extra cadence outputs for serving, for D_pp: |p| (kg m/s, FP64), or the
+ * relative momentum |p|/p_ref - 1 (float32, p_ref = the request's out_p_ref). */
 #define PT_OUT_PMAG 3
+#define PT_OUT_DPREL 4
 
 typedef struct {
     int32_t field_id;        /* from pt_serve_add_field */
-    int32_t out_kind;        /* PT_OUT_MU (float32 cache) or PT_OUT_PMAG (float64 cache) */
+    int32_t out_kind;        /* PT_OUT_MU (float32), PT_OUT_PMAG (float64) or PT_OUT_DPREL (float32) cache */
     int64_t n;               /* particles */
     double dt_s, charge_C, mass_kg;
     int64_t total_steps;     /* n_integration_steps: steps si = 1..total_steps */
@@ -321,6 +335,7 @@ typedef struct {
     int32_t reserved0;
     int64_t inject_first_index;
     pt_inject_params inject;
+    double out_p_ref;         /* p_ref (kg m/s) of PT_OUT_DPREL */
 } pt_serve_request;
 
 /* Reserves one device arena of the budget plus 12.5 % (for best-fit

@@ -28,6 +28,7 @@ EXPORTS = (
     "pt_ens_create", "pt_ens_create_injected", "pt_inject", "pt_ens_destroy", "pt_ens_advance",
     "pt_ens_get_state", "pt_ens_last_timing", "pt_ens_sort",
     "pt_mu_from_phase", "pt_dmumu_accumulate", "pt_dpp_accumulate", "pt_dpp_accumulate_pmag",
+    "pt_dpp_accumulate_dprel",
     "pt_energy_snapshots", "pt_buffer_alloc", "pt_buffer_free", "pt_memcpy", "pt_host_alloc", "pt_host_free",
     "pt_serve_create", "pt_serve_destroy", "pt_serve_add_field", "pt_serve_submit", "pt_serve_step",
     "pt_serve_drain", "pt_serve_take_done", "pt_serve_poll", "pt_serve_fetch", "pt_serve_copy_cache", "pt_serve_release",

@@ -44,6 +44,7 @@ struct DevReq {
     int64_t n, total, stride, block0;
     Cadence cad;
     int32_t out_kind, periodic;
+    double p_ref;  // PT_OUT_DPREL: delta = |p| / p_ref - 1
     FieldView fv;
     SplitGeom g;
     Phys ph;
@@ -84,6 +85,7 @@ k_serve(const DevReq *__restrict__ reqs, int nreq) {
     void *out = R.out;
     const float qsign = R.ph.qsign;
     const double pout = R.ph.pout;
+    const double p_ref = R.p_ref;
     // The request's parameters as values: read through R, the loop's stores
     // (cache, death counter) could alias the table and force reloads each push.
     const FieldView fv = R.fv;
@@ -98,6 +100,9 @@ k_serve(const DevReq *__restrict__ reqs, int nreq) {
             if (!valid) return;
             if (kind == PT_OUT_MU) {
                 static_cast<float *>(out)[slot * n + p] = alv ? mu_f32(u[0], u[1], u[2], v[0], v[1], v[2], qsign) : NAN;
+            } else if (kind == PT_OUT_DPREL) {
+                static_cast<float *>(out)[slot * n + p] =
+                    alv ? static_cast<float>(sqrt(p2) * pout / p_ref - 1.0) : NAN;
             } else {
                 static_cast<double *>(out)[slot * n + p] = alv ? sqrt(p2) * pout : NAN;
             }
@@ -366,6 +371,7 @@ void fill_dev(pt_serve *srv, SReq &r) {
     D.total = r.d.total_steps;
     D.stride = r.d.save_stride;
     D.out_kind = r.d.out_kind;
+    D.p_ref = r.d.out_p_ref;
     D.periodic = r.d.periodic ? 1 : 0;
     D.fv.data = f->data;
     for (int c = 0; c < 3; ++c) {
@@ -621,8 +627,11 @@ int pt_serve_submit(pt_serve *srv, const pt_serve_request *req, int64_t *request
         PT_REQUIRE(std::isfinite(d.mass_kg) && d.mass_kg > 0, "pt_serve_submit: mass must be positive");
         PT_REQUIRE(std::isfinite(d.charge_C) && d.charge_C != 0, "pt_serve_submit: charge must be non-zero");
         PT_REQUIRE(d.total_steps >= 1 && d.save_stride >= 1, "pt_serve_submit: total_steps and save_stride must be >= 1");
-        PT_REQUIRE(d.out_kind == PT_OUT_NONE || d.out_kind == PT_OUT_MU || d.out_kind == PT_OUT_PMAG,
-                   "pt_serve_submit: out_kind must be PT_OUT_NONE, PT_OUT_MU or PT_OUT_PMAG");
+        PT_REQUIRE(d.out_kind == PT_OUT_NONE || d.out_kind == PT_OUT_MU || d.out_kind == PT_OUT_PMAG ||
+                       d.out_kind == PT_OUT_DPREL,
+                   "pt_serve_submit: out_kind must be PT_OUT_NONE, PT_OUT_MU, PT_OUT_PMAG or PT_OUT_DPREL");
+        if (d.out_kind == PT_OUT_DPREL)
+            PT_REQUIRE(std::isfinite(d.out_p_ref) && d.out_p_ref > 0, "pt_serve_submit: out_p_ref must be positive");
         PT_REQUIRE(d.inject_on_device == 0 || d.inject_on_device == 1, "pt_serve_submit: inject_on_device must be 0 or 1");
         const pt_field *f = srv->fields[static_cast<size_t>(d.field_id)];
         if (d.inject_on_device) {

@@ -477,11 +477,15 @@ __global__ void k_pmag(const float *__restrict__ mom, int64_t nsave, int64_t tot
     out[t] = pmag(mom, s, total, col);
 }
 
-// D_pp pairs over an FP64 |p| series [save][total]: the walk of k_dmumu
-// without bins.  inv_p0 = 1/p0 (the normalised sums differ from Delta p / p0
-// by at most an ulp per term).
+// D_pp pairs over a series [save][total]: the walk of k_dmumu without bins.
+// T = double, kDelta = false: the FP64 |p| series; inv_p0 = 1/p0 (the
+// normalised sums differ from Delta p / p0 by at most an ulp per term).
+// T = float, kDelta = true: a relative series delta = |p|/p_ref - 1; the
+// kernel sums d = delta(s+lag) - delta(s) (exact in FP64) and d^2 only, and the
+// host scales them by p_ref and p_ref/p0.
+template <class T, bool kDelta>
 __global__ void __launch_bounds__(kTPB)
-k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
+k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
       const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags, double inv_p0,
       double *__restrict__ part_d, long long *__restrict__ part_i, int nbx, int ngy,
       const int32_t *__restrict__ fin, int lane32) {
@@ -529,20 +533,22 @@ k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel
     }
     if (j < n_sel) {
         const int64_t col = pidx ? pidx[j] : j;
-        const double *row = pm + col;
-        // `pe[g]` = |p| at s + lag[g] (any value where `valid[g]` is false)
+        const T *row = pm + col;
+        // `pe[g]` = the series at s + lag[g] (any value where `valid[g]` is false)
         auto visit = [&](double pst, const double *pe, const bool *valid) {
             if (!isfinite(pst)) return;
 #pragma unroll
             for (int g = 0; g < kG; ++g) {
                 const bool ok = valid[g] && isfinite(pe[g]);
                 const double d = ok ? pe[g] - pst : 0.0;
-                const double dn = d * inv_p0;
                 cnt[g] += ok ? 1u : 0u;
                 s1[g] += d;
                 s2[g] = fma(d, d, s2[g]);
-                s3[g] += dn;
-                s4[g] = fma(dn, dn, s4[g]);
+                if constexpr (!kDelta) {
+                    const double dn = d * inv_p0;
+                    s3[g] += dn;
+                    s4[g] = fma(dn, dn, s4[g]);
+                }
             }
         };
         bool all[kG];
@@ -551,12 +557,13 @@ k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel
         int64_t s = 0;
         if (s_fin > 0) {
             // every pair valid: the count is the number of starts
-            const uint32_t lane_bytes = static_cast<uint32_t>(col * 8);
-            auto at = [&](const double *uniform_row) {
-                return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(uniform_row) + lane_bytes);
+            const uint32_t lane_bytes = static_cast<uint32_t>(col * static_cast<int64_t>(sizeof(T)));
+            auto at = [&](const T *uniform_row) {
+                return static_cast<double>(
+                    *reinterpret_cast<const T *>(reinterpret_cast<const char *>(uniform_row) + lane_bytes));
             };
             auto load = [&](int64_t t, double &a, double *b) {
-                const double *r = pm + t * total;
+                const T *r = pm + t * total;
                 a = at(r);
 #pragma unroll
                 for (int g = 0; g < kG; ++g) b[g] = at(r + off[g]);
@@ -565,11 +572,13 @@ k_dpp(const double *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel
 #pragma unroll
                 for (int g = 0; g < kG; ++g) {
                     const double d = pe[g] - pst;
-                    const double dn = d * inv_p0;
                     s1[g] += d;
                     s2[g] = fma(d, d, s2[g]);
-                    s3[g] += dn;
-                    s4[g] = fma(dn, dn, s4[g]);
+                    if constexpr (!kDelta) {
+                        const double dn = d * inv_p0;
+                        s3[g] += dn;
+                        s4[g] = fma(dn, dn, s4[g]);
+                    }
                 }
             };
             double a0, a1, a2, b0[kG], b1[kG], b2[kG];
@@ -997,11 +1006,14 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
 
 namespace pt {
 namespace {
-template <class Src>
+// Src = float momenta [save][3][total] (comps 3, |p| in FP64 first), double
+// |p| [save][total] (comps 1), or with kDelta a float relative series
+// |p|/p_ref - 1 [save][total] (comps 1).
+template <class Src, bool kDelta = false>
 int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave, int64_t total, int64_t n_sel,
                    const int64_t *particle_index, const int64_t *lag_steps, int64_t nlags, double p0,
                    int64_t *counts, double *sum_delta_p, double *sum_delta_p2, double *sum_delta_p_norm,
-                   double *sum_delta_p_norm2, int64_t comps) {
+                   double *sum_delta_p_norm2, int64_t comps, double p_ref = 0.0) {
     return guarded([&] {
         PT_REQUIRE(ctx && momenta && lag_steps && counts && sum_delta_p && sum_delta_p2 && sum_delta_p_norm &&
                        sum_delta_p_norm2,
@@ -1014,6 +1026,7 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
             for (int64_t j = 0; j < n_sel; ++j)
                 PT_REQUIRE(particle_index[j] >= 0 && particle_index[j] < total, "pt_dpp_accumulate: particle index out of range");
         PT_REQUIRE(std::isfinite(p0) && p0 > 0.0, "pt_dpp_accumulate: p0 must be finite and positive");
+        if (kDelta) PT_REQUIRE(std::isfinite(p_ref) && p_ref > 0.0, "pt_dpp_accumulate_dprel: p_ref must be finite and positive");
         if (n_sel == 0) return;
         PT_HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t st = ctx->stream;
@@ -1023,8 +1036,9 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         const int64_t *dlag = to_device(lags.data(), 0, lags.size(), hlag, st);
         // FP32 momenta: |p| once per (sample, particle) into an FP64 scratch
         // series, a chunk of particles at a time (at most ~2 GiB).
+        constexpr bool kMomenta = !kDelta && !std::is_same<Src, double>::value;
         int64_t chunk = n_sel;
-        if constexpr (!std::is_same<Src, double>::value) {
+        if constexpr (kMomenta) {
             const int64_t cap = std::max<int64_t>(kTPB, ((int64_t{1} << 31) / (8 * nsave)) / kTPB * kTPB);
             chunk = std::min(n_sel, cap);
         }
@@ -1040,12 +1054,13 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         std::vector<long long> hi(nl, 0), ci(nl);
         DevBuf hfin(ctx);
         auto *fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(std::max(chunk, n_sel)) * 4));
-        auto run = [&](const double *pm, int64_t row_len, int64_t nsel_c, const int64_t *idx) {
+        auto run = [&](const auto *pm, int64_t row_len, int64_t nsel_c, const int64_t *idx) {
+            using T = std::remove_cv_t<std::remove_pointer_t<decltype(pm)>>;
             const int nbx = static_cast<int>((nsel_c + kTPB - 1) / kTPB);
             launch_finite_prefix(pm, nsave, nsel_c, row_len, int64_t{1}, idx, fin, st);
             // the fast path's 32-bit lane offsets (bytes) cover every column
             const int lane32 = row_len < (int64_t{1} << 29) && nsave < (int64_t{1} << 31) ? 1 : 0;
-            k_dpp<<<tile_group_blocks(nbx, ngy), kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag,
+            k_dpp<T, kDelta><<<tile_group_blocks(nbx, ngy), kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag,
                                                                 static_cast<int>(nlags), 1.0 / p0, part_d, part_i,
                                                                 nbx, ngy, fin, lane32);
             PT_HIP_CHECK(hipGetLastError());
@@ -1058,7 +1073,7 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
             for (size_t q = 0; q < nl * 4; ++q) hd[q] += cd[q];
             for (size_t q = 0; q < nl; ++q) hi[q] += ci[q];
         };
-        if constexpr (std::is_same<Src, double>::value) {
+        if constexpr (!kMomenta) {
             run(dm, total, n_sel, didx);
         } else {
             auto *pm = static_cast<double *>(hpm.alloc(static_cast<size_t>(nsave * chunk) * 8));
@@ -1070,13 +1085,21 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
                 run(pm, nc, nc, nullptr);
             }
         }
+        const double rn = p_ref / p0;  // delta units -> Delta p / p0
         for (size_t li = 0; li < nl; ++li) {
             if (lags[li] >= nsave) continue;
             counts[li] += hi[li];
-            sum_delta_p[li] += hd[li * 4 + 0];
-            sum_delta_p2[li] += hd[li * 4 + 1];
-            sum_delta_p_norm[li] += hd[li * 4 + 2];
-            sum_delta_p_norm2[li] += hd[li * 4 + 3];
+            if constexpr (kDelta) {
+                sum_delta_p[li] += p_ref * hd[li * 4 + 0];
+                sum_delta_p2[li] += p_ref * p_ref * hd[li * 4 + 1];
+                sum_delta_p_norm[li] += rn * hd[li * 4 + 0];
+                sum_delta_p_norm2[li] += rn * rn * hd[li * 4 + 1];
+            } else {
+                sum_delta_p[li] += hd[li * 4 + 0];
+                sum_delta_p2[li] += hd[li * 4 + 1];
+                sum_delta_p_norm[li] += hd[li * 4 + 2];
+                sum_delta_p_norm2[li] += hd[li * 4 + 3];
+            }
         }
     });
 }
@@ -1102,6 +1125,15 @@ int pt_dpp_accumulate_pmag(pt_ctx *ctx, const double *pmag, int on_device, int64
                            double *sum_delta_p_norm2) {
     return pt::dpp_accumulate(ctx, pmag, on_device, nsave, total, n_sel, particle_index, lag_steps, nlags, p0,
                               counts, sum_delta_p, sum_delta_p2, sum_delta_p_norm, sum_delta_p_norm2, 1);
+}
+
+int pt_dpp_accumulate_dprel(pt_ctx *ctx, const float *dprel, int on_device, int64_t nsave, int64_t total,
+                            int64_t n_sel, const int64_t *particle_index, const int64_t *lag_steps, int64_t nlags,
+                            double p_ref, double p0, int64_t *counts, double *sum_delta_p, double *sum_delta_p2,
+                            double *sum_delta_p_norm, double *sum_delta_p_norm2) {
+    return pt::dpp_accumulate<float, true>(ctx, dprel, on_device, nsave, total, n_sel, particle_index, lag_steps,
+                                           nlags, p0, counts, sum_delta_p, sum_delta_p2, sum_delta_p_norm,
+                                           sum_delta_p_norm2, 1, p_ref);
 }
 
 int pt_energy_snapshots(pt_ctx *ctx, const float *momenta, int on_device, int64_t nsave,

@@ -28,7 +28,12 @@ from ._lib import check, ptr
 from .injection import C_LIGHT, M_P, Q_E, energy_to_gamma
 
 QUEUED, RUNNING, DONE, RELEASED = 0, 1, 2, 3
-OUT_NONE, OUT_MU, OUT_PMAG = 0, 2, 3
+OUT_NONE, OUT_MU, OUT_PMAG, OUT_DPREL = 0, 2, 3, 4
+
+
+def cache_dtype(out_kind):
+    """Element type of a request's cadence cache."""
+    return np.float64 if out_kind == OUT_PMAG else np.float32
 
 
 class ServeRequest(ctypes.Structure):
@@ -39,7 +44,7 @@ class ServeRequest(ctypes.Structure):
                 ("x", ctypes.POINTER(ctypes.c_double) * 3), ("p", ctypes.POINTER(ctypes.c_double) * 3),
                 ("phase", ctypes.POINTER(ctypes.c_double)), ("inject_on_device", ctypes.c_int32),
                 ("reserved0", ctypes.c_int32), ("inject_first_index", ctypes.c_int64),
-                ("inject", _lib.InjectParams)]
+                ("inject", _lib.InjectParams), ("out_p_ref", ctypes.c_double)]
 
 
 def _bind(L):
@@ -62,6 +67,9 @@ def _bind(L):
         "pt_serve_stats": [vp, i64p, i64p, ctypes.POINTER(ctypes.c_double)],
         "pt_dpp_accumulate_pmag": [vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int, i64, i64, i64, i64p, i64p,
                                    i64, ctypes.c_double, i64p, _lib._dp, _lib._dp, _lib._dp, _lib._dp],
+        "pt_dpp_accumulate_dprel": [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, i64, i64, i64, i64p, i64p,
+                                    i64, ctypes.c_double, ctypes.c_double, i64p, _lib._dp, _lib._dp, _lib._dp,
+                                    _lib._dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -86,7 +94,8 @@ class Server:
         return fid.value
 
     def submit(self, field_id, x=None, p=None, *, dt, total_steps, stride, out_kind=OUT_MU, charge=Q_E, mass=M_P,
-               periodic=True, mu0=None, phase=None, p_magnitude=None, n=None, inject=None, first_index=0) -> int:
+               periodic=True, mu0=None, phase=None, p_magnitude=None, n=None, inject=None, first_index=0,
+               p_ref=None) -> int:
         """Host state: x (and p, or mu0 + phase + p_magnitude for fixed-mu).
         Device state: n particles and `inject` (a _lib.InjectParams), drawn at
         admission as pt_inject draws particles first_index .. first_index+n-1."""
@@ -94,6 +103,8 @@ class Server:
         r.field_id, r.out_kind = int(field_id), int(out_kind)
         r.dt_s, r.charge_C, r.mass_kg = float(dt), float(charge), float(mass)
         r.total_steps, r.save_stride, r.periodic = int(total_steps), int(stride), int(bool(periodic))
+        if out_kind == OUT_DPREL:
+            r.out_p_ref = float(p_ref)
         if inject is not None:
             r.n, r.inject_on_device, r.inject_first_index, r.inject = int(n), 1, int(first_index), inject
             rid = ctypes.c_int64()
@@ -195,6 +206,20 @@ def dpp_on_device(ctx, cache_ptr, nsave, n, lag_steps, p0):
     return acc
 
 
+def dpp_dprel_on_device(ctx, cache_ptr, nsave, n, lag_steps, p_ref, p0):
+    """D_pp accumulators from a device relative cache |p|/p_ref - 1 [save][n] (float32)."""
+    from .dpp import new_accumulators
+    acc = new_accumulators(len(lag_steps))
+    lags = np.ascontiguousarray(lag_steps, dtype=np.int64)
+    L = _bind(_lib.load())
+    check(L.pt_dpp_accumulate_dprel(ctx.h, ctypes.cast(ctypes.c_void_p(cache_ptr), _lib._fp), 1, nsave, n, n, None,
+                                    ptr(lags, _lib._i64p), len(lags), float(p_ref), float(p0),
+                                    ptr(acc.counts, _lib._i64p), ptr(acc.sum_delta_p, _lib._dp),
+                                    ptr(acc.sum_delta_p2, _lib._dp), ptr(acc.sum_delta_p_norm, _lib._dp),
+                                    ptr(acc.sum_delta_p_norm2, _lib._dp)))
+    return acc
+
+
 def log_lag_steps(nsave, count):
     """Up to ``count`` log-spaced cadence offsets in [1, nsave-1], deduplicated."""
     if nsave < 2:
@@ -216,7 +241,7 @@ def make_stream(n_requests, n_fields, B0s, *, rate_hz=20.0, seed=7, e_range=(1.0
         dur = float(rng.uniform(*dur_range))
         fixed = bool(rng.random() < 0.5)
         mu0 = float(rng.uniform(-1, 1)) if fixed else None
-        kind = OUT_MU if rng.random() < 0.5 else OUT_PMAG
+        kind = OUT_MU if rng.random() < 0.5 else OUT_DPREL  # D_pp from the relative |p| cache
         gamma = float(energy_to_gamma(E))
         Omega0 = Q_E * B0s[fid] / (gamma * M_P)
         dt = (2 * math.pi / steps_per_gp) / Omega0
@@ -258,12 +283,13 @@ def submit_request(server, req, L, on_device=True):
     """Submit one stream request.  on_device: its initial state is drawn at
     admission (request_inject_params), so the host does no O(n) work per
     request; otherwise request_state draws it on the host."""
+    pref = req.gamma * M_P * C_LIGHT * math.sqrt(1 - 1 / req.gamma ** 2)  # the injected |p|
     if on_device:
         return server.submit(req.field_id, n=req.n, inject=request_inject_params(req), dt=req.dt,
-                             total_steps=req.total_steps, stride=req.stride, out_kind=req.out_kind)
+                             total_steps=req.total_steps, stride=req.stride, out_kind=req.out_kind, p_ref=pref)
     x, p, phase, pmag = request_state(req, L)
     return server.submit(req.field_id, x, p, dt=req.dt, total_steps=req.total_steps, stride=req.stride,
-                         out_kind=req.out_kind, mu0=req.mu0, phase=phase, p_magnitude=pmag)
+                         out_kind=req.out_kind, mu0=req.mu0, phase=phase, p_magnitude=pmag, p_ref=pref)
 
 
 def request_product(stats_ctx, cache_ptr, nsave, n, req):
@@ -274,7 +300,9 @@ def request_product(stats_ctx, cache_ptr, nsave, n, req):
         return None
     if req.out_kind == OUT_MU:
         return dmumu_on_device(stats_ctx, cache_ptr, nsave, n, lags, np.linspace(0.0, 1.0, 21))
-    pmag = req.gamma * M_P * C_LIGHT * math.sqrt(1 - 1 / req.gamma ** 2)
+    pmag = req.gamma * M_P * C_LIGHT * math.sqrt(1 - 1 / req.gamma ** 2)  # p0: every particle starts at it
+    if req.out_kind == OUT_DPREL:
+        return dpp_dprel_on_device(stats_ctx, cache_ptr, nsave, n, lags, pmag, pmag)
     return dpp_on_device(stats_ctx, cache_ptr, nsave, n, lags, pmag)
 
 
@@ -337,7 +365,7 @@ def run_stream(push_ctx, stats_ctx, fields, reqs, *, budget_bytes, steps_per_tic
                 res = fut.result()
                 lat.append(time.perf_counter() - t0 - rq.arrival * time_scale)
                 if keep_results:
-                    results[rq.i] = (res, srv.copy_cache(rid, np.float32 if rq.out_kind == OUT_MU else np.float64))
+                    results[rq.i] = (res, srv.copy_cache(rid, cache_dtype(rq.out_kind)))
                 srv.release(rid)
             if log and now - last_log >= 1.0:
                 last_log = now

@@ -400,6 +400,44 @@ def test_dpp_from_fp64_pmag_cache_matches_oracle(ctx):
         assert np.allclose(getattr(acc, k), oacc[k], rtol=1e-11, atol=1e-12 * np.max(np.abs(oacc[k]))), k
 
 
+def test_dpp_from_relative_cache(ctx):
+    """pt_dpp_accumulate_dprel over a float32 relative cache delta = |p|/p_ref - 1
+    (the serving path's PT_OUT_DPREL): counts and sums equal a float64 brute
+    force over the same delta series, Delta p = p_ref (delta(s+l) - delta(s)),
+    and agree with the oracle's D_pp on the FP32 momenta the series was made
+    from to the rounding of delta to float32."""
+    from particle_transport_amd import _lib, serving
+    rng = np.random.default_rng(11)
+    nsave, n = 300, 700
+    mom = (rng.normal(0, 1e-18, (nsave, 3, n)) + 5e-17).astype(np.float32)
+    mom[:, :, 5:9][rng.random((nsave, 3, 4)) < 0.05] = np.nan
+    pm = np.sqrt((mom.astype(np.float64) ** 2).sum(axis=1))
+    p_ref, p0 = float(np.nanmedian(pm)), float(pm[0, 0])
+    delta = np.ascontiguousarray((pm / p_ref - 1.0).astype(np.float32))
+    lags = list(range(1, 40, 3)) + [100, 299, 300, 450]
+    buf = _lib.DeviceBuffer.from_host(ctx, delta)
+    acc = serving.dpp_dprel_on_device(ctx, buf.ptr, nsave, n, lags, p_ref, p0)
+    buf.free()
+    for li, lag in enumerate(lags):
+        if lag >= nsave:  # the reference skips lags beyond the series
+            assert acc.counts[li] == 0 and acc.sum_delta_p2[li] == 0.0
+            continue
+        a, b = delta[:-lag].astype(np.float64), delta[lag:].astype(np.float64)
+        ok = np.isfinite(a) & np.isfinite(b)
+        d = (b - a)[ok]
+        assert acc.counts[li] == ok.sum()
+        r = p_ref / p0
+        for k, want in (("sum_delta_p", p_ref * d.sum()), ("sum_delta_p2", p_ref ** 2 * (d * d).sum()),
+                        ("sum_delta_p_norm", r * d.sum()), ("sum_delta_p_norm2", r * r * (d * d).sum())):
+            got = getattr(acc, k)[li]
+            assert abs(got - want) <= 1e-11 * abs(want) + 1e-13 * abs(p_ref ** 2 * (d * d).sum()), (k, lag)
+    oacc = oracle.new_dpp_accumulators(len(lags))
+    oracle.dpp(mom, lags, p0, oacc)
+    assert np.array_equal(acc.counts, oacc["counts"])
+    for k in ("sum_delta_p2", "sum_delta_p_norm2"):
+        assert np.allclose(getattr(acc, k), oacc[k], rtol=1e-6, atol=0), k
+
+
 @pytest.mark.parametrize("storage", ["f64_planar", "f32_planar"])
 def test_mu_from_phase_matches_oracle(ctx, storage):
     F, x, p, dt, g = make_case(n=200)

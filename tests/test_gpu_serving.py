@@ -43,7 +43,7 @@ def run_alone(ctx, fld, r, steps_per_tick=64, on_device=True):
     rid = serving.submit_request(srv, r, L_BOX, on_device=on_device)
     while srv.poll(rid) != serving.DONE:
         srv.step()
-    out = srv.copy_cache(rid, np.float32 if r.out_kind == serving.OUT_MU else np.float64)
+    out = srv.copy_cache(rid, serving.cache_dtype(r.out_kind))
     _, _, _, alive = srv.fetch(rid)
     srv.release(rid)
     srv.close()
@@ -71,7 +71,7 @@ def test_alone_equals_mix_and_single_campaign_pusher(setup, on_device):
         srv.step()
         for rid, r in ids.items():
             if r.i not in mixed and srv.poll(rid) == serving.DONE:
-                mixed[r.i] = (srv.copy_cache(rid, np.float32 if r.out_kind == serving.OUT_MU else np.float64),
+                mixed[r.i] = (srv.copy_cache(rid, serving.cache_dtype(r.out_kind)),
                               srv.fetch(rid)[3])
                 res_mix = serving.finish_request(srv, stats, rid, r)
                 if r.i == 0:
@@ -105,6 +105,31 @@ def test_alone_equals_mix_and_single_campaign_pusher(setup, on_device):
                         out_kind="mu", steps_per_launch=100)
         ens.close()
         assert np.array_equal(r.out_a, alone[R.i][0]), R.i
+
+
+def test_relative_momentum_cache_and_dpp(setup):
+    """A D_pp request with the relative cache (PT_OUT_DPREL) holds exactly
+    float32(|p|/p_ref - 1) of the same request's FP64 |p| cache, and its D_pp
+    agrees with the one from the |p| cache to the float32 rounding of delta."""
+    ctx, stats, fld, F = setup
+    A = req(0, 4000, 3e4, 301, serving.OUT_PMAG, fld.B0, seed=12)
+    B = req(0, 4000, 3e4, 301, serving.OUT_DPREL, fld.B0, seed=12)
+    pm, _ = run_alone(ctx, fld, A)
+    dr, _ = run_alone(ctx, fld, B)
+    p_ref = A.gamma * M_P * C_LIGHT * math.sqrt(1 - 1 / A.gamma ** 2)
+    want = (pm / p_ref - 1.0).astype(np.float32)
+    assert dr.dtype == np.float32 and dr.shape == pm.shape
+    assert np.array_equal(np.isnan(dr), np.isnan(want)) and np.array_equal(dr[~np.isnan(dr)], want[~np.isnan(want)])
+    from particle_transport_amd._lib import DeviceBuffer
+    lags = serving.log_lag_steps(pm.shape[0], 64)
+    b1, b2 = DeviceBuffer.from_host(stats, pm), DeviceBuffer.from_host(stats, dr)
+    a1 = serving.dpp_on_device(stats, b1.ptr, pm.shape[0], 4000, lags, p_ref)
+    a2 = serving.dpp_dprel_on_device(stats, b2.ptr, pm.shape[0], 4000, lags, p_ref, p_ref)
+    b1.free()
+    b2.free()
+    assert np.array_equal(a1.counts, a2.counts)
+    assert np.allclose(a2.sum_delta_p2, a1.sum_delta_p2, rtol=1e-5, atol=0)
+    assert np.allclose(a2.sum_delta_p_norm2, a1.sum_delta_p_norm2, rtol=1e-5, atol=0)
 
 
 def test_budget_admission_fifo(setup):
