@@ -255,7 +255,6 @@ class Arena {
         }
         insert(off, n);
     }
-    size_t largest() const { return by_len_.empty() ? 0 : by_len_.rbegin()->first; }
 
   private:
     void insert(size_t off, size_t len) {
