@@ -320,6 +320,19 @@ def stats_rates(ctx, F32, L, x, p, dt, log_fn, cpu=True):
     out["dpp"] = {"pair_visits_per_s": visits / el, "pair_visits": visits, "seconds": el,
                   "shape": f"{n} particles x {nsave} samples, {len(lags)} lags, FP32 momenta -> FP64 |p|"}
     log_fn(f"[bench] D_pp: {visits / el:.4g} pair visits/s")
+    # the same pairs over the serving path's relative float32 series |p|/p_ref - 1
+    from particle_transport_amd import serving
+    pm = np.sqrt((mom.to_host().astype(np.float64) ** 2).sum(axis=1))
+    rel = DeviceBuffer.from_host(ctx, np.ascontiguousarray((pm / p0 - 1.0).astype(np.float32)))
+    del pm
+    for rep in range(2):
+        t0 = time.perf_counter()
+        serving.dpp_dprel_on_device(ctx, rel.ptr, nsave, n, lags, p0, p0)
+        el = time.perf_counter() - t0
+    out["dpp_relative"] = {"pair_visits_per_s": visits / el, "pair_visits": visits, "seconds": el,
+                           "shape": f"{n} particles x {nsave} samples, {len(lags)} lags, float32 |p|/p_ref - 1"}
+    log_fn(f"[bench] D_pp (relative series): {visits / el:.4g} pair visits/s")
+    rel.free()
     mom.free()
     fld.close()
     return out
