@@ -66,10 +66,21 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
     // 1/gamma- from the hardware reciprocal square root of 1 + |U-|^2
     const float r = __builtin_amdgcn_rsqf(fmaf(pmx, pmx, fmaf(pmy, pmy, fmaf(pmz, pmz, 1.0f))));
     const float tx = bx * r, ty = by * r, tz = bz * r;
-    // s = 2t/(1+t^2): one Newton step on the hardware reciprocal (~0.5 ulp).
-    const float d = fmaf(tx, tx, fmaf(ty, ty, fmaf(tz, tz, 1.0f)));
-    const float r0 = __builtin_amdgcn_rcpf(d);
-    const float f = 2.0f * fmaf(r0, fmaf(-d, r0, 1.0f), r0);
+    // s = 2t/(1+t^2).  For d = t^2 <= 2^-7 (|t| <= 0.088: more than 35 pushes
+    // per local gyration) the series 2(1 - d + d^2 - d^3) is within 2d^4 <
+    // 8e-9 of 2/(1+d), below FP32 resolution, and needs no transcendental.  A
+    // lane with a larger t takes one Newton step on the hardware reciprocal
+    // (~0.5 ulp) instead; the branch is taken only by waves holding such a
+    // lane, and each lane's value depends on its own t alone.
+    const float d = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
+    float f = fmaf(d, fmaf(d, fmaf(d, -2.0f, 2.0f), -2.0f), 2.0f);
+    if (__builtin_expect(__ballot(d > 0x1p-7f) != 0, 0)) {
+        if (d > 0x1p-7f) {
+            const float d1 = d + 1.0f;
+            const float r0 = __builtin_amdgcn_rcpf(d1);
+            f = 2.0f * fmaf(r0, fmaf(-d1, r0, 1.0f), r0);
+        }
+    }
     const float sx = tx * f, sy = ty * f, sz = tz * f;
     // p' = p- + p- x t and p+ = p- + p' x s, two fused multiply-adds per component
     const float ppx = fmaf(pmy, tz, fmaf(-pmz, ty, pmx));
@@ -507,7 +518,7 @@ __host__ inline Cadence make_cadence(int64_t si0, int K, int64_t total_steps, in
 // All 64 lanes of the wave must run the loop (lanes without a particle with
 // alive = false): the coefficient reload is wave-cooperative (coop_reload)
 // and uses the wave's `lds` area.
-template <int STORE, class SaveFn, class DieFn>
+template <int STORE, bool CUBIC, class SaveFn, class DieFn>
 __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[3], double &P2, bool &alive,
                                          const FieldView &fv, const SplitGeom &g, const Phys &ph,
                                          const Cadence &cad, int periodic, ReloadLds *lds, SaveFn save, DieFn die) {
@@ -552,9 +563,17 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
         const bool renorm = (((si_mod4 + k) & 3) == 0) | (phase == 0);
         float inv_gn;
         boris_split(U[0], U[1], U[2], P2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn, renorm);
-        // all three offsets first; one rare branch for any cell change
+        // all three offsets first; one rare branch for any cell change.  With
+        // cubic cells (CUBIC: g.k equal on every axis) the step factor is one
+        // product, the same value each axis would compute.
+        if constexpr (CUBIC) {
+            const float w = g.k[0] * inv_gn;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) off[c] = fmaf(U[c], g.k[c] * inv_gn, off[c]);
+            for (int c = 0; c < 3; ++c) off[c] = fmaf(U[c], w, off[c]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) off[c] = fmaf(U[c], g.k[c] * inv_gn, off[c]);
+        }
         // every offset in [0, 1) <=> every bit pattern below that of 1.0f
         // (negative values, -0 and NaN compare above it and take the slow path)
         const uint32_t top = max(max(__float_as_uint(off[0]), __float_as_uint(off[1])), __float_as_uint(off[2]));

@@ -107,7 +107,7 @@ k_push_plain(SplitState s, uint8_t *__restrict__ alive_arr, const float *__restr
 // Five waves per SIMD (at most 102 VGPRs): the launch constants the compiler
 // then keeps in scratch are read only on the rare paths (coefficient reload,
 // cell change, save), never in the push itself.
-template <int STORE>
+template <int STORE, bool CUBIC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_push_fast(SplitState s, uint8_t *__restrict__ alive_arr, FieldView fv, SplitGeom g, Phys ph,
             int64_t n, Cadence cad, int out_kind, int out_f64, void *out_a, void *out_b, int periodic,
@@ -132,7 +132,7 @@ k_push_fast(SplitState s, uint8_t *__restrict__ alive_arr, FieldView fv, SplitGe
     }
     const double mr2 = static_cast<double>(ph.mr) * static_cast<double>(ph.mr);
     double P2 = s.P2[q] * mr2;
-    run_lane<STORE>(
+    run_lane<STORE, CUBIC>(
         cell, off, U, P2, alive, fv, g, ph, cad, periodic, &lds[threadIdx.x >> 6],
         [&](int64_t slot, bool alv, const int32_t *ce, const float *of, const float *u, double, const float *v) {
             if (valid) write_save(out_kind, out_f64, out_a, out_b, slot, n, col, alv, ce, of, u, v, g, ph);
@@ -163,6 +163,16 @@ void make_geom_phys(const pt_field *f, const PushArgs &a, SplitGeom &g, Phys &ph
     ph.qsign = a.q > 0 ? 1.0f : -1.0f;
     ph.mr = static_cast<float>(kMp / a.m);
     ph.pout = a.m * kC;
+}
+
+// One k_push_fast launch; cubic cells (the same c dt / dx on every axis)
+// take the variant that forms the position step factor once per push.
+template <int STORE, class... Args>
+void launch_fast(const SplitGeom &g, unsigned blocks, hipStream_t stream, Args... args) {
+    if (g.k[0] == g.k[1] && g.k[0] == g.k[2])
+        k_push_fast<STORE, true><<<blocks, 256, 0, stream>>>(args...);
+    else
+        k_push_fast<STORE, false><<<blocks, 256, 0, stream>>>(args...);
 }
 
 }  // namespace
@@ -207,26 +217,16 @@ int64_t launch_push_fast(pt_ens *e, const pt_field *f, const PushArgs &a) {
         const int64_t si0 = a.first_step + done;
         const Cadence cad = make_cadence(si0, k, a.total_steps, a.stride, a.out_kind != PT_OUT_NONE, a.save_base);
         switch (f->storage) {
-            case PT_STORE_F32_PLANAR:
-                k_push_fast<PT_STORE_F32_PLANAR><<<blocks, 256, 0, e->ctx->stream>>>(
-                    split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
-                    e->periodic, a.death_base, dl, e->perm);
-                break;
-            case PT_STORE_F32_BRICK:
-                k_push_fast<PT_STORE_F32_BRICK><<<blocks, 256, 0, e->ctx->stream>>>(
-                    split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
-                    e->periodic, a.death_base, dl, e->perm);
-                break;
-            case PT_STORE_BF16_BRICK:
-                k_push_fast<PT_STORE_BF16_BRICK><<<blocks, 256, 0, e->ctx->stream>>>(
-                    split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
-                    e->periodic, a.death_base, dl, e->perm);
-                break;
-            case PT_STORE_F16_BRICK:
-                k_push_fast<PT_STORE_F16_BRICK><<<blocks, 256, 0, e->ctx->stream>>>(
-                    split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, a.out_f64, a.out_a, a.out_b,
-                    e->periodic, a.death_base, dl, e->perm);
-                break;
+#define PT_LAUNCH_FAST(ST)                                                                                  \
+    case ST:                                                                                                \
+        launch_fast<ST>(g, blocks, e->ctx->stream, split_state(e), e->alive, fv, g, ph, e->n, cad, a.out_kind, \
+                        a.out_f64, a.out_a, a.out_b, e->periodic, a.death_base, dl, e->perm);               \
+        break;
+            PT_LAUNCH_FAST(PT_STORE_F32_PLANAR)
+            PT_LAUNCH_FAST(PT_STORE_F32_BRICK)
+            PT_LAUNCH_FAST(PT_STORE_BF16_BRICK)
+            PT_LAUNCH_FAST(PT_STORE_F16_BRICK)
+#undef PT_LAUNCH_FAST
             default:
                 throw Error{PT_ERR_INVALID, "PT_PUSH_FAST: unsupported field storage"};
         }

@@ -94,7 +94,7 @@ k_serve(const DevReq *__restrict__ reqs, int nreq) {
     const Cadence cad = R.cad;
     const int periodic = R.periodic;
     unsigned long long *const deaths = R.deaths;
-    run_lane<STORE>(
+    run_lane<STORE, false>(
         cell, off, U, P2, alive, fv, g, ph, cad, periodic, &lds[threadIdx.x >> 6],
         [&](int64_t slot, bool alv, const int32_t *, const float *, const float *u, double p2, const float *v) {
             if (!valid) return;

@@ -494,6 +494,75 @@ def test_fp32_momentum_magnitude_conserved_in_uniform_B(ctx, pusher, storage):
     assert np.allclose(ps[2], pz, rtol=1e-6)
 
 
+@pytest.mark.parametrize("shape", [(9, 9, 9), (9, 17, 5)])
+def test_rotation_angle_at_any_pushes_per_gyration(ctx, shape):
+    """Uniform B, u = 0: every push turns p by 2*atan(|t|) with t = q B dt / (2 m gamma)
+    and keeps |p|.  Lorentz factors from 1.2 to 60 at one dt give 4 to 200
+    pushes per gyration, so lanes with t^2 on both sides of 2^-7 (where the
+    pusher switches from the series for 2t/(1+t^2) to the reciprocal) share
+    every wave.  The (9, 17, 5) grid has a different cell size on each axis
+    (the general position step), the (9, 9, 9) grid cubic cells."""
+    n, n_run = 2048, 2000
+    Bz = 5e-10
+    F = [np.asfortranarray(np.full(shape, v)) for v in (0.0, 0.0, Bz, 0.0, 0.0, 0.0)]
+    rng = np.random.default_rng(4)
+    x = rng.random((3, n)) * L_BOX
+    gamma = rng.uniform(1.2, 60.0, n)
+    pm = M_P * C_LIGHT * np.sqrt(gamma ** 2 - 1)
+    ang0 = rng.random(n) * 2 * math.pi
+    pz = pm * rng.uniform(-0.9, 0.9, n)
+    pperp = np.sqrt(pm ** 2 - pz ** 2)
+    p = np.stack([pperp * np.cos(ang0), pperp * np.sin(ang0), pz])
+    dt = 2 * math.pi * 60.0 * M_P / (Q_E * Bz) / 200
+    t = Q_E * Bz * dt / (2 * M_P * gamma)
+    assert (t ** 2 > 2.0 ** -7).mean() > 0.1 and (t ** 2 < 2.0 ** -7).mean() > 0.5
+    fld = Field(ctx, F, L_BOX, "f32_brick")
+    ens = Ensemble(ctx, fld, x, p, pusher="fast")
+    ens.advance(dt=dt, first_step=1, n_steps=n_run, total_steps=n_run + 1, stride=1, steps_per_launch=500)
+    xs, ps, alive = ens.state()
+    ens.close()
+    fld.close()
+    assert alive.all()
+    assert np.max(np.abs(np.linalg.norm(ps, axis=0) / pm - 1)) <= 2e-7
+    theta = 2 * np.arctan(t)
+    expected = ang0 - n_run * theta
+    dphi = (np.arctan2(ps[1], ps[0]) - expected + math.pi) % (2 * math.pi) - math.pi
+    # FP32 t carries a relative error of a few 1e-7 (rsq, folded factors);
+    # the series without its fallback would be off by 13 % at t^2 = 0.6
+    assert np.all(np.abs(dphi) <= n_run * theta * 1e-6 + 1e-6)
+    assert np.allclose(ps[2], pz, rtol=1e-6)
+    # the guiding centre drifts along z at v_par, which the oracle fixes too
+    xo, po = x.copy(), p.copy()
+    g = oracle.make_geom(*shape, L_BOX)
+    oracle.run(F, g, xo, po, dt, 1, n_run, n_run + 1, 1, True, 0)
+    W = g.xmax
+    d = (xs - xo + W / 2) % W - W / 2
+    assert np.max(np.abs(d[2]) / g.dz) <= 1e-3
+
+
+def test_fast_pusher_on_non_cubic_cells_tracks_oracle(ctx):
+    """A turbulent field on a 33 x 17 x 25 grid (three cell sizes): the
+    production pusher's general position step vs the FP64 oracle, to the
+    same bounds as on cubic cells."""
+    Fc, x, p, dt, _ = make_case(n=256)
+    F = [np.asfortranarray(a[:, :17, :25]) for a in Fc]
+    g = oracle.make_geom(33, 17, 25, L_BOX)
+    steps = 1000
+    fld = Field(ctx, F, L_BOX, "f32_brick")
+    ens = Ensemble(ctx, fld, x, p, pusher="fast")
+    ens.advance(dt=dt, first_step=1, n_steps=steps, total_steps=steps + 1, stride=1, steps_per_launch=128)
+    xs, ps, alive = ens.state()
+    ens.close()
+    fld.close()
+    xo, po = x.copy(), p.copy()
+    oracle.run(F, g, xo, po, dt, 1, steps, steps + 1, 1, True, 0)
+    W = np.array([[g.xmax], [g.ymax], [g.zmax]])
+    d = (xs - xo + W / 2) % W - W / 2
+    assert np.max(np.abs(d) / np.array([[g.dx], [g.dy], [g.dz]])) <= 1e-4
+    assert np.max(np.abs(ps - po) / np.linalg.norm(po, axis=0)) <= 1e-4
+    assert alive.all()
+
+
 @pytest.mark.parametrize("periodic", [True, False])
 def test_morton_sort_leaves_results_bit_identical(ctx, periodic):
     """Re-sorting changes which lane pushes which particle, never the
