@@ -50,7 +50,9 @@ struct DevReq {
     Phys ph;
 };
 
-template <int STORE>
+// CUBIC: every request of the launch has cubic cells (the same c dt / dx on
+// every axis), so the position step factor is formed once per push.
+template <int STORE, bool CUBIC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_serve(const DevReq *__restrict__ reqs, int nreq) {
     const int64_t b = blockIdx.x;
@@ -94,7 +96,7 @@ k_serve(const DevReq *__restrict__ reqs, int nreq) {
     const Cadence cad = R.cad;
     const int periodic = R.periodic;
     unsigned long long *const deaths = R.deaths;
-    run_lane<STORE, false>(
+    run_lane<STORE, CUBIC>(
         cell, off, U, P2, alive, fv, g, ph, cad, periodic, &lds[threadIdx.x >> 6],
         [&](int64_t slot, bool alv, const int32_t *, const float *, const float *u, double p2, const float *v) {
             if (!valid) return;
@@ -480,8 +482,12 @@ void complete_all(pt_serve *srv) {
 }
 
 template <int STORE>
-void launch_tick(pt_serve *srv, const DevReq *table, int nreq, int64_t blocks) {
-    k_serve<STORE><<<static_cast<unsigned>(blocks), 256, 0, srv->ctx->stream>>>(table, nreq);
+void launch_tick(pt_serve *srv, const DevReq *table, int nreq, int64_t blocks, bool cubic) {
+    const unsigned grid = static_cast<unsigned>(blocks);
+    if (cubic)
+        k_serve<STORE, true><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
+    else
+        k_serve<STORE, false><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
     PT_HIP_CHECK(hipGetLastError());
 }
 
@@ -508,9 +514,12 @@ void enqueue_tick(pt_serve *srv) {
         t.cap = cap;
     }
     int64_t blocks = 0;
+    bool cubic = true;
     t.work.clear();
     for (int i = 0; i < nreq; ++i) {
         SReq &r = *work[static_cast<size_t>(i)];
+        const float *k = r.dev.g.k;
+        cubic = cubic && k[0] == k[1] && k[0] == k[2];
         const int64_t K = std::min<int64_t>(srv->steps_per_tick, r.d.total_steps - r.si_planned + 1);
         r.dev.cad = make_cadence(r.si_planned, static_cast<int>(K), r.d.total_steps, r.d.save_stride,
                                  r.d.out_kind != PT_OUT_NONE, 0);
@@ -525,9 +534,9 @@ void enqueue_tick(pt_serve *srv) {
     PT_HIP_CHECK(hipEventRecord(t.ev0, st));
     PT_HIP_CHECK(hipMemcpyAsync(t.d_table, t.h_table, sizeof(DevReq) * nreq, hipMemcpyHostToDevice, st));
     switch (srv->storage) {
-        case PT_STORE_F32_BRICK: launch_tick<PT_STORE_F32_BRICK>(srv, t.d_table, nreq, blocks); break;
-        case PT_STORE_BF16_BRICK: launch_tick<PT_STORE_BF16_BRICK>(srv, t.d_table, nreq, blocks); break;
-        case PT_STORE_F16_BRICK: launch_tick<PT_STORE_F16_BRICK>(srv, t.d_table, nreq, blocks); break;
+        case PT_STORE_F32_BRICK: launch_tick<PT_STORE_F32_BRICK>(srv, t.d_table, nreq, blocks, cubic); break;
+        case PT_STORE_BF16_BRICK: launch_tick<PT_STORE_BF16_BRICK>(srv, t.d_table, nreq, blocks, cubic); break;
+        case PT_STORE_F16_BRICK: launch_tick<PT_STORE_F16_BRICK>(srv, t.d_table, nreq, blocks, cubic); break;
         default: throw Error{PT_ERR_STATE, "server storage not set"};
     }
     PT_HIP_CHECK(hipEventRecord(t.ev1, st));

@@ -107,6 +107,23 @@ def test_alone_equals_mix_and_single_campaign_pusher(setup, on_device):
         assert np.array_equal(r.out_a, alone[R.i][0]), R.i
 
 
+def test_non_cubic_grid_equals_single_campaign_pusher(setup):
+    """A field with three cell sizes (33 x 17 x 25 nodes) takes the general
+    position step in both the server and the single-campaign pusher: the
+    request's mu cache equals the pusher's bit for bit."""
+    ctx, stats, _, F = setup
+    fld = Field(ctx, [np.asfortranarray(a[:, :17, :25]) for a in F], L_BOX, "bf16_brick")
+    R = req(0, 3000, 1e4, 301, serving.OUT_MU, fld.B0, seed=5)
+    out, alive = run_alone(ctx, fld, R)
+    ens = Ensemble.injected(ctx, fld, R.n, serving.request_inject_params(R), pusher="fast")
+    r = ens.advance(dt=R.dt, first_step=1, n_steps=R.total_steps, total_steps=R.total_steps, stride=R.stride,
+                    out_kind="mu", steps_per_launch=100)
+    ens.close()
+    fld.close()
+    assert np.isfinite(out).any()
+    assert np.array_equal(r.out_a, out, equal_nan=True)
+
+
 def test_relative_momentum_cache_and_dpp(setup):
     """A D_pp request with the relative cache (PT_OUT_DPREL) holds exactly
     float32(|p|/p_ref - 1) of the same request's FP64 |p| cache, and its D_pp
