@@ -25,8 +25,9 @@ Beside the headline number the JSON line carries:
                 particle-step 48*s_f + (36+36)/K + 4/20, K pushes per launch) from
                 HIP-event launch durations on the library's stream, plus the
                 FP32 VALU fraction;
-  cpu_baseline  the CPU oracle (C restatement of the reference pusher, FP64,
-                one core) on a bounded sample of the same workload.
+  cpu_baseline  the CPU oracle (C restatement of the reference pusher, FP64)
+                on a bounded sample of the same workload in 16 host threads,
+                with a one-core figure beside it.
 """
 from __future__ import annotations
 
@@ -548,8 +549,9 @@ def main():
                          "--seed 2024 --dB 1.0 --kmax 64")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the pusher on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print("bench.py measures one GPU: the pusher runs on a single MI355X; particles would shard "
+              "across GPUs with replicated fields and one reduction of the pair sums, which is not run here.",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

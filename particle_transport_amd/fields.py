@@ -2,8 +2,9 @@
 
 ``load_static_fields`` / ``build_uniform_coords`` / ``estimate_min_dx`` restate
 phase_space_gpu_runner.jl:95-134.  The reference reads HDF5 datasets named by
-``B_paths``/``v_paths``; h5py is not part of this build, so the loader reads
-the same dataset names from a ``.npz`` archive (or takes arrays directly).
+``B_paths``/``v_paths``; the loader reads them from an HDF5 file through
+libpt_h5 (h5py is not installed), from a ``.npz`` archive with the same names,
+or takes arrays directly.
 
 ``generate_turbulence`` writes the synthetic inputs the benchmark configs
 describe (the reference ships no generator): B = B0 z + dB and u = du, each
@@ -58,13 +59,28 @@ def load_static_fields(cfg, dtype=np.float64, arrays=None):
     Fortran-ordered (nx,ny,nz) arrays in SI units (tesla, m/s).
 
     ``arrays`` may hold the six raw datasets (microgauss, velocity units);
-    otherwise ``cfg["file"]`` names a .npz archive with the B_paths/v_paths
-    keys, each stored as a C-order (nz,ny,nx) array (the HDF5 on-disk order of
-    a Julia (nx,ny,nz) dataset)."""
+    otherwise ``cfg["file"]`` names an HDF5 file (``.h5``/``.hdf5``) or a .npz
+    archive with the B_paths/v_paths datasets, each a Julia (nx,ny,nz) array,
+    i.e. the C-order (nz,ny,nx) dataspace HDF5 stores.  ``field_subset``
+    (phase_space_gpu_runner.jl:100-104) keeps the first (nx,ny,nz) nodes; an
+    HDF5 file is read as that box only."""
     subset = cfg.get("field_subset")
     bp = tuple(cfg.get("B_paths", B_PATHS))
     vp = tuple(cfg.get("v_paths", V_PATHS))
-    if arrays is None:
+    if arrays is None and str(cfg["file"]).lower().endswith((".h5", ".hdf5")):
+        from . import h5
+        raw = []
+        with h5.H5File(str(cfg["file"]), "r") as f:
+            for k in (*bp, *vp):
+                ds = f.dataset(k)
+                if ds.ndim != 3:
+                    raise ValueError(f"{k}: expected a 3-D field dataset, got shape {ds.shape}")
+                count = tuple(ds.shape)
+                if subset:
+                    nx, ny, nz = subset
+                    count = (min(nz, count[0]), min(ny, count[1]), min(nx, count[2]))
+                raw.append(ds.read_box((0, 0, 0), count).transpose(2, 1, 0))
+    elif arrays is None:
         with np.load(cfg["file"], allow_pickle=False) as z:
             raw = [np.asarray(z[k]).transpose(2, 1, 0) for k in (*bp, *vp)]
     else:

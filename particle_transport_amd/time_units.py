@@ -446,8 +446,8 @@ def configured_lag_bounds(cfg, cache):
     return cmin, cmax, lag_min, lag_max, math.nan, math.nan
 
 
-# time_units.jl:301-315 (plus the log-spaced mode of the BASELINE lag grids,
-# which maps through the same nearest/dedupe/sort steps below)
+# time_units.jl:301-315 (plus a log-spaced mode for lag grids such as 64
+# log-spaced lags, which maps through the same nearest/dedupe/sort steps below)
 def build_requested_lags_from_bounds(cfg, lag_min, lag_max):
     mode = _lag_mode(cfg)
     lo, hi = float(lag_min), float(lag_max)

@@ -326,3 +326,32 @@ def test_float32_time_axes_fail_the_reference_consistency_check(tmp_path):
     r = cache_h5.validate_mu_cache(path)
     assert not r.valid and r.reason == "inconsistent_time_units"
     assert "t_norm and t_gyroperiods are inconsistent" in r.message
+
+
+@needs_tools
+def test_field_file_loader_reads_julia_arrays(tmp_path):
+    """load_static_fields from an HDF5 field file (phase_space_gpu_runner.jl:95-123):
+    each dataset is a Julia (nx, ny, nz) array, which HDF5 stores as the
+    dataspace (nz, ny, nx).  h5dump fixes where one node sits in the file; the
+    loader must return it at [i, j, k] in SI units, and field_subset must keep
+    the first nodes of each axis."""
+    from particle_transport_amd import fields
+    nx, ny, nz = 5, 4, 3
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    jl = [np.asfortranarray(1000.0 * c + 100 * i + 10 * j + k) for c in range(6)]
+    path = str(tmp_path / "field.h5")
+    with h5.H5File(path, "w") as f:
+        for name, a in zip(fields.B_PATHS + fields.V_PATHS, jl):
+            f[name] = np.ascontiguousarray(a.T)  # numpy (nz, ny, nx) = Julia (nx, ny, nz)
+    out = dump("-d", "j_velocity", "-s", "2,1,3", "-c", "1,1,1", path)
+    assert "DATASPACE  SIMPLE { ( 3, 4, 5 ) / ( 3, 4, 5 ) }" in out
+    assert float(out.split("(2,1,3):")[1].split("\n")[0]) == 4000.0 + 100 * 3 + 10 * 1 + 2
+    cfg = dict(file=path, velocity_unit_in_m_per_s=1e3)
+    got = fields.load_static_fields(cfg)
+    want = fields.load_static_fields(cfg, arrays=jl)
+    for g, w, c in zip(got, want, range(6)):
+        assert g.shape == (nx, ny, nz) and g.flags.f_contiguous
+        assert np.array_equal(g, w)
+        assert g[3, 1, 2] == (1000.0 * c + 312) * (1e-10 if c < 3 else 1e3)
+    sub = fields.load_static_fields(dict(cfg, field_subset=(3, 2, 2)))
+    assert all(np.array_equal(s, w[:3, :2, :2]) for s, w in zip(sub, want))
