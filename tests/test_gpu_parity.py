@@ -658,3 +658,41 @@ def test_dmumu_offsets_beyond_32_bits(ctx):
             _check_dmumu(acc, oacc)
     finally:
         dev.free()
+
+
+# The motional field E = -u x B of every pusher (phase_space_gpu_runner.jl:365-368),
+# checked as the oracle is (tests/test_oracle.py): in B = B0 z with a uniform
+# flow u = u0 x, the guiding centre drifts at +u0 x and a particle at rest in
+# the flow frame keeps v = u0 x.  A flipped sign or swapped operands drift at -u0.
+@pytest.mark.parametrize("pusher,storage", [("ref_f64", "f64_planar"), ("plain_f32", "f32_planar"),
+                                            ("fast", "f32_planar"), ("fast", "f32_brick"),
+                                            ("fast", "f16_brick"), ("fast", "bf16_brick")])
+def test_motional_field_drifts_with_the_flow(ctx, pusher, storage):
+    B0, u0, gamma = 5e-10, 3.0e6, 1.0001
+    n_nodes, Lb = 5, 1.0e12
+    F = [np.full((n_nodes,) * 3, v, order="F") for v in (0.0, 0.0, B0, u0, 0.0, 0.0)]
+    vmag = C_LIGHT * math.sqrt(1 - 1 / gamma ** 2)
+    g1 = 1 / math.sqrt(1 - (u0 / C_LIGHT) ** 2)
+    x = np.array([[0.4 * Lb, 0.4 * Lb], [0.5 * Lb, 0.5 * Lb], [0.5 * Lb, 0.5 * Lb]])
+    p = np.array([[0.0, g1 * M_P * u0], [gamma * M_P * vmag, 0.0], [0.0, 0.0]])
+    dt = 2 * math.pi * gamma * M_P / (Q_E * B0) / 100
+    nsteps, stride = 4001, 10  # 40 gyrations
+    fld = Field(ctx, F, Lb, storage)
+    ens = Ensemble(ctx, fld, x, p, pusher=pusher, periodic=False)
+    r = ens.advance(dt=dt, first_step=1, n_steps=nsteps, total_steps=nsteps, stride=stride, out_kind="phase",
+                    out_dtype=np.float64, steps_per_launch=200)
+    ens.close()
+    fld.close()
+    pos = r.out_a  # (nsave, 3, 2)
+    assert np.isfinite(pos).all() and list(r.alive_counts[-1:]) == [2]
+    t = np.arange(pos.shape[0]) * stride * dt
+    assert np.polyfit(t, pos[:, 0, 0], 1)[0] == pytest.approx(u0, rel=0.02)
+    assert abs(np.polyfit(t, pos[:, 1, 0], 1)[0]) <= 0.02 * u0
+    # at rest in the flow frame: the straight line x0 + u0 t.  FP64 keeps it to
+    # the Boris step's O((u0/c)^2) balance (1e-6 of the path); the FP32
+    # pushers' split position resolves 6e-8 of a cell (1.5e4 m, 1e-6 of the
+    # path) per push.  A flipped E gyrates with radius 1.6 % of the path.
+    path = u0 * t[-1]
+    tol = 1e-6 if pusher == "ref_f64" else 2e-4
+    assert np.max(np.abs(pos[:, 0, 1] - (0.4 * Lb + u0 * t))) <= tol * path
+    assert np.max(np.abs(pos[:, 1, 1] - 0.5 * Lb)) <= tol * path

@@ -117,6 +117,99 @@ def test_run_cadence_and_kill():
     assert np.isnan(o["pos"][2, 0, 0])  # save at si = 11 is NaN (dead)
 
 
+# E = -u x B inside the step loop (phase_space_gpu_runner.jl:365-368): pinned
+# through oracle.run with a uniform flow, not through boris_push's E argument.
+# In B = B0 z and u = u0 x the motional field is E = -u x B = u0 B0 y, whose
+# E x B drift E x B / B^2 = +u0 x carries plasma and particles along with the
+# flow.  A flipped sign, or the operands swapped (B x u), drifts at -u0 x.
+DRIFT_B0, DRIFT_U0 = 5e-10, 3.0e6
+
+
+def drift_case(n_gyr=40, steps_per_gyr=100, gamma=1.0001, stride=10):
+    n = 5
+    Lb = 1.0e12
+    g = oracle.make_geom(n, n, n, Lb)
+    F = uniform_fields(n, B=(0.0, 0.0, DRIFT_B0), u=(DRIFT_U0, 0.0, 0.0))
+    vmag = C_LIGHT * math.sqrt(1 - 1 / gamma ** 2)
+    # particle 0: gyrating (p perpendicular to B); particle 1: at rest in the
+    # flow frame, v = u0 x, where E + v x B = 0 and the Boris step applies no force
+    g1 = 1 / math.sqrt(1 - (DRIFT_U0 / C_LIGHT) ** 2)
+    x = np.array([[0.4 * Lb, 0.4 * Lb], [0.5 * Lb, 0.5 * Lb], [0.5 * Lb, 0.5 * Lb]])
+    p = np.array([[0.0, g1 * M_P * DRIFT_U0], [gamma * M_P * vmag, 0.0], [0.0, 0.0]])
+    T_g = 2 * math.pi * gamma * M_P / (Q_E * DRIFT_B0)
+    dt = T_g / steps_per_gyr
+    nsteps = n_gyr * steps_per_gyr + 1
+    return F, g, x, p, dt, nsteps, stride
+
+
+def test_run_motional_field_drifts_with_the_flow():
+    """Uniform B = B0 z, uniform u = u0 x through oracle.run: the gyrating
+    particle's guiding centre (least-squares line through every cached
+    position over 40 gyrations) moves at +u0 x within 2 %, with no y drift;
+    the particle that starts at v = u0 x keeps that velocity (no net force)."""
+    F, g, x, p, dt, nsteps, stride = drift_case()
+    o = oracle.run(F, g, x, p, dt, 1, nsteps, nsteps, stride, periodic=False, out_kind=1)
+    t = np.arange(o["nsaved"]) * stride * dt
+    pos = o["pos"]  # (nsave, 3, n)
+    vx = np.polyfit(t, pos[:, 0, 0], 1)[0]
+    vy = np.polyfit(t, pos[:, 1, 0], 1)[0]
+    assert vx == pytest.approx(DRIFT_U0, rel=0.02)
+    assert abs(vy) <= 0.02 * DRIFT_U0
+    # at rest in the flow frame: a straight line at u0 with momentum kept.
+    # The relativistic Boris step balances E + v x B only to O((u0/c)^2 dt)
+    # per step (Vay 2008), ~5e-8 of the path here; a flipped E would instead
+    # gyrate at 2 u0 with radius 2 u0 / Omega = 1.6 % of the path.
+    path = DRIFT_U0 * t[-1]
+    assert np.max(np.abs(pos[:, 0, 1] - (0.4 * g.xmax + DRIFT_U0 * t))) <= 1e-6 * path
+    assert np.max(np.abs(pos[:, 1, 1] - 0.5 * g.xmax)) <= 1e-6 * path
+    assert np.max(np.abs(o["mom"][:, 0, 1] / p[0, 1] - 1.0)) <= 1e-6
+
+
+def test_run_drift_check_fails_for_a_flipped_motional_field():
+    """The same check fed E = +u x B (by reversing u) drifts at -u0 x: the
+    drift test above separates the two signs."""
+    F, g, x, p, dt, nsteps, stride = drift_case()
+    F[3] = -F[3]
+    o = oracle.run(F, g, x, p, dt, 1, nsteps, nsteps, stride, periodic=False, out_kind=1)
+    t = np.arange(o["nsaved"]) * stride * dt
+    vx = np.polyfit(t, o["pos"][:, 0, 0], 1)[0]
+    assert vx == pytest.approx(-DRIFT_U0, rel=0.02)
+
+
+def test_mu_cache_is_mu_of_the_pre_push_phase_cache():
+    """out_kind=2 (reconstruct_mu_step_kernel! at save steps,
+    multimode_multienergy_cache_pipeline.jl:1349-1374) equals
+    reconstruct_mu_chunk (compute_delta_mu2_curve.jl:112-149) applied to the
+    same run's out_kind=1 cache, which holds the state before each save
+    step's push (test_run_cadence_and_kill pins that); the mu of the
+    post-push state differs."""
+    from particle_transport_amd import fields
+    n_nodes, n = 9, 64
+    Lb = fields.box_length_m(10.0)
+    raw = fields.generate_turbulence(n_nodes, seed=4, kmax=3, dB_over_B0=1.0)["total"]
+    F = [np.asfortranarray(a) for a in fields.to_si(raw)]
+    g = oracle.make_geom(n_nodes, n_nodes, n_nodes, Lb)
+    rng = np.random.default_rng(9)
+    x = rng.random((3, n)) * Lb
+    d = rng.standard_normal((3, n))
+    d /= np.linalg.norm(d, axis=0)
+    p = d * 1e3 * M_P * C_LIGHT  # gamma ~ 1000: about a cell per 10 pushes at this dt
+    dt = 2 * math.pi * 1e3 * M_P / (Q_E * 5e-10) / 20
+    total, stride = 201, 10
+    o1 = oracle.run(F, g, x.copy(), p.copy(), dt, 1, total, total, stride, True, 1)
+    o2 = oracle.run(F, g, x.copy(), p.copy(), dt, 1, total, total, stride, True, 2)
+    mu_phase = oracle.reconstruct_mu_chunk(o1["pos"], o1["mom"], F[:3], g)  # (n, nsave)
+    assert o2["mu"].shape == (o1["nsaved"], n)
+    assert np.array_equal(o2["mu"], mu_phase.T)
+    # the state after one more push gives other values
+    o3 = oracle.run(F, g, x.copy(), p.copy(), dt, 1, total, total, stride, True, 1)
+    xa, pa = x.copy(), p.copy()
+    oracle.run(F, g, xa, pa, dt, 1, 1, total, 1, True, 0)
+    post = oracle.reconstruct_mu_chunk(xa[None], pa[None], F[:3], g)[:, 0]
+    assert not np.allclose(post, o2["mu"][0], rtol=0, atol=1e-9)
+    assert np.array_equal(o3["pos"], o1["pos"])
+
+
 # ----------------------------------------------------------------------- mu
 
 def test_mu_closed_forms():
