@@ -105,7 +105,7 @@ def traffic_from_profiles(fmt):
     return None
 
 
-def pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total):
+def pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total, formats=FORMATS, sort_every=0):
     """The production kernel at the bench shape, three launches per field
     format, then three 1 GiB device-to-device copies whose FETCH_SIZE and
     WRITE_SIZE calibrate the counters (run under rocprofv3 --pmc)."""
@@ -118,7 +118,7 @@ def pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total):
     from particle_transport_amd.injection import M_P, Q_E
     from particle_transport_amd.runner import Ensemble, Field
     n = x.shape[1]
-    for fmt in FORMATS:
+    for fmt in formats:
         fld = Field(ctx, F32, L, fmt)
         dt = (2 * math.pi / S) / (Q_E * fld.B0 / (gamma * M_P))
         ens = Ensemble(ctx, fld, x, p, pusher="fast")
@@ -126,7 +126,7 @@ def pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total):
         si = 1
         for _ in range(3):
             r = ens.advance(dt=dt, first_step=si, n_steps=S, total_steps=total, stride=stride, out_kind="mu",
-                            out_a=mu, steps_per_launch=S)
+                            out_a=mu, steps_per_launch=S, sort_every_steps=sort_every)
             si += S
             log(f"[probe] {fmt} launch {r.kernel_ms:.3f} ms")
         ens.close()
@@ -440,10 +440,11 @@ def pcie_inclusive(F32, L, x, p, energy_gev, log_fn, n=4194304, duration_gp=20.0
                     "chunks of 10 samples and pinned staging"}
 
 
-def serving_run(args, log_fn):
-    """The serving stream on four resident 512^3 BF16 fields: Poisson arrivals
-    at 20/s, per-request field/energy/count/duration/injection/output drawn as
-    the serving configuration specifies, 200 GB budget.  Runs the first
+def serving_run(args, log_fn, fmt="bf16_brick"):
+    """The serving stream on four resident 512^3 16-bit fields (`fmt`, the
+    format the D_mumu check selected): Poisson arrivals at 20/s, per-request
+    field/energy/count/duration/injection/output drawn as the serving
+    configuration specifies, 200 GB budget.  Runs the first
     args.serving_requests requests of the stream in real time."""
     from particle_transport_amd import serving
     from particle_transport_amd.fields import box_length_m
@@ -454,7 +455,7 @@ def serving_run(args, log_fn):
     for k in range(4):
         log_fn(f"[serve] field {k}")
         F = make_fields(args.cells + 1, 1.0, 64.0, seed=3000 + k)[0]
-        flds.append(Field(push_ctx, F, L, "bf16_brick"))
+        flds.append(Field(push_ctx, F, L, fmt))
         del F
     B0s = [f.B0 for f in flds]
     reqs = serving.make_stream(args.serving_requests, 4, B0s, rate_hz=20.0, seed=77)
@@ -469,7 +470,8 @@ def serving_run(args, log_fn):
             "wall_s": res.wall_s, "tick_kernel_s": res.kernel_ms / 1e3,
             "tick_kernel_particle_steps_per_s": res.particle_steps / max(res.kernel_ms / 1e3, 1e-12),
             "product_thread_busy_s": res.product_busy_s, "tick_steps": args.serving_tick_steps,
-            "failed_requests": len(res.failed), "stream": "Poisson 20/s, 4 resident BF16 fields, 200 GB budget, "
+            "failed_requests": len(res.failed), "field_format": fmt,
+            "stream": f"Poisson 20/s, 4 resident {fmt} fields, 200 GB budget, "
             "energy log-U[1 GeV,100 TeV], n log-U[1e4,1e6], duration U[10,500] T_g0, mu+D_mumu(64x20) or D_pp"}
 
 
@@ -544,10 +546,17 @@ def main():
     ap.add_argument("--mode-gp", type=int, default=10)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run the production kernel 3x per field format plus 1 GiB copies (for rocprofv3 --pmc)")
+    ap.add_argument("--formats", default=",".join(FORMATS),
+                    help="comma-separated field formats to time (bf16_brick, f16_brick, f32_brick)")
+    ap.add_argument("--sort-every", type=int, default=0,
+                    help="Morton re-sort of the particles every this many pushes (0 = never)")
     ap.add_argument("--field-file", default=None,
                     help="pre-generated field: the stdout of python -m particle_transport_amd.fields --nodes 513 "
                          "--seed 2024 --dB 1.0 --kmax 64")
     args = ap.parse_args()
+    formats = tuple(f for f in args.formats.split(",") if f)
+    if not formats or any(f not in FORMATS for f in formats):
+        ap.error(f"--formats takes a subset of {','.join(FORMATS)}")
     if args.gpus != 1:
         print("bench.py measures one GPU: the pusher runs on a single MI355X; particles would shard "
               "across GPUs with replicated fields and one reduction of the pair sums, which is not run here.",
@@ -579,14 +588,14 @@ def main():
     total = max(500, args.warmup + args.steps) * S + 1  # the campaign: 500 T_g0 (more if asked for)
     nsave = S // stride
     if args.pmc_probe:
-        pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total)
+        pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total, formats, args.sort_every)
         ctx.close()
         return
 
     # The production pusher on each 16-bit brick format and on FP32 bricks.
     timing = {}
     dt = None
-    for fmt in FORMATS:
+    for fmt in formats:
         log(f"[bench] packing the {fmt} field")
         fld = Field(ctx, F32, L, fmt)
         if dt is None:
@@ -598,7 +607,7 @@ def main():
         def bench_step():
             nonlocal si
             r = ens.advance(dt=dt, first_step=si, n_steps=S, total_steps=total, stride=stride, out_kind="mu",
-                            out_a=mu_buf, steps_per_launch=S)
+                            out_a=mu_buf, steps_per_launch=S, sort_every_steps=args.sort_every)
             si += S
             return r
 
@@ -663,13 +672,21 @@ def main():
     ctx.close()
     del F32
 
-    serve = serving_run(args, log) if args.serving_requests > 0 else None
-    modes = None if args.no_modes else modes_run(args, log)
-
     # Headline: the fastest field format whose D_mumu passes the check against
-    # the FP64 pusher (the fastest format when the check is skipped).
-    passing = [f for f in FORMATS if check is None or check[FORMAT_KEY[f]]["pass"]]
-    head = max(passing, key=lambda f: timing[f]["value"]) if passing else "bf16_brick"
+    # the FP64 pusher (the fastest format when the check is skipped).  With no
+    # passing format there is no headline value.
+    passing = [f for f in formats if check is None or check[FORMAT_KEY[f]]["pass"]]
+    head = max(passing, key=lambda f: timing[f]["value"]) if passing else max(formats, key=lambda f: timing[f]["value"])
+    # serving runs on BF16 when it passes the check, else on FP16 when that
+    # passes; a 16-bit format that failed the check serves nothing
+    serve_fmt = next((f for f in ("bf16_brick", "f16_brick") if f in passing), None)
+    serve = None
+    if args.serving_requests > 0:
+        if serve_fmt is None:
+            serve = {"skipped": "no 16-bit field format passed the D_mumu check"}
+        else:
+            serve = serving_run(args, log, serve_fmt)
+    modes = None if args.no_modes else modes_run(args, log)
     t = timing[head]
     value, wall, avg_launch_s = t["value"], t["wall"], t["avg_launch_s"]
     K = S
@@ -681,7 +698,7 @@ def main():
     traffic = traffic_from_profiles(head) if profiled else None
     out = {
         "metric": "particle-steps/s (FP32 vs BF16 field) at D_mumu err<=1%; p99 campaign latency",
-        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "value": value if passing else None, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"precision form: {args.cells}^3 cells (dB/B0=1, k=[1,64]), field as {head}, {n} "
@@ -689,7 +706,10 @@ def main():
                                f"{total // S} T_g0 campaign; 1 step = 1 T_g0",
                    "field_format": head,
                    "headline_rule": "fastest field format whose D_mumu passes the FP64 check"
-                                    + (" (check skipped: fastest format)" if check is None else ""),
+                                    + (" (check skipped: fastest format)" if check is None else "")
+                                    + ("" if passing else "; no format passed, so value is null and the "
+                                       "fastest format's figures are reported for reference only"),
+                   "no_format_passed": not passing,
                    "global_batch": n, "seq_len": S, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -702,10 +722,10 @@ def main():
                              "them only on a cell change, so frac exceeds 1 and the FP32 ALU fraction is the "
                              "binding roof"},
         "formats": {f: {"value": timing[f]["value"], "avg_launch_ms": timing[f]["avg_launch_s"] * 1e3,
-                        "dmumu_pass": None if check is None else check[FORMAT_KEY[f]]["pass"]} for f in FORMATS},
+                        "dmumu_pass": None if check is None else check[FORMAT_KEY[f]]["pass"]} for f in formats},
         "plain_f32": {"value": plain_rate, "kernel_particle_steps_per_s": plain_kernel,
                       "speedup_of_headline": value / plain_rate,
-                      "speedup_of_bf16": timing["bf16_brick"]["value"] / plain_rate},
+                      "speedup_of_bf16": timing["bf16_brick"]["value"] / plain_rate if "bf16_brick" in timing else None},
         "dmumu_check": check,
         "statistics": stats,
         "injection": inject,

@@ -46,7 +46,9 @@ struct InjectGeom {
     double lo[3], hi[3], d[3];
     int64_t n[3];
     int64_t nb[3];
-    float scale[3], offset[3];  // 16-bit bricks: decoded = stored * scale + offset
+    float scale[3];             // 16-bit bricks: decoded = (stored + coarse part) * scale
+    const float *coarse;        // 16-bit bricks: coarse-part coefficients
+    int32_t cn[3];
 };
 
 // Value of B component comp (0..2) at 0-based node (i, j, k) as the field
@@ -68,8 +70,8 @@ __device__ __forceinline__ double node_b(const void *__restrict__ data, const In
         } else {
             const uint16_t h = static_cast<const uint16_t *>(data)[rec];
             const float v = STORE == PT_STORE_BF16_BRICK ? bf16_bits_to_float(h) : f16_bits_to_float(h);
-            return static_cast<double>(v) * static_cast<double>(g.scale[comp]) +
-                   static_cast<double>(g.offset[comp]);
+            return (static_cast<double>(v) + coarse_at_node(g.coarse, g.cn, comp, i, j, k)) *
+                   static_cast<double>(g.scale[comp]);
         }
     }
 }
@@ -204,8 +206,9 @@ void launch_inject(const pt_field *f, const pt_inject_params &ip, int64_t first,
         g.n[c] = f->g.n[c];
         g.nb[c] = f->g.nb[c];
         g.scale[c] = f->sc.scale[c];
-        g.offset[c] = f->sc.offset[c];
+        g.cn[c] = f->cn[c];
     }
+    g.coarse = f->coarse;
     const unsigned blocks = static_cast<unsigned>((count + 255) / 256);
     switch (f->storage) {
         case PT_STORE_F64_PLANAR:

@@ -50,9 +50,39 @@ struct Geom {
 };
 
 struct FieldScale {
-    float scale[6];    // decoded value = stored * scale + offset
-    float offset[6];
+    float scale[6];    // decoded value = (stored + coarse part, 16-bit bricks) * scale
 };
+
+// 16-bit bricks store each component as a coarse part plus a 16-bit residual.
+// The coarse part is the trilinear interpolant of the field's own values at
+// every 4th node per axis (the last coarse cell ends at the last node): per
+// coarse cell and component, the FP32 coefficients of that interpolant as a
+// polynomial a0 + a1 X + a2 Y + a3 Z + a4 XY + a5 XZ + a6 YZ + a7 XYZ in
+// fine-node units X, Y, Z from the coarse cell's first node, in units of the
+// component's scale_c (a power of two at least the largest residual, so the
+// unit change is exact).  A node stores value / scale_c - coarse part in
+// BF16/FP16, so the 16-bit rounding acts on the field's variation inside a
+// 4^3-cell block instead of on its whole range.
+constexpr int kCoarseShift = 2;   // coarse cell = 4 fine cells per axis
+constexpr int kCoarseWidth = 1 << kCoarseShift;
+
+// Coarse part at node (i, j, k) in FP64 from the stored coefficients: the
+// node's coarse cell is floor(i/4) per axis, clamped to the last one (whose
+// end node is the grid's last node).
+__host__ __device__ inline double coarse_at_node(const float *__restrict__ table, const int32_t cn[3], int comp,
+                                                 int64_t i, int64_t j, int64_t k) {
+    int64_t I = i >> kCoarseShift, J = j >> kCoarseShift, K = k >> kCoarseShift;
+    I = I < cn[0] - 1 ? I : cn[0] - 1;
+    J = J < cn[1] - 1 ? J : cn[1] - 1;
+    K = K < cn[2] - 1 ? K : cn[2] - 1;
+    const double X = static_cast<double>(i - (I << kCoarseShift));
+    const double Y = static_cast<double>(j - (J << kCoarseShift));
+    const double Z = static_cast<double>(k - (K << kCoarseShift));
+    const float *a = table + ((I + cn[0] * (J + cn[1] * K)) * 6 + comp) * 8;
+    return static_cast<double>(a[0]) + static_cast<double>(a[1]) * X + static_cast<double>(a[2]) * Y +
+           static_cast<double>(a[3]) * Z + static_cast<double>(a[4]) * X * Y + static_cast<double>(a[5]) * X * Z +
+           static_cast<double>(a[6]) * Y * Z + static_cast<double>(a[7]) * X * Y * Z;
+}
 
 }  // namespace pt
 
@@ -73,6 +103,9 @@ struct pt_field {
     void *data = nullptr;        // planar: 6 consecutive component arrays; brick: records
     size_t bytes = 0;
     pt::FieldScale sc{};         // brick storages only
+    float *coarse = nullptr;     // 16-bit bricks: coarse-part coefficients [cell][6][8]
+    size_t coarse_bytes = 0;
+    int32_t cn[3] = {0, 0, 0};   // coarse cells per axis
     double B0 = 0.0;             // FP64 mean |B| of the source values
 };
 
@@ -161,8 +194,6 @@ int64_t launch_push_fast(pt_ens *e, const pt_field *f, const PushArgs &a);
 void launch_pack_field(pt_field *f, const void *const src[6], int src_f64, hipStream_t s);
 void launch_mean_abs_b(pt_ctx *ctx, int64_t n, int src_f64, const void *bx, const void *by,
                        const void *bz, double *out_host);
-void launch_component_stats(pt_ctx *ctx, int64_t n, int src_f64, const void *src, double *mean,
-                            double *maxdev);
 void launch_split_from_si(pt_ens *e, const double *dx, const double *dp);
 void morton_sort(pt_ens *e);  // pt_sort.hip
 void launch_split_to_si(pt_ens *e, double *dx, double *dp, uint8_t *alive_out);
@@ -190,6 +221,27 @@ __host__ __device__ inline int64_t brick_node_index(const int64_t nb[3], int64_t
                                                     int64_t k) {
     int64_t brick = ((k >> 3) * nb[1] + (j >> 3)) * nb[0] + (i >> 3);
     return (brick << 9) + (((k & 7) << 6) | ((j & 7) << 3) | (i & 7));
+}
+
+// Split coordinate of u = x / dx on an axis of nc cells: cell = floor(u) and
+// the FP32 offset u - cell (an offset that rounds up to 1 moves to the next
+// cell at offset 0).  A periodic axis holds the upper face x == L as the last
+// cell at offset 1: the particle samples the field there as trilinear_sample
+// clamps it (phase_space_gpu_runner.jl:247-276), and the first push that moves
+// it outward takes it past offset 1, where the periodic wrap moves it to cell
+// 0, as wrap_periodic_coordinate (:325-328) maps x > L to x - L.
+__host__ __device__ inline void split_coord(double u, int32_t nc, bool periodic, int32_t &cell, float &off) {
+    const double fl = floor(u);
+    off = static_cast<float>(u - fl);
+    cell = static_cast<int32_t>(fl);
+    if (off >= 1.0f) {
+        off = 0.0f;
+        cell += 1;
+    }
+    if (periodic && cell == nc && off == 0.0f) {
+        cell = nc - 1;
+        off = 1.0f;
+    }
 }
 
 __device__ inline float bf16_bits_to_float(uint16_t h) {

@@ -20,9 +20,26 @@ struct FieldView {
     const void *data;
     int64_t n[3];
     int64_t nb[3];
-    float scale[6];
-    float offset[6];
+    float scale[6];       // decoded value = stored * scale (+ the coarse part, 16-bit bricks)
+    const float *coarse;  // 16-bit bricks: coarse-part coefficients [coarse cell][6][8]
+    int32_t cn[3];        // coarse cells per axis
 };
+
+__host__ inline FieldView make_field_view(const pt_field *f) {
+    FieldView fv;
+    fv.data = f->data;
+    for (int c = 0; c < 3; ++c) {
+        fv.n[c] = f->g.n[c];
+        fv.nb[c] = f->g.nb[c];
+        fv.cn[c] = f->cn[c];
+    }
+    for (int c = 0; c < 6; ++c) fv.scale[c] = f->sc.scale[c];
+    fv.coarse = f->coarse;
+    return fv;
+}
+
+template <int STORE>
+constexpr bool kHierarchical = STORE == PT_STORE_BF16_BRICK || STORE == PT_STORE_F16_BRICK;
 
 struct SplitGeom {
     int32_t nc[3];   // cells per axis (nodes - 1)
@@ -285,11 +302,11 @@ __device__ __forceinline__ void load_corners(const FieldView &fv, int32_t gi, in
 
 // Trilinear polynomial coefficients of one component over a cell:
 // f(x,y,z) = a0 + a1 x + a2 y + a3 z + a4 xy + a5 xz + a6 yz + a7 xyz,
-// decoded (stored*scale + offset) and multiplied by the physical factor.
-__device__ __forceinline__ void coeffs(const float s[8], float scale, float offset, float a[8]) {
+// decoded (stored * scale) and multiplied by the physical factor.
+__device__ __forceinline__ void coeffs(const float s[8], float scale, float a[8]) {
     const float dx00 = s[1] - s[0], dx10 = s[3] - s[2], dx01 = s[5] - s[4], dx11 = s[7] - s[6];
     const float dy0 = s[2] - s[0], dy1 = s[6] - s[4];
-    a[0] = s[0] * scale + offset;
+    a[0] = s[0] * scale;
     a[1] = dx00 * scale;
     a[2] = dy0 * scale;
     a[3] = (s[4] - s[0]) * scale;
@@ -313,6 +330,41 @@ __device__ __forceinline__ void shift_coeffs(float a[8], float bx, float by, flo
     a[1] = fmaf(a[5], bz, a[1]);
     a[2] = fmaf(a[6], bz, a[2]);
     a[4] = fmaf(a[7], bz, a[4]);
+}
+
+// Add the coarse part of one component over fine cell (gi, gj, gk) of a
+// 16-bit brick field to its residual polynomial a: the coarse cell's
+// polynomial (fine-node units from its first node, in units of the residual
+// scale) moved to the cell's origin, times `scale` (the residual scale times
+// the physical factor).  The cell lies inside coarse cell (gi >> 2, gj >> 2,
+// gk >> 2).  The table index counts 16-byte units in 32 bits: a brick field
+// has fewer than 2^31 nodes, so fewer than 2^26 coarse cells.
+__device__ __forceinline__ void add_coarse(const FieldView &fv, int32_t gi, int32_t gj, int32_t gk, int comp,
+                                           float scale, float a[8]) {
+    const uint32_t cell = static_cast<uint32_t>(
+        (gi >> kCoarseShift) + fv.cn[0] * ((gj >> kCoarseShift) + fv.cn[1] * (gk >> kCoarseShift)));
+    const float4 *src = reinterpret_cast<const float4 *>(fv.coarse) + (cell * 6u + static_cast<uint32_t>(comp)) * 2u;
+    const float4 lo = src[0], hi = src[1];
+    float A[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    shift_coeffs(A, static_cast<float>(gi & (kCoarseWidth - 1)), static_cast<float>(gj & (kCoarseWidth - 1)),
+                 static_cast<float>(gk & (kCoarseWidth - 1)));
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] = fmaf(A[q], scale, a[q]);
+}
+
+// Cell polynomial of one component from its 8 corner values in storage
+// units: scale already carries the physical factor.  16-bit
+// bricks then add their coarse part, loaded only after the residual
+// coefficients are formed (the barrier keeps the scheduler from holding both
+// sets of loads in flight, which costs registers in the reload path).
+template <int STORE>
+__device__ __forceinline__ void cell_coeffs(const FieldView &fv, int32_t gi, int32_t gj, int32_t gk, int comp,
+                                            const float s[8], float scale, float a[8]) {
+    coeffs(s, scale, a);
+    if constexpr (kHierarchical<STORE>) {
+        __builtin_amdgcn_sched_barrier(0);
+        add_coarse(fv, gi, gj, gk, comp, scale, a);
+    }
 }
 
 __device__ __forceinline__ float eval_poly(const float a[8], float x, float y, float z) {
@@ -397,7 +449,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 struct ReloadLane {
     int lane, entry, comp;  // helper role: posted cell `entry`, component `comp`
-    float scale, offset;    // decode of `comp` times its physical factor
+    float scale;            // decode of `comp` times its physical factor
 };
 
 __device__ __forceinline__ ReloadLane reload_lane(const FieldView &fv, const Phys &ph) {
@@ -406,13 +458,9 @@ __device__ __forceinline__ ReloadLane reload_lane(const FieldView &fv, const Phy
     r.entry = r.lane / 6;
     r.comp = r.lane - 6 * r.entry;
     r.scale = fv.scale[0] * ph.fac[0];
-    r.offset = fv.offset[0] * ph.fac[0];
 #pragma unroll
     for (int c = 1; c < 6; ++c) {
-        if (r.comp == c) {
-            r.scale = fv.scale[c] * ph.fac[c];
-            r.offset = fv.offset[c] * ph.fac[c];
-        }
+        if (r.comp == c) r.scale = fv.scale[c] * ph.fac[c];
     }
     return r;
 }
@@ -429,7 +477,8 @@ __device__ __forceinline__ void lane_reload(const int32_t cell[3], const FieldVi
     float sv[6][8];
     load_corners<STORE>(fv, gi, gj, gk, sv);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) coeffs(sv[c], fv.scale[c] * ph.fac[c], fv.offset[c] * ph.fac[c], a[c]);
+    for (int c = 0; c < 6; ++c)
+        cell_coeffs<STORE>(fv, gi, gj, gk, c, sv[c], fv.scale[c] * ph.fac[c], a[c]);
     if (b[0] != 0.0f || b[1] != 0.0f || b[2] != 0.0f) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) shift_coeffs(a[c], b[0], b[1], b[2]);
@@ -459,7 +508,7 @@ __device__ __forceinline__ void coop_reload(int &need, const int32_t cell[3], co
             const int4 t = lds->slot[rl.entry];
             float sv[8], c8[8];
             load_component_corners<STORE>(fv, t.x, t.y, t.z, rl.comp, sv);
-            coeffs(sv, rl.scale, rl.offset, c8);
+            cell_coeffs<STORE>(fv, t.x, t.y, t.z, rl.comp, sv, rl.scale, c8);
             if (t.w) shift_coeffs(c8, (t.w & 1) ? 1.0f : 0.0f, (t.w & 2) ? 1.0f : 0.0f, (t.w & 4) ? 1.0f : 0.0f);
             float4 *dst = reinterpret_cast<float4 *>(lds->coef[rl.entry][rl.comp]);
             dst[0] = make_float4(c8[0], c8[1], c8[2], c8[3]);

@@ -133,16 +133,21 @@ __global__ void k_pack_brick_f32(const Src *__restrict__ src, float *__restrict_
     dst[brick_node_index(nb, i, j, k) * 8 + c] = static_cast<float>(src[idx]);
 }
 
+// 16-bit residual of component c: value / scale - coarse part (the coarse
+// coefficients are stored in units of scale), rounded to nearest even.
 template <class Src, bool BF16>
 __global__ void k_pack_brick_16(const Src *__restrict__ src, uint16_t *__restrict__ dst, int c,
                                 int64_t nx, int64_t ny, int64_t nz, int64_t nbx, int64_t nby,
-                                int64_t nbz, float inv_scale, double offset) {
+                                int64_t nbz, float inv_scale, const float *__restrict__ coarse, int32_t cn0,
+                                int32_t cn1, int32_t cn2) {
     int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     int64_t n = nx * ny * nz;
     if (idx >= n) return;
     int64_t i = idx % nx, j = (idx / nx) % ny, k = idx / (nx * ny);
     int64_t nb[3] = {nbx, nby, nbz};
-    float v = static_cast<float>((static_cast<double>(src[idx]) - offset)) * inv_scale;
+    const int32_t cn[3] = {cn0, cn1, cn2};
+    const double r = static_cast<double>(src[idx]) * static_cast<double>(inv_scale) - coarse_at_node(coarse, cn, c, i, j, k);
+    float v = static_cast<float>(r);
     uint16_t bits;
     if (BF16) {
         bits = __bfloat16_as_ushort(__float2bfloat16(v));  // round to nearest even
@@ -181,46 +186,83 @@ __global__ void k_sum_partials(const double *__restrict__ part, int nparts, doub
     }
 }
 
-// Per-component mean (FP64) and min/max for BF16/FP16 scale selection.
+// Coarse-part coefficients of component c (16-bit bricks): one thread per
+// coarse cell reads the field at the cell's 8 coarse nodes (every 4th node,
+// the last cell ending at the grid's last node), forms the trilinear
+// polynomial in fine-node units and stores it in FP32.
 template <class Src>
-__global__ void k_comp_partials(const Src *__restrict__ v, int64_t n, double *__restrict__ psum,
-                                double *__restrict__ pmin, double *__restrict__ pmax) {
-    __shared__ double rs[256], rmin[256], rmax[256];
-    double s = 0.0, lo = INFINITY, hi = -INFINITY;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        double x = v[i];
-        s += x;
-        lo = fmin(lo, x);
-        hi = fmax(hi, x);
+__global__ void k_coarse_coeffs(const Src *__restrict__ src, int c, int64_t nx, int64_t ny, int64_t nz,
+                                int32_t cn0, int32_t cn1, int32_t cn2, float *__restrict__ table) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t ncell = static_cast<int64_t>(cn0) * cn1 * cn2;
+    if (t >= ncell) return;
+    const int64_t I = t % cn0, J = (t / cn0) % cn1, K = t / (static_cast<int64_t>(cn0) * cn1);
+    auto node = [](int64_t q, int64_t n) { return (q << kCoarseShift) < n - 1 ? (q << kCoarseShift) : n - 1; };
+    const int64_t i0 = node(I, nx), i1 = node(I + 1, nx);
+    const int64_t j0 = node(J, ny), j1 = node(J + 1, ny);
+    const int64_t k0 = node(K, nz), k1 = node(K + 1, nz);
+    const double wx = static_cast<double>(i1 - i0), wy = static_cast<double>(j1 - j0),
+                 wz = static_cast<double>(k1 - k0);
+    double v[8];
+    for (int ci = 0; ci < 8; ++ci) {
+        const int64_t i = (ci & 1) ? i1 : i0, j = (ci & 2) ? j1 : j0, k = (ci & 4) ? k1 : k0;
+        v[ci] = static_cast<double>(static_cast<float>(src[i + nx * (j + ny * k)]));
     }
-    rs[threadIdx.x] = s;
-    rmin[threadIdx.x] = lo;
-    rmax[threadIdx.x] = hi;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            rs[threadIdx.x] += rs[threadIdx.x + w];
-            rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + w]);
-            rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + w]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        psum[blockIdx.x] = rs[0];
-        pmin[blockIdx.x] = rmin[0];
-        pmax[blockIdx.x] = rmax[0];
-    }
+    const double dx00 = v[1] - v[0], dx10 = v[3] - v[2], dx01 = v[5] - v[4], dx11 = v[7] - v[6];
+    const double dy0 = v[2] - v[0], dy1 = v[6] - v[4];
+    float *a = table + (t * 6 + c) * 8;
+    a[0] = static_cast<float>(v[0]);
+    a[1] = static_cast<float>(dx00 / wx);
+    a[2] = static_cast<float>(dy0 / wy);
+    a[3] = static_cast<float>((v[4] - v[0]) / wz);
+    a[4] = static_cast<float>((dx10 - dx00) / (wx * wy));
+    a[5] = static_cast<float>((dx01 - dx00) / (wx * wz));
+    a[6] = static_cast<float>((dy1 - dy0) / (wy * wz));
+    a[7] = static_cast<float>(((dx11 - dx01) - (dx10 - dx00)) / (wx * wy * wz));
 }
 
-// SI state -> split state.  cell = floor(x/d), off = x/d - cell rounded to
-// FP32 (an offset that rounds up to 1 moves to the next cell at offset 0).
+// Coarse coefficients of component c times inv_scale (a power of two: exact).
+__global__ void k_scale_coarse(float *__restrict__ table, int c, int64_t ncell, float inv_scale) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= ncell) return;
+    float *a = table + (t * 6 + c) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] *= inv_scale;
+}
+
+// Largest |value - coarse part| of component c, as the bit pattern of a
+// non-negative float (atomic max on the bits orders it as the values).
+template <class Src>
+__global__ void k_resid_absmax(const Src *__restrict__ src, int c, int64_t nx, int64_t ny, int64_t nz,
+                               const float *__restrict__ table, int32_t cn0, int32_t cn1, int32_t cn2,
+                               unsigned *__restrict__ out_bits) {
+    __shared__ float red[256];
+    const int32_t cn[3] = {cn0, cn1, cn2};
+    const int64_t n = nx * ny * nz;
+    float m = 0.0f;
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; idx < n;
+         idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t i = idx % nx, j = (idx / nx) % ny, k = idx / (nx * ny);
+        const double r = static_cast<double>(src[idx]) - coarse_at_node(table, cn, c, i, j, k);
+        m = fmaxf(m, static_cast<float>(fabs(r)));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out_bits, __float_as_uint(red[0]));
+}
+
+// SI state -> split state (split_coord per axis).
 __global__ void k_split_from_si(const double *__restrict__ x, const double *__restrict__ p,
                                 int64_t n, SplitState s, double d0, double d1, double d2,
-                                double pnorm) {
+                                int32_t nc0, int32_t nc1, int32_t nc2, int periodic, double pnorm) {
     int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double d[3] = {d0, d1, d2};
+    const int32_t nc[3] = {nc0, nc1, nc2};
     double p2 = 0.0;
     for (int c = 0; c < 3; ++c) {
         const double pc = p[c * n + i] / pnorm;
@@ -228,14 +270,9 @@ __global__ void k_split_from_si(const double *__restrict__ x, const double *__re
     }
     s.P2[i] = p2;
     for (int c = 0; c < 3; ++c) {
-        double u = x[c * n + i] / d[c];
-        double fl = floor(u);
-        float off = static_cast<float>(u - fl);
-        int32_t cell = static_cast<int32_t>(fl);
-        if (off >= 1.0f) {
-            off = 0.0f;
-            cell += 1;
-        }
+        int32_t cell;
+        float off;
+        split_coord(x[c * n + i] / d[c], nc[c], periodic != 0, cell, off);
         s.cell[c][i] = cell;
         s.off[c][i] = off;
         s.P[c][i] = static_cast<float>(p[c * n + i] / pnorm);
@@ -300,22 +337,23 @@ void launch_pack_field(pt_field *f, const void *const src[6], int src_f64, hipSt
             case PT_STORE_BF16_BRICK:
             case PT_STORE_F16_BRICK: {
                 uint16_t *dst = static_cast<uint16_t *>(f->data);
-                float inv = 1.0f / f->sc.scale[c];
-                double off = f->sc.offset[c];
-                bool bf = f->storage == PT_STORE_BF16_BRICK;
-                if (src_f64) {
-                    auto *sp = static_cast<const double *>(src[c]);
+                const float inv = 1.0f / f->sc.scale[c];
+                const bool bf = f->storage == PT_STORE_BF16_BRICK;
+                auto pack = [&](auto *sp) {
+                    using Src = std::remove_cv_t<std::remove_pointer_t<decltype(sp)>>;
                     if (bf)
-                        k_pack_brick_16<double, true><<<grid_for(n), 256, 0, s>>>(sp, dst, c, nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2], inv, off);
+                        k_pack_brick_16<Src, true><<<grid_for(n), 256, 0, s>>>(
+                            sp, dst, c, nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2], inv, f->coarse, f->cn[0],
+                            f->cn[1], f->cn[2]);
                     else
-                        k_pack_brick_16<double, false><<<grid_for(n), 256, 0, s>>>(sp, dst, c, nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2], inv, off);
-                } else {
-                    auto *sp = static_cast<const float *>(src[c]);
-                    if (bf)
-                        k_pack_brick_16<float, true><<<grid_for(n), 256, 0, s>>>(sp, dst, c, nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2], inv, off);
-                    else
-                        k_pack_brick_16<float, false><<<grid_for(n), 256, 0, s>>>(sp, dst, c, nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2], inv, off);
-                }
+                        k_pack_brick_16<Src, false><<<grid_for(n), 256, 0, s>>>(
+                            sp, dst, c, nx, ny, nz, f->g.nb[0], f->g.nb[1], f->g.nb[2], inv, f->coarse, f->cn[0],
+                            f->cn[1], f->cn[2]);
+                };
+                if (src_f64)
+                    pack(static_cast<const double *>(src[c]));
+                else
+                    pack(static_cast<const float *>(src[c]));
                 break;
             }
         }
@@ -344,35 +382,49 @@ void launch_mean_abs_b(pt_ctx *ctx, int64_t n, int src_f64, const void *bx, cons
     *out_host = sum / static_cast<double>(n);
 }
 
-void launch_component_stats(pt_ctx *ctx, int64_t n, int src_f64, const void *src, double *mean,
-                            double *maxdev) {
-    const int nparts = 512;
-    double *part = static_cast<double *>(ctx_scratch(ctx, 3 * nparts * sizeof(double)));
+// Coarse part of component c and the scale of its 16-bit residual (the
+// largest |value - coarse part|; 1 where the field equals its coarse part).
+static void build_coarse_component(pt_field *f, int c, int src_f64, const void *src) {
+    pt_ctx *ctx = f->ctx;
+    const int64_t nx = f->g.n[0], ny = f->g.n[1], nz = f->g.n[2];
+    const int64_t ncell = static_cast<int64_t>(f->cn[0]) * f->cn[1] * f->cn[2];
+    unsigned *bits = static_cast<unsigned *>(ctx_scratch(ctx, 256));
+    PT_HIP_CHECK(hipMemsetAsync(bits, 0, sizeof(unsigned), ctx->stream));
+    auto run = [&](auto *sp) {
+        using Src = std::remove_cv_t<std::remove_pointer_t<decltype(sp)>>;
+        k_coarse_coeffs<Src><<<grid_for(ncell), 256, 0, ctx->stream>>>(sp, c, nx, ny, nz, f->cn[0], f->cn[1],
+                                                                         f->cn[2], f->coarse);
+        PT_HIP_CHECK(hipGetLastError());
+        k_resid_absmax<Src><<<1024, 256, 0, ctx->stream>>>(sp, c, nx, ny, nz, f->coarse, f->cn[0], f->cn[1],
+                                                            f->cn[2], bits);
+        PT_HIP_CHECK(hipGetLastError());
+    };
     if (src_f64)
-        k_comp_partials<<<nparts, 256, 0, ctx->stream>>>(static_cast<const double *>(src), n, part,
-                                                         part + nparts, part + 2 * nparts);
+        run(static_cast<const double *>(src));
     else
-        k_comp_partials<<<nparts, 256, 0, ctx->stream>>>(static_cast<const float *>(src), n, part,
-                                                         part + nparts, part + 2 * nparts);
-    PT_HIP_CHECK(hipGetLastError());
-    std::vector<double> h(3 * nparts);
-    PT_HIP_CHECK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        run(static_cast<const float *>(src));
+    unsigned h = 0;
+    PT_HIP_CHECK(hipMemcpyAsync(&h, bits, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
     PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    double s = 0.0, lo = INFINITY, hi = -INFINITY;
-    for (int i = 0; i < nparts; ++i) {
-        s += h[i];
-        lo = std::min(lo, h[nparts + i]);
-        hi = std::max(hi, h[2 * nparts + i]);
-    }
-    *mean = s / static_cast<double>(n);
-    *maxdev = std::max(hi - *mean, *mean - lo);
+    float m;
+    std::memcpy(&m, &h, sizeof(float));
+    // A power of two at least the largest residual: the coarse coefficients
+    // are then stored in units of it exactly, and the decode multiplies the
+    // coarse part and the residual by the same factor.
+    int e = 0;
+    std::frexp(static_cast<double>(m), &e);
+    const double scale = m > 0.0f && std::isfinite(m) ? std::ldexp(1.0, e) : 1.0;
+    f->sc.scale[c] = static_cast<float>(scale);
+    k_scale_coarse<<<grid_for(ncell), 256, 0, ctx->stream>>>(f->coarse, c, ncell, static_cast<float>(1.0 / scale));
+    PT_HIP_CHECK(hipGetLastError());
 }
 
 void launch_split_from_si(pt_ens *e, const double *dx, const double *dp) {
     SplitState s = split_state(e);
     const Geom &g = e->field->g;
-    k_split_from_si<<<grid_for(e->n), 256, 0, e->ctx->stream>>>(dx, dp, e->n, s, g.d[0], g.d[1],
-                                                               g.d[2], kMp * kC);
+    k_split_from_si<<<grid_for(e->n), 256, 0, e->ctx->stream>>>(
+        dx, dp, e->n, s, g.d[0], g.d[1], g.d[2], static_cast<int32_t>(g.n[0] - 1), static_cast<int32_t>(g.n[1] - 1),
+        static_cast<int32_t>(g.n[2] - 1), e->periodic, kMp * kC);
     PT_HIP_CHECK(hipGetLastError());
 }
 
@@ -613,6 +665,12 @@ int pt_field_create(pt_ctx *ctx, int64_t nx, int64_t ny, int64_t nz, double box_
             }
             f->data = dev_alloc(ctx, f->bytes);
             PT_HIP_CHECK(hipMemsetAsync(f->data, 0, f->bytes, ctx->stream));
+            if (storage == PT_STORE_BF16_BRICK || storage == PT_STORE_F16_BRICK) {
+                for (int c = 0; c < 3; ++c)
+                    f->cn[c] = static_cast<int32_t>((nn[c] - 1 + kCoarseWidth - 1) / kCoarseWidth);
+                f->coarse_bytes = static_cast<size_t>(f->cn[0]) * f->cn[1] * f->cn[2] * 48 * sizeof(float);
+                f->coarse = static_cast<float *>(dev_alloc(ctx, f->coarse_bytes));
+            }
             for (int half = 0; half < 2; ++half) {
                 const void *dsrc[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
                 for (int c = 0; c < 3; ++c) {
@@ -630,12 +688,8 @@ int pt_field_create(pt_ctx *ctx, int64_t nx, int64_t ny, int64_t nz, double box_
                 for (int c = 0; c < 3; ++c) {
                     const int comp = 3 * half + c;
                     if (storage == PT_STORE_BF16_BRICK || storage == PT_STORE_F16_BRICK) {
-                        double mean = 0, maxdev = 0;
-                        launch_component_stats(ctx, n, src_dtype == PT_F64, dsrc[comp], &mean, &maxdev);
-                        f->sc.offset[comp] = static_cast<float>(mean);
-                        f->sc.scale[comp] = maxdev > 0 ? static_cast<float>(maxdev) : 1.0f;
+                        build_coarse_component(f, comp, src_dtype == PT_F64, dsrc[comp]);
                     } else {
-                        f->sc.offset[comp] = 0.0f;
                         f->sc.scale[comp] = 1.0f;
                     }
                 }
@@ -648,6 +702,7 @@ int pt_field_create(pt_ctx *ctx, int64_t nx, int64_t ny, int64_t nz, double box_
             (void)hipStreamSynchronize(ctx->stream);
             dev_free(ctx, staging, staging_bytes);
             if (f->data) dev_free(ctx, f->data, f->bytes);
+            if (f->coarse) dev_free(ctx, f->coarse, f->coarse_bytes);
             delete f;
             throw;
         }
@@ -660,6 +715,7 @@ int pt_field_destroy(pt_field *field) {
         if (!field) return;
         (void)hipStreamSynchronize(field->ctx->stream);
         dev_free(field->ctx, field->data, field->bytes);
+        if (field->coarse) dev_free(field->ctx, field->coarse, field->coarse_bytes);
         delete field;
     });
 }

@@ -53,8 +53,7 @@ struct DevReq {
 // CUBIC: every request of the launch has cubic cells (the same c dt / dx on
 // every axis), so the position step factor is formed once per push.
 template <int STORE, bool CUBIC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
-k_serve(const DevReq *__restrict__ reqs, int nreq) {
+__device__ __forceinline__ void serve_tick(const DevReq *__restrict__ reqs, int nreq) {
     const int64_t b = blockIdx.x;
     int lo = 0, hi = nreq - 1;
     while (lo < hi) {
@@ -121,6 +120,20 @@ k_serve(const DevReq *__restrict__ reqs, int nreq) {
     R.alive[p] = alive ? 1 : 0;
 }
 
+// One tick over all running requests: five waves per SIMD on FP32 fields,
+// four on 16-bit fields (whose reload adds the coarse part; see the
+// single-campaign kernels in pt_push_fast.hip).
+template <int STORE, bool CUBIC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_serve(const DevReq *__restrict__ reqs, int nreq) {
+    serve_tick<STORE, CUBIC>(reqs, nreq);
+}
+template <int STORE, bool CUBIC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_serve16(const DevReq *__restrict__ reqs, int nreq) {
+    serve_tick<STORE, CUBIC>(reqs, nreq);
+}
+
 // Initial state of one request: split position from metres; momentum either
 // given (isotropic or any host-built state) or built on the device for a
 // fixed pitch cosine mu0 about the local field (fixed_mu_unit_vector,
@@ -132,14 +145,9 @@ __global__ void k_serve_init(DevReq R, const double *__restrict__ init, int fixe
     const int64_t n = R.n;
     if (i >= n) return;
     for (int c = 0; c < 3; ++c) {
-        const double u = init[c * n + i] / R.g.d[c];
-        const double fl = floor(u);
-        float off = static_cast<float>(u - fl);
-        int32_t cell = static_cast<int32_t>(fl);
-        if (off >= 1.0f) {
-            off = 0.0f;
-            cell += 1;
-        }
+        int32_t cell;
+        float off;
+        split_coord(init[c * n + i] / R.g.d[c], R.g.nc[c], R.periodic != 0, cell, off);
         R.cell[c][i] = cell;
         R.off[c][i] = off;
     }
@@ -159,7 +167,7 @@ __global__ void k_serve_init(DevReq R, const double *__restrict__ init, int fixe
     float sv[6][8], a[8], b[3];
     load_corners<STORE>(R.fv, g[0], g[1], g[2], sv);
     for (int c = 0; c < 3; ++c) {
-        coeffs(sv[c], R.fv.scale[c], R.fv.offset[c], a);
+        cell_coeffs<STORE>(R.fv, g[0], g[1], g[2], c, sv[c], R.fv.scale[c], a);
         b[c] = eval_poly(a, t[0], t[1], t[2]);
     }
     const float bn = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
@@ -374,18 +382,14 @@ void fill_dev(pt_serve *srv, SReq &r) {
     D.out_kind = r.d.out_kind;
     D.p_ref = r.d.out_p_ref;
     D.periodic = r.d.periodic ? 1 : 0;
-    D.fv.data = f->data;
+    D.fv = make_field_view(f);
     for (int c = 0; c < 3; ++c) {
-        D.fv.n[c] = f->g.n[c];
-        D.fv.nb[c] = f->g.nb[c];
         D.g.nc[c] = static_cast<int32_t>(f->g.n[c] - 1);
         D.g.k[c] = static_cast<float>(kC * r.d.dt_s / f->g.d[c]);
         D.g.d[c] = f->g.d[c];
     }
     const double bfac = r.d.charge_C * r.d.dt_s / (2.0 * r.d.mass_kg);
     for (int c = 0; c < 6; ++c) {
-        D.fv.scale[c] = f->sc.scale[c];
-        D.fv.offset[c] = f->sc.offset[c];
         D.ph.fac[c] = static_cast<float>(c < 3 ? bfac : 1.0 / kC);
     }
     D.ph.qsign = r.d.charge_C > 0 ? 1.0f : -1.0f;
@@ -484,10 +488,17 @@ void complete_all(pt_serve *srv) {
 template <int STORE>
 void launch_tick(pt_serve *srv, const DevReq *table, int nreq, int64_t blocks, bool cubic) {
     const unsigned grid = static_cast<unsigned>(blocks);
-    if (cubic)
-        k_serve<STORE, true><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
-    else
-        k_serve<STORE, false><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
+    if constexpr (kHierarchical<STORE>) {
+        if (cubic)
+            k_serve16<STORE, true><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
+        else
+            k_serve16<STORE, false><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
+    } else {
+        if (cubic)
+            k_serve<STORE, true><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
+        else
+            k_serve<STORE, false><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
+    }
     PT_HIP_CHECK(hipGetLastError());
 }
 

@@ -10,9 +10,13 @@
 //  Per-particle means are merged across particles with Chan's formula in a
 //  fixed tree (wavefront shuffles, then LDS), per-workgroup partials are
 //  merged in workgroup order, so the Welford statistics are deterministic.
-//  Integer counts are exact; sums are FP64.
+//  The per-bin sums are accumulated as 128-bit fixed-point integers (2^-80
+//  units), whose addition is exact and associative, so they are the same
+//  whatever order the lanes and workgroups flush in.  Integer counts are
+//  exact; the remaining sums are FP64.
 #include <algorithm>
 #include <cmath>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -108,29 +112,63 @@ __host__ inline unsigned tile_group_blocks(int nbx, int ngy) {
 // successor of the largest float <= the last edge, so for every float mu these
 // comparisons equal mu_bin_index_full's FP64 comparisons
 // (compute_delta_mu2_dmumu_full.jl:308-318), right-closed last bin included.
+// 128-bit two's-complement fixed point in units of 2^-80: {lo, hi} words.
+// A bin's sum over a whole call stays below 2^47 in magnitude (|dmu| <= 2,
+// dmu^2 <= 4, so any call under 3.5e13 pair visits per bin), which leaves 80
+// fraction bits; each converted run sum is truncated by less than 2^-80.
+constexpr double kFixUnit = 0x1p-80;
+
+// v -> {lo, hi}: magnitude truncated toward zero, then negated if v < 0.
+__device__ __forceinline__ void fix_from_double(double v, unsigned long long &lo, unsigned long long &hi) {
+    const double a = fabs(v) * 0x1p16;             // |v| 2^80 / 2^64
+    const double h = floor(a);
+    const unsigned long long uh = static_cast<unsigned long long>(h);
+    const unsigned long long ul = static_cast<unsigned long long>((a - h) * 0x1p64);  // a - h exact, in [0, 1)
+    if (v < 0.0) {
+        lo = ~ul + 1ull;
+        hi = ~uh + (ul == 0ull ? 1ull : 0ull);
+    } else {
+        lo = ul;
+        hi = uh;
+    }
+}
+
+// *p += {lo, hi} (p[0] = lo word, p[1] = hi word) with atomics: the carry out
+// of the low word is known exactly from its old value, so concurrent adds in
+// any order leave the same 128-bit sum.
+__device__ __forceinline__ void fix_atomic_add(unsigned long long *p, unsigned long long lo, unsigned long long hi) {
+    const unsigned long long old = atomicAdd(&p[0], lo);
+    const unsigned long long carry = old + lo < old ? 1ull : 0ull;
+    const unsigned long long h = hi + carry;
+    if (h) atomicAdd(&p[1], h);
+}
+
+__host__ inline double fix_to_double(unsigned long long lo, unsigned long long hi) {
+    return static_cast<double>(static_cast<long long>(hi)) * 0x1p-16 + static_cast<double>(lo) * kFixUnit;
+}
+
+// Bin storage of one lag: count (u64), sum dmu and sum dmu^2 (128-bit fixed
+// point, two u64 words each) = 5 words per bin; a lag's bins are
+// [count x nbins][s1 x 2 nbins][s2 x 2 nbins].
+constexpr int kBinWords = 5;
+
 __global__ void __launch_bounds__(kTPB)
 k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, int64_t sp,
         const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags,
         const float *__restrict__ lo_g, int nbins, float hi_last, int bin_abs, int injection,
-        unsigned long long *__restrict__ g_cnt, double *__restrict__ g_s1, double *__restrict__ g_s2,
-        double *__restrict__ part_d, long long *__restrict__ part_i, int nbx, int ngy,
-        const int32_t *__restrict__ fin, int lane32) {
+        unsigned long long *__restrict__ g_bins, double *__restrict__ part_d, long long *__restrict__ part_i,
+        int nbx, int ngy, const int32_t *__restrict__ fin, int lane32) {
     const TileGroup tg = tile_group(ngy);
     if (tg.tile >= nbx) return;  // whole block: before any barrier
     extern __shared__ unsigned char smem[];
-    unsigned long long *b_cnt = reinterpret_cast<unsigned long long *>(smem);
-    double *b_s1 = reinterpret_cast<double *>(b_cnt + kGmu * nbins);
-    double *b_s2 = b_s1 + kGmu * nbins;
-    float *lo = reinterpret_cast<float *>(b_s2 + kGmu * nbins);
+    // LDS bins of the group's kGmu lags, laid out as g_bins per lag
+    unsigned long long *b_bins = reinterpret_cast<unsigned long long *>(smem);
+    float *lo = reinterpret_cast<float *>(b_bins + kGmu * kBinWords * nbins);
     __shared__ Welford sh_w[kTPB / 64];
     __shared__ double sh_d[kTPB / 64];
     __shared__ long long sh_i[kTPB / 64];
 
-    for (int i = threadIdx.x; i < kGmu * nbins; i += kTPB) {
-        b_cnt[i] = 0;
-        b_s1[i] = 0.0;
-        b_s2[i] = 0.0;
-    }
+    for (int i = threadIdx.x; i < kGmu * kBinWords * nbins; i += kTPB) b_bins[i] = 0ull;
     for (int i = threadIdx.x; i < nbins; i += kTPB) lo[i] = lo_g[i];
     __syncthreads();
 
@@ -169,9 +207,13 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         for (int g = 0; g < kGmu; ++g) {
             rc[g] += run;
             if (cur_bin >= 0 && rc[g]) {
-                atomicAdd(&b_cnt[g * nbins + cur_bin], static_cast<unsigned long long>(rc[g]));
-                atomicAdd(&b_s1[g * nbins + cur_bin], r1[g]);
-                atomicAdd(&b_s2[g * nbins + cur_bin], r2[g]);
+                unsigned long long *bl = b_bins + g * kBinWords * nbins;
+                atomicAdd(&bl[cur_bin], static_cast<unsigned long long>(rc[g]));
+                unsigned long long l, h;
+                fix_from_double(r1[g], l, h);
+                fix_atomic_add(&bl[nbins + 2 * cur_bin], l, h);
+                fix_from_double(r2[g], l, h);
+                fix_atomic_add(&bl[3 * nbins + 2 * cur_bin], l, h);
             }
             pc[g] += rc[g];
             ps[g] += r2[g];
@@ -360,12 +402,19 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     for (int i = threadIdx.x; i < kGmu * nbins; i += kTPB) {
         const int g = i / nbins, b = i % nbins;
         const int li = gy * kGmu + g;
-        if (li < nlags && b_cnt[i]) {
-            atomicAdd(&g_cnt[static_cast<int64_t>(li) * nbins + b], b_cnt[i]);
-            atomicAdd(&g_s1[static_cast<int64_t>(li) * nbins + b], b_s1[i]);
-            atomicAdd(&g_s2[static_cast<int64_t>(li) * nbins + b], b_s2[i]);
+        const unsigned long long *bl = b_bins + g * kBinWords * nbins;
+        if (li < nlags && bl[b]) {
+            unsigned long long *gl = g_bins + static_cast<int64_t>(li) * kBinWords * nbins;
+            atomicAdd(&gl[b], bl[b]);
+            fix_atomic_add(&gl[nbins + 2 * b], bl[nbins + 2 * b], bl[nbins + 2 * b + 1]);
+            fix_atomic_add(&gl[3 * nbins + 2 * b], bl[3 * nbins + 2 * b], bl[3 * nbins + 2 * b + 1]);
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_fill_u64(unsigned long long *__restrict__ a, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0ull;
 }
 
 // fin[j] = index of the first non-finite sample of selected particle j (nsave
@@ -833,6 +882,144 @@ void combine_welford_host(int64_t *counts, double *means, double *m2s, int64_t l
     counts[li] = cc;
 }
 
+// ------------------------------------------------ D_mumu call plumbing ---
+//
+// One call: validate and derive the bin thresholds (dmumu_plan), place the
+// device work areas (dmumu_carve), run the kernels (dmumu_enqueue), copy the
+// results back (dmumu_download) and merge them into the caller's
+// accumulators (dmumu_merge).
+
+constexpr int64_t kMaxLagsPerCall = 1 << 20;
+
+struct DmumuPlan {
+    int64_t nsave, n_sel, ss, sp, nlags, nbins;
+    int64_t max_col;
+    size_t mu_count;         // extent of mu the kernels may touch (elements)
+    int nbx, ngy, lane32;
+    const int64_t *lags;     // caller's host lag steps
+    float lo[kMaxBins];      // float thresholds equivalent to the FP64 edges
+    float hi_last;
+    size_t bins_words, parts;
+    size_t work_bytes;       // bins + partials + merged partials
+};
+
+struct DmumuDev {
+    const float *mu;
+    const int64_t *idx, *lags;
+    const float *lo;
+    unsigned long long *bins;
+    double *part_d, *out_d;
+    long long *part_i, *out_i;
+    int32_t *fin;
+};
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t{255}; }
+
+DmumuPlan dmumu_plan(pt_ctx *ctx, const float *mu, int64_t nsave, int64_t n_sel, int64_t stride_s,
+                     int64_t stride_p, const int64_t *particle_index, const int64_t *lag_steps, int64_t nlags,
+                     const double *mu_edges, int64_t n_bins, int start_mode) {
+    PT_REQUIRE(ctx && mu && lag_steps && mu_edges, "pt_dmumu_accumulate: null argument");
+    PT_REQUIRE(nsave >= 1 && n_sel >= 0 && nlags >= 1, "pt_dmumu_accumulate: empty input");
+    PT_REQUIRE(n_bins >= 1 && n_bins <= kMaxBins, "pt_dmumu_accumulate: n_bins must be in [1, 256]");
+    PT_REQUIRE(start_mode == 0 || start_mode == 1, "Unknown dmumu_start_mode");
+    PT_REQUIRE(nlags <= kMaxLagsPerCall, "pt_dmumu_accumulate: too many lags");
+    for (int64_t q = 0; q < nlags; ++q) PT_REQUIRE(lag_steps[q] >= 1, "pt_dmumu_accumulate: lag steps must be >= 1");
+    for (int64_t b = 0; b < n_bins; ++b)
+        PT_REQUIRE(mu_edges[b] < mu_edges[b + 1], "pt_dmumu_accumulate: edges must increase");
+    DmumuPlan P;
+    P.nsave = nsave;
+    P.n_sel = n_sel;
+    P.ss = stride_s;
+    P.sp = stride_p;
+    P.nlags = nlags;
+    P.nbins = n_bins;
+    P.lags = lag_steps;
+    for (int64_t b = 0; b < n_bins; ++b) P.lo[b] = ceil_to_float(mu_edges[b]);
+    P.hi_last = std::nextafter(floor_to_float(mu_edges[n_bins]), INFINITY);
+    // extent of the mu array the kernels may touch, for host staging
+    P.max_col = n_sel > 0 ? n_sel - 1 : 0;
+    if (particle_index) {
+        P.max_col = 0;
+        for (int64_t j = 0; j < n_sel; ++j) {
+            PT_REQUIRE(particle_index[j] >= 0, "pt_dmumu_accumulate: negative particle index");
+            P.max_col = std::max(P.max_col, particle_index[j]);
+        }
+    }
+    P.mu_count = static_cast<size_t>((nsave - 1) * stride_s + P.max_col * stride_p + 1);
+    P.nbx = static_cast<int>((n_sel + kTPB - 1) / kTPB);
+    P.ngy = static_cast<int>((nlags + kGmu - 1) / kGmu);
+    // the fast path's 32-bit lane offsets (bytes) cover every particle
+    P.lane32 = P.max_col * stride_p < (int64_t{1} << 30) && nsave < (int64_t{1} << 31) ? 1 : 0;
+    const size_t nl = static_cast<size_t>(nlags), nb = static_cast<size_t>(n_bins);
+    P.bins_words = nl * nb * kBinWords;
+    P.parts = nl * static_cast<size_t>(P.nbx);
+    P.work_bytes = align256(P.bins_words * 8) + align256(P.parts * 24) + align256(P.parts * 16) +
+                   align256(nl * 24) + align256(nl * 16);
+    return P;
+}
+
+void dmumu_carve(const DmumuPlan &P, char *base, DmumuDev &D) {
+    const size_t nl = static_cast<size_t>(P.nlags);
+    size_t off = 0;
+    D.bins = reinterpret_cast<unsigned long long *>(base + off);
+    off += align256(P.bins_words * 8);
+    D.part_d = reinterpret_cast<double *>(base + off);
+    off += align256(P.parts * 24);
+    D.part_i = reinterpret_cast<long long *>(base + off);
+    off += align256(P.parts * 16);
+    D.out_d = reinterpret_cast<double *>(base + off);
+    off += align256(nl * 24);
+    D.out_i = reinterpret_cast<long long *>(base + off);
+}
+
+void dmumu_enqueue(const DmumuPlan &P, const DmumuDev &D, int bin_abs, int start_mode, hipStream_t st) {
+    // the bins start at zero; a fill kernel rather than a memset node keeps
+    // the call valid inside a captured graph
+    const int64_t nw = static_cast<int64_t>(P.bins_words);
+    k_fill_u64<<<static_cast<unsigned>((nw + 255) / 256), 256, 0, st>>>(D.bins, nw);
+    PT_HIP_CHECK(hipGetLastError());
+    launch_finite_prefix(D.mu, P.nsave, P.n_sel, P.ss, P.sp, D.idx, D.fin, st);
+    const size_t shm = static_cast<size_t>(kGmu) * static_cast<size_t>(P.nbins) * kBinWords * 8 +
+                       static_cast<size_t>(P.nbins) * 4;
+    k_dmumu<<<tile_group_blocks(P.nbx, P.ngy), kTPB, shm, st>>>(
+        D.mu, P.nsave, P.n_sel, P.ss, P.sp, D.idx, D.lags, static_cast<int>(P.nlags), D.lo,
+        static_cast<int>(P.nbins), P.hi_last, bin_abs ? 1 : 0, start_mode, D.bins, D.part_d, D.part_i, P.nbx,
+        P.ngy, D.fin, P.lane32);
+    PT_HIP_CHECK(hipGetLastError());
+    k_merge_partials<<<static_cast<unsigned>(P.nlags), kTPB, 0, st>>>(D.part_d, D.part_i,
+                                                                      static_cast<int>(P.nlags), P.nbx, D.out_d,
+                                                                      D.out_i);
+    PT_HIP_CHECK(hipGetLastError());
+}
+
+void dmumu_download(const DmumuPlan &P, const DmumuDev &D, unsigned long long *hb, double *hd, long long *hi,
+                    hipStream_t st) {
+    const size_t nl = static_cast<size_t>(P.nlags);
+    PT_HIP_CHECK(hipMemcpyAsync(hb, D.bins, P.bins_words * 8, hipMemcpyDeviceToHost, st));
+    PT_HIP_CHECK(hipMemcpyAsync(hd, D.out_d, nl * 24, hipMemcpyDeviceToHost, st));
+    PT_HIP_CHECK(hipMemcpyAsync(hi, D.out_i, nl * 16, hipMemcpyDeviceToHost, st));
+    PT_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void dmumu_merge(const DmumuPlan &P, const unsigned long long *hb, const double *hd, const long long *hi,
+                 int64_t *particle_counts, double *particle_means, double *particle_m2s, double *pair_sum_squares,
+                 int64_t *pair_counts, int64_t *dmumu_counts, double *dmumu_sum_delta, double *dmumu_sum_delta2) {
+    const size_t nb = static_cast<size_t>(P.nbins), nl = static_cast<size_t>(P.nlags);
+    for (size_t li = 0; li < nl; ++li) {
+        if (P.lags[li] >= P.nsave) continue;  // the reference skips lags beyond the series
+        const unsigned long long *bl = hb + li * nb * kBinWords;
+        for (size_t b = 0; b < nb; ++b) {
+            dmumu_counts[b + nb * li] += static_cast<int64_t>(bl[b]);
+            dmumu_sum_delta[b + nb * li] += fix_to_double(bl[nb + 2 * b], bl[nb + 2 * b + 1]);
+            dmumu_sum_delta2[b + nb * li] += fix_to_double(bl[3 * nb + 2 * b], bl[3 * nb + 2 * b + 1]);
+        }
+        combine_welford_host(particle_counts, particle_means, particle_m2s, static_cast<int64_t>(li), hi[li * 2],
+                             hd[li * 3], hd[li * 3 + 1]);
+        pair_sum_squares[li] += hd[li * 3 + 2];
+        pair_counts[li] += hi[li * 2 + 1];
+    }
+}
+
 }  // namespace
 }  // namespace pt
 
@@ -912,93 +1099,30 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
                         double *pair_sum_squares, int64_t *pair_counts, int64_t *dmumu_counts,
                         double *dmumu_sum_delta, double *dmumu_sum_delta2) {
     return guarded([&] {
-        PT_REQUIRE(ctx && mu && lag_steps && mu_edges, "pt_dmumu_accumulate: null argument");
+        DmumuPlan P = dmumu_plan(ctx, mu, nsave, n_sel, stride_s, stride_p, particle_index, lag_steps, nlags,
+                                 mu_edges, n_bins, start_mode);
         PT_REQUIRE(particle_counts && particle_means && particle_m2s && pair_sum_squares && pair_counts &&
                        dmumu_counts && dmumu_sum_delta && dmumu_sum_delta2,
                    "pt_dmumu_accumulate: null accumulator");
-        PT_REQUIRE(nsave >= 1 && n_sel >= 0 && nlags >= 1, "pt_dmumu_accumulate: empty input");
-        PT_REQUIRE(n_bins >= 1 && n_bins <= kMaxBins, "pt_dmumu_accumulate: n_bins must be in [1, 256]");
-        PT_REQUIRE(start_mode == 0 || start_mode == 1, "Unknown dmumu_start_mode");
-        PT_REQUIRE(nlags <= (1 << 20), "pt_dmumu_accumulate: too many lags");
-        std::vector<int64_t> lags(lag_steps, lag_steps + nlags);
-        for (int64_t l : lags) PT_REQUIRE(l >= 1, "pt_dmumu_accumulate: lag steps must be >= 1");
-        for (int64_t b = 0; b < n_bins; ++b)
-            PT_REQUIRE(mu_edges[b] < mu_edges[b + 1], "pt_dmumu_accumulate: edges must increase");
         if (n_sel == 0) return;
         PT_HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t st = ctx->stream;
-
-        // float thresholds equivalent to the FP64 edge comparisons
-        std::vector<float> lo(static_cast<size_t>(n_bins));
-        for (int64_t b = 0; b < n_bins; ++b) lo[b] = ceil_to_float(mu_edges[b]);
-        const float hi_last = std::nextafter(floor_to_float(mu_edges[n_bins]), INFINITY);
-
-        // Extent of the mu array the kernel may touch, for host staging.
-        int64_t max_col = n_sel - 1;
-        if (particle_index) {
-            max_col = 0;
-            for (int64_t j = 0; j < n_sel; ++j) {
-                PT_REQUIRE(particle_index[j] >= 0, "pt_dmumu_accumulate: negative particle index");
-                max_col = std::max(max_col, particle_index[j]);
-            }
-        }
-        const size_t mu_count = static_cast<size_t>((nsave - 1) * stride_s + max_col * stride_p + 1);
-        DevBuf hmu(ctx), hidx(ctx), hlag(ctx), hlo(ctx), hout(ctx);
-        const float *dmu = to_device(mu, mu_on_device, mu_count, hmu, st);
-        const int64_t *didx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
-        const int64_t *dlag = to_device(lags.data(), 0, lags.size(), hlag, st);
-        const float *dlo = to_device(lo.data(), 0, lo.size(), hlo, st);
-
-        const int nbx = static_cast<int>((n_sel + kTPB - 1) / kTPB);
-        const int ngy = static_cast<int>((nlags + kGmu - 1) / kGmu);
+        DevBuf hmu(ctx), hidx(ctx), hlag(ctx), hlo(ctx), hout(ctx), hfin(ctx);
+        DmumuDev D;
+        D.mu = to_device(mu, mu_on_device, P.mu_count, hmu, st);
+        D.idx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
+        D.lags = to_device(P.lags, 0, static_cast<size_t>(nlags), hlag, st);
+        D.lo = to_device(P.lo, 0, static_cast<size_t>(n_bins), hlo, st);
+        dmumu_carve(P, static_cast<char *>(hout.alloc(P.work_bytes)), D);
+        D.fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(n_sel) * 4));
+        dmumu_enqueue(P, D, bin_abs, start_mode, st);
         const size_t nb = static_cast<size_t>(n_bins), nl = static_cast<size_t>(nlags);
-        const size_t parts = nl * static_cast<size_t>(nbx);
-        // device layout: counts u64 [nlags][nbins], s1, s2, part_d, part_i, out_d, out_i
-        const size_t bytes = nl * nb * 24 + parts * (24 + 16) + nl * (24 + 16);
-        char *base = static_cast<char *>(hout.alloc(bytes));
-        auto *g_cnt = reinterpret_cast<unsigned long long *>(base);
-        auto *g_s1 = reinterpret_cast<double *>(g_cnt + nl * nb);
-        auto *g_s2 = g_s1 + nl * nb;
-        auto *part_d = g_s2 + nl * nb;
-        auto *part_i = reinterpret_cast<long long *>(part_d + parts * 3);
-        auto *out_d = reinterpret_cast<double *>(part_i + parts * 2);
-        auto *out_i = reinterpret_cast<long long *>(out_d + nl * 3);
-        PT_HIP_CHECK(hipMemsetAsync(base, 0, nl * nb * 24, st));
-
-        DevBuf hfin(ctx);
-        auto *fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(n_sel) * 4));
-        launch_finite_prefix(dmu, nsave, n_sel, stride_s, stride_p, didx, fin, st);
-        // the fast path's 32-bit lane offsets (bytes) cover every particle
-        const int lane32 = max_col * stride_p < (int64_t{1} << 30) && nsave < (int64_t{1} << 31) ? 1 : 0;
-        const size_t shm = static_cast<size_t>(kGmu) * nb * 24 + nb * 4;
-        k_dmumu<<<tile_group_blocks(nbx, ngy), kTPB, shm, st>>>(
-            dmu, nsave, n_sel, stride_s, stride_p, didx, dlag, static_cast<int>(nlags), dlo, static_cast<int>(n_bins),
-            hi_last, bin_abs ? 1 : 0, start_mode, g_cnt, g_s1, g_s2, part_d, part_i, nbx, ngy, fin, lane32);
-        PT_HIP_CHECK(hipGetLastError());
-        k_merge_partials<<<static_cast<unsigned>(nlags), kTPB, 0, st>>>(part_d, part_i, static_cast<int>(nlags), nbx,
-                                                                        out_d, out_i);
-        PT_HIP_CHECK(hipGetLastError());
-        std::vector<unsigned long long> hc(nl * nb);
-        std::vector<double> h1(nl * nb), h2(nl * nb), hd(nl * 3);
+        std::vector<unsigned long long> hb(nl * nb * kBinWords);
+        std::vector<double> hd(nl * 3);
         std::vector<long long> hi(nl * 2);
-        PT_HIP_CHECK(hipMemcpyAsync(hc.data(), g_cnt, nl * nb * 8, hipMemcpyDeviceToHost, st));
-        PT_HIP_CHECK(hipMemcpyAsync(h1.data(), g_s1, nl * nb * 8, hipMemcpyDeviceToHost, st));
-        PT_HIP_CHECK(hipMemcpyAsync(h2.data(), g_s2, nl * nb * 8, hipMemcpyDeviceToHost, st));
-        PT_HIP_CHECK(hipMemcpyAsync(hd.data(), out_d, nl * 3 * 8, hipMemcpyDeviceToHost, st));
-        PT_HIP_CHECK(hipMemcpyAsync(hi.data(), out_i, nl * 2 * 8, hipMemcpyDeviceToHost, st));
-        PT_HIP_CHECK(hipStreamSynchronize(st));
-        for (size_t li = 0; li < nl; ++li) {
-            if (lags[li] >= nsave) continue;  // the reference skips lags beyond the series
-            for (size_t b = 0; b < nb; ++b) {
-                dmumu_counts[b + nb * li] += static_cast<int64_t>(hc[li * nb + b]);
-                dmumu_sum_delta[b + nb * li] += h1[li * nb + b];
-                dmumu_sum_delta2[b + nb * li] += h2[li * nb + b];
-            }
-            combine_welford_host(particle_counts, particle_means, particle_m2s, static_cast<int64_t>(li), hi[li * 2],
-                                 hd[li * 3], hd[li * 3 + 1]);
-            pair_sum_squares[li] += hd[li * 3 + 2];
-            pair_counts[li] += hi[li * 2 + 1];
-        }
+        dmumu_download(P, D, hb.data(), hd.data(), hi.data(), st);
+        dmumu_merge(P, hb.data(), hd.data(), hi.data(), particle_counts, particle_means, particle_m2s,
+                    pair_sum_squares, pair_counts, dmumu_counts, dmumu_sum_delta, dmumu_sum_delta2);
     });
 }
 

@@ -29,6 +29,8 @@ import math
 
 import numpy as np
 
+from .time_units import PipelineError
+
 UGAUSS_TO_T = 1e-10
 KMPS_TO_MPS = 1e3
 PC_TO_M = 3.085677581e16
@@ -77,8 +79,8 @@ def load_static_fields(cfg, dtype=np.float64, arrays=None):
                     raise ValueError(f"{k}: expected a 3-D field dataset, got shape {ds.shape}")
                 count = tuple(ds.shape)
                 if subset:
-                    nx, ny, nz = subset
-                    count = (min(nz, count[0]), min(ny, count[1]), min(nx, count[2]))
+                    nx, ny, nz = _check_subset(subset, (count[2], count[1], count[0]), k)
+                    count = (nz, ny, nx)
                 raw.append(ds.read_box((0, 0, 0), count).transpose(2, 1, 0))
     elif arrays is None:
         with np.load(cfg["file"], allow_pickle=False) as z:
@@ -89,13 +91,25 @@ def load_static_fields(cfg, dtype=np.float64, arrays=None):
     for i, a in enumerate(raw):
         a = np.asarray(a)
         if subset:
-            nx, ny, nz = subset
+            nx, ny, nz = _check_subset(subset, a.shape, (*bp, *vp)[i])
             a = a[:nx, :ny, :nz]
         a = np.asfortranarray(a, dtype=dtype).copy(order="F")
         scale = UGAUSS_TO_T if i < 3 else float(cfg.get("velocity_unit_in_m_per_s", KMPS_TO_MPS))
         a *= dtype(scale)
         out.append(a)
     return tuple(out)
+
+
+def _check_subset(subset, shape, name):
+    """field_subset (nx,ny,nz) against a dataset's Julia shape: the reference
+    slices data[1:nx,1:ny,1:nz] (phase_space_gpu_runner.jl:100-104), which
+    raises a BoundsError when the subset is larger than the dataset."""
+    sub = tuple(int(v) for v in subset)
+    if len(sub) != 3 or min(sub) < 1:
+        raise PipelineError(f"field_subset must be three positive sizes, got {tuple(subset)}")
+    if any(a > b for a, b in zip(sub, shape)):
+        raise PipelineError(f"field_subset {sub} exceeds the shape {tuple(shape)} of dataset {name}")
+    return sub
 
 
 def _wavenumbers(M: int, xp, device=None):

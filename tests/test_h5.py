@@ -355,3 +355,10 @@ def test_field_file_loader_reads_julia_arrays(tmp_path):
         assert g[3, 1, 2] == (1000.0 * c + 312) * (1e-10 if c < 3 else 1e3)
     sub = fields.load_static_fields(dict(cfg, field_subset=(3, 2, 2)))
     assert all(np.array_equal(s, w[:3, :2, :2]) for s, w in zip(sub, want))
+    # a subset larger than the dataset is an error, as data[1:nx,1:ny,1:nz] is
+    # in the reference (phase_space_gpu_runner.jl:100-104), from a file or arrays
+    from particle_transport_amd.time_units import PipelineError
+    with pytest.raises(PipelineError, match="exceeds"):
+        fields.load_static_fields(dict(cfg, field_subset=(3, 5, 2)))
+    with pytest.raises(PipelineError, match="exceeds"):
+        fields.load_static_fields(dict(cfg, field_subset=(6, 2, 2)), arrays=jl)
