@@ -255,6 +255,26 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
                         double *pair_sum_squares, int64_t *pair_counts, int64_t *dmumu_counts,
                         double *dmumu_sum_delta, double *dmumu_sum_delta2);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the pass of pt_dmumu_accumulate over a mu cache already
+ * in device memory (mu must be a device pointer), with every device work area
+ * in a fixed 32 MiB array the library holds on each GPU it runs on (looked up
+ * at pt_ctx_create) and its host staging in a fixed static area.  It calls no
+ * allocator.  Same arguments, accumulators and results as
+ * pt_dmumu_accumulate (bitwise).  Limits: at most 4096 lags and 65536
+ * (lag, bin) cells, and about 40 nlags (n_bins + ceil(n_sel / 256) + 1) + 4 n_sel
+ * (+ 8 n_sel with particle_index) bytes within the 32 MiB; beyond them the
+ * call fails with PT_ERR_MEMORY before any kernel runs.  Calls that overlap on
+ * one GPU share the array and race; calls from several host threads are
+ * serialised on the host staging area. */
+int pt_dmumu_accumulate_noalloc(pt_ctx *ctx, const float *mu, int64_t nsave, int64_t n_sel, int64_t stride_s,
+                                int64_t stride_p, const int64_t *particle_index, const int64_t *lag_steps,
+                                int64_t nlags, const double *mu_edges, int64_t n_bins, int start_mode, int bin_abs,
+                                int64_t *particle_counts, double *particle_means, double *particle_m2s,
+                                double *pair_sum_squares, int64_t *pair_counts, int64_t *dmumu_counts,
+                                double *dmumu_sum_delta, double *dmumu_sum_delta2);
+
 /* Replaces process_momenta_chunk_dpp_cpu! (compute_dpp_full.jl:247-288).
  * momenta: float32, Julia (total,3,nsave) (C [s][3][total]) with the chunk's
  * selected columns given by particle_index (or 0..n_sel-1).  |p| in FP64

@@ -204,6 +204,9 @@ void launch_inject(const pt_field *f, const pt_inject_params &ip, int64_t first,
 // The reference's checks on injection settings (pt_runtime.hip); throws.
 void check_inject_params(const pt_inject_params &ip, const pt_field *field);
 
+// Looks up (once per device, with `device` current) the device address of
+// the fixed scratch array that pt_dmumu_accumulate_noalloc works in.
+void noalloc_scratch_init(int device);
 // Device scratch owned by the context, grown on demand (never shrinks).
 void *ctx_scratch(pt_ctx *ctx, size_t bytes);
 // Per-call device buffers cached by the context: pool_get returns a buffer of

@@ -536,6 +536,7 @@ int pt_ctx_create(int device, pt_ctx **out) {
         PT_HIP_CHECK(hipGetDeviceCount(&count));
         PT_REQUIRE(device >= 0 && device < count, "pt_ctx_create: device ordinal out of range");
         PT_HIP_CHECK(hipSetDevice(device));
+        noalloc_scratch_init(device);
         auto *c = new pt_ctx();
         c->device = device;
         // Default-priority, non-blocking stream owned by the context.

@@ -16,6 +16,7 @@
 //  exact; the remaining sums are FP64.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -887,7 +888,9 @@ void combine_welford_host(int64_t *counts, double *means, double *m2s, int64_t l
 // One call: validate and derive the bin thresholds (dmumu_plan), place the
 // device work areas (dmumu_carve), run the kernels (dmumu_enqueue), copy the
 // results back (dmumu_download) and merge them into the caller's
-// accumulators (dmumu_merge).
+// accumulators (dmumu_merge).  pt_dmumu_accumulate takes its device memory
+// from the context's pool; pt_dmumu_accumulate_noalloc from the library's
+// fixed scratch array.
 
 constexpr int64_t kMaxLagsPerCall = 1 << 20;
 
@@ -1020,7 +1023,55 @@ void dmumu_merge(const DmumuPlan &P, const unsigned long long *hb, const double 
     }
 }
 
+// ------------------------------------------ fixed scratch (no allocation) ---
+//
+// pt_dmumu_accumulate_noalloc places every device work area in this array
+// (one copy per GPU, held until the process ends) and its host staging in a
+// static host area, so it calls no allocator.  Host code must never pass the
+// array's name to a kernel or copy: it is a host-side stand-in.  Its device
+// address is looked up once per device (hipGetSymbolAddress) and kept in
+// g_noalloc_dev.
+constexpr size_t kNoAllocScratchBytes = size_t{32} << 20;
+constexpr int64_t kNoAllocMaxLags = 4096;
+constexpr int64_t kNoAllocMaxLagBins = 65536;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+void *g_noalloc_dev[kMaxDevices] = {};
+std::mutex g_noalloc_dev_mutex;
+
+struct NoAllocHost {
+    unsigned long long bins[kNoAllocMaxLagBins * kBinWords];
+    double out_d[kNoAllocMaxLags * 3];
+    long long out_i[kNoAllocMaxLags * 2];
+};
+NoAllocHost g_noalloc_host;
+std::mutex g_noalloc_host_mutex;
+
+// Device address of `scratch` on `device` (must be the current device).
+void *noalloc_scratch(int device) {
+    PT_REQUIRE(device >= 0 && device < kMaxDevices, "pt_dmumu_accumulate_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_noalloc_dev_mutex);
+    if (!g_noalloc_dev[device]) {
+        void *ptr = nullptr;
+        PT_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        PT_REQUIRE(ptr != nullptr, "pt_dmumu_accumulate_noalloc: no device address for the scratch array");
+        g_noalloc_dev[device] = ptr;
+    }
+    return g_noalloc_dev[device];
+}
+
+size_t noalloc_bytes(const DmumuPlan &P, bool with_index) {
+    return align256(P.work_bytes) + align256(static_cast<size_t>(P.n_sel) * 4) +
+           align256(static_cast<size_t>(P.nlags) * 8) + align256(static_cast<size_t>(P.nbins) * 4) +
+           (with_index ? align256(static_cast<size_t>(P.n_sel) * 8) : 0);
+}
+
 }  // namespace
+
+void noalloc_scratch_init(int device) { noalloc_scratch(device); }
+
 }  // namespace pt
 
 using namespace pt;
@@ -1123,6 +1174,61 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
         dmumu_download(P, D, hb.data(), hd.data(), hi.data(), st);
         dmumu_merge(P, hb.data(), hd.data(), hi.data(), particle_counts, particle_means, particle_m2s,
                     pair_sum_squares, pair_counts, dmumu_counts, dmumu_sum_delta, dmumu_sum_delta2);
+    });
+}
+
+int pt_dmumu_accumulate_noalloc(pt_ctx *ctx, const float *mu, int64_t nsave, int64_t n_sel, int64_t stride_s,
+                                int64_t stride_p, const int64_t *particle_index, const int64_t *lag_steps,
+                                int64_t nlags, const double *mu_edges, int64_t n_bins, int start_mode, int bin_abs,
+                                int64_t *particle_counts, double *particle_means, double *particle_m2s,
+                                double *pair_sum_squares, int64_t *pair_counts, int64_t *dmumu_counts,
+                                double *dmumu_sum_delta, double *dmumu_sum_delta2) {
+    return guarded([&] {
+        DmumuPlan P = dmumu_plan(ctx, mu, nsave, n_sel, stride_s, stride_p, particle_index, lag_steps, nlags,
+                                 mu_edges, n_bins, start_mode);
+        PT_REQUIRE(particle_counts && particle_means && particle_m2s && pair_sum_squares && pair_counts &&
+                       dmumu_counts && dmumu_sum_delta && dmumu_sum_delta2,
+                   "pt_dmumu_accumulate_noalloc: null accumulator");
+        PT_REQUIRE(nlags <= kNoAllocMaxLags && nlags * n_bins <= kNoAllocMaxLagBins,
+                   "pt_dmumu_accumulate_noalloc: at most 4096 lags and 65536 (lag, bin) cells");
+        const size_t need = noalloc_bytes(P, particle_index != nullptr);
+        if (need > kNoAllocScratchBytes)
+            throw Error{PT_ERR_MEMORY, "pt_dmumu_accumulate_noalloc: the call needs " + std::to_string(need) +
+                                           " bytes of scratch, more than the library's fixed " +
+                                           std::to_string(kNoAllocScratchBytes) +
+                                           " (fewer particles or lags per call)"};
+        if (n_sel == 0) return;
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        char *base = static_cast<char *>(noalloc_scratch(ctx->device));
+        // carve: [work][fin][lags][lo][index]; every piece 256-byte aligned
+        DmumuDev D;
+        D.mu = mu;
+        dmumu_carve(P, base, D);
+        size_t off = align256(P.work_bytes);
+        D.fin = reinterpret_cast<int32_t *>(base + off);
+        off += align256(static_cast<size_t>(n_sel) * 4);
+        int64_t *dl = reinterpret_cast<int64_t *>(base + off);
+        off += align256(static_cast<size_t>(nlags) * 8);
+        float *dlo = reinterpret_cast<float *>(base + off);
+        off += align256(static_cast<size_t>(n_bins) * 4);
+        PT_HIP_CHECK(hipMemcpyAsync(dl, P.lags, static_cast<size_t>(nlags) * 8, hipMemcpyHostToDevice, st));
+        PT_HIP_CHECK(hipMemcpyAsync(dlo, P.lo, static_cast<size_t>(n_bins) * 4, hipMemcpyHostToDevice, st));
+        D.lags = dl;
+        D.lo = dlo;
+        D.idx = nullptr;
+        if (particle_index) {
+            int64_t *di = reinterpret_cast<int64_t *>(base + off);
+            PT_HIP_CHECK(hipMemcpyAsync(di, particle_index, static_cast<size_t>(n_sel) * 8, hipMemcpyHostToDevice, st));
+            D.idx = di;
+        }
+        // host staging is a fixed static area: calls from several threads serialise here
+        std::lock_guard<std::mutex> lock(g_noalloc_host_mutex);
+        dmumu_enqueue(P, D, bin_abs, start_mode, st);
+        dmumu_download(P, D, g_noalloc_host.bins, g_noalloc_host.out_d, g_noalloc_host.out_i, st);
+        dmumu_merge(P, g_noalloc_host.bins, g_noalloc_host.out_d, g_noalloc_host.out_i, particle_counts,
+                    particle_means, particle_m2s, pair_sum_squares, pair_counts, dmumu_counts, dmumu_sum_delta,
+                    dmumu_sum_delta2);
     });
 }
 
