@@ -146,6 +146,63 @@ def cpu_threads():
     return max(1, min(16, os.cpu_count() or 1))
 
 
+def host_cpu():
+    """The host's CPU model (lscpu "Model name", else /proc/cpuinfo) and the
+    number of logical CPUs this process may run on."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None and os.path.exists("/proc/cpuinfo"):
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "nproc": nproc}
+
+
+def cpu_plain_small(threads):
+    """The plain-method configuration run in full by the oracle (the
+    reference's FP64 step loop in C, advance_particles_kernel! with mu at
+    cadence): 64^3-node FP64 field (dB/B0 = 0.5, k = [1, 16], seed 42),
+    4096 protons at 1 GeV, 50 T_g0 at 100 steps/T_g0, mu every 10 steps."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    import oracle
+    from particle_transport_amd import fields
+    from particle_transport_amd.injection import M_P, Q_E
+    n_nodes, n, S = 64, 4096, 100
+    raw = fields.generate_turbulence(n_nodes, seed=42, kmax=16, dB_over_B0=0.5)["total"]
+    F = [np.asfortranarray(a) for a in fields.to_si(raw)]
+    L = fields.box_length_m(10.0)
+    g = oracle.make_geom(n_nodes, n_nodes, n_nodes, L)
+    x, p, gamma = make_particles(n, L, 1.0, seed=42)
+    B0 = oracle.mean_abs_b(F[0], F[1], F[2])
+    dt = (2 * math.pi / S) / (Q_E * B0 / (gamma * M_P))
+    steps = 50 * S + 1
+    bounds = np.linspace(0, n, threads + 1).astype(int)
+    parts = [(np.ascontiguousarray(x[:, a:b]), np.ascontiguousarray(p[:, a:b])) for a, b in zip(bounds, bounds[1:])]
+    oracle.lib()
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda xp: oracle.run(F, g, xp[0], xp[1], dt, 1, steps, steps, 10, True, 2), parts))
+    el = time.perf_counter() - t0
+    return {"value": n * (steps - 1) / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "seconds": el,
+            "sample": "the whole plain-method configuration: 64^3 FP64 field, 4096 p+ at 1 GeV, 50 T_g0 at "
+                      "100 steps/T_g0 (5000 pushes), mu every 10 steps, FP64 oracle"}
+
+
 def cpu_baseline(F32, L, x, p, dt, n_cpu, steps_cpu, threads=1):
     """The oracle's FP64 step loop (C restatement of advance_particles_kernel!,
     mu every 20 steps) on n_cpu particles, split into `threads` particle
@@ -546,6 +603,8 @@ def main():
     ap.add_argument("--mode-gp", type=int, default=10)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run the production kernel 3x per field format plus 1 GiB copies (for rocprofv3 --pmc)")
+    ap.add_argument("--stats-probe", action="store_true",
+                    help="only run the statistics legs (D_mumu, D_pp) on pusher-written series (for rocprofv3)")
     ap.add_argument("--formats", default=",".join(FORMATS),
                     help="comma-separated field formats to time (bf16_brick, f16_brick, f32_brick)")
     ap.add_argument("--sort-every", type=int, default=0,
@@ -590,6 +649,12 @@ def main():
     if args.pmc_probe:
         pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total, formats, args.sort_every)
         ctx.close()
+        return
+    if args.stats_probe:
+        B0 = ctx.mean_abs_b(F32[0], F32[1], F32[2])
+        res = stats_rates(ctx, F32, L, x, p, (2 * math.pi / S) / (Q_E * B0 / (gamma * M_P)), log, cpu=False)
+        ctx.close()
+        print(json.dumps(res), flush=True)
         return
 
     # The production pusher on each 16-bit brick format and on FP32 bricks.
@@ -663,12 +728,15 @@ def main():
         rate, el = cpu_baseline(F32, L, x, p, dt, n_cpu, args.cpu_steps, threads)
         log(f"[bench] CPU oracle on {args.cpu_particles} particles x {args.cpu_steps} steps, 1 thread")
         rate1, el1 = cpu_baseline(F32, L, x, p, dt, args.cpu_particles, args.cpu_steps, 1)
-        cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+        log(f"[bench] CPU oracle on the whole plain-method configuration, {threads} threads")
+        c1_cpu = cpu_plain_small(threads)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port", "host": host_cpu(),
                "sample": f"{n_cpu} particles x {args.cpu_steps} pushes of the same field and dt in {threads} host "
                          f"threads, FP64 oracle (C restatement of advance_particles_kernel!), mu every 20 steps, "
                          f"{el:.1f} s",
                "single_core": {"value": rate1, "cores": 1,
-                               "sample": f"{args.cpu_particles} particles x {args.cpu_steps} pushes, {el1:.1f} s"}}
+                               "sample": f"{args.cpu_particles} particles x {args.cpu_steps} pushes, {el1:.1f} s"},
+               "plain_method_config": c1_cpu}
     ctx.close()
     del F32
 

@@ -138,14 +138,17 @@ namespace pt {
 
 // Split-precision particle state (structure of arrays, n elements each):
 // double P2 = |p|^2 / (M_P c)^2, int32 cell[3], float off[3] in [0,1) cell
-// units, float P[3] = p / (M_P c).  44 bytes per particle.
+// units, float P[3] = p / (M_P c), float comp[3] = the rounding error the
+// offset's last additions left (position = cell + off + comp, compensated
+// summation).  56 bytes per particle.
 struct SplitState {
     double *P2;
     int32_t *cell[3];
     float *off[3];
     float *P[3];
+    float *comp[3];
 };
-constexpr size_t kSplitStateBytes = 44;
+constexpr size_t kSplitStateBytes = 56;
 // FP64 reference-structured state: x in metres, p in kg m/s.
 struct RefState {
     double *x[3];
@@ -161,6 +164,7 @@ inline SplitState split_state(pt_ens *e) {
     for (int c = 0; c < 3; ++c) s.cell[c] = reinterpret_cast<int32_t *>(b + c * n * 4);
     for (int c = 0; c < 3; ++c) s.off[c] = reinterpret_cast<float *>(b + (3 + c) * n * 4);
     for (int c = 0; c < 3; ++c) s.P[c] = reinterpret_cast<float *>(b + (6 + c) * n * 4);
+    for (int c = 0; c < 3; ++c) s.comp[c] = reinterpret_cast<float *>(b + (9 + c) * n * 4);
     return s;
 }
 inline RefState ref_state(pt_ens *e) {
@@ -226,20 +230,25 @@ __host__ __device__ inline int64_t brick_node_index(const int64_t nb[3], int64_t
     return (brick << 9) + (((k & 7) << 6) | ((j & 7) << 3) | (i & 7));
 }
 
-// Split coordinate of u = x / dx on an axis of nc cells: cell = floor(u) and
-// the FP32 offset u - cell (an offset that rounds up to 1 moves to the next
-// cell at offset 0).  A periodic axis holds the upper face x == L as the last
-// cell at offset 1: the particle samples the field there as trilinear_sample
-// clamps it (phase_space_gpu_runner.jl:247-276), and the first push that moves
-// it outward takes it past offset 1, where the periodic wrap moves it to cell
-// 0, as wrap_periodic_coordinate (:325-328) maps x > L to x - L.
-__host__ __device__ inline void split_coord(double u, int32_t nc, bool periodic, int32_t &cell, float &off) {
+// Split coordinate of u = x / dx on an axis of nc cells: cell = floor(u), the
+// FP32 offset u - cell (an offset that rounds up to 1 moves to the next cell
+// at offset 0) and comp, the part of u - cell the offset's rounding dropped.
+// A periodic axis holds the upper face x == L as the last cell at offset 1:
+// the particle samples the field there as trilinear_sample clamps it
+// (phase_space_gpu_runner.jl:247-276), and the first push that moves it
+// outward takes it past offset 1, where the periodic wrap moves it to cell 0,
+// as wrap_periodic_coordinate (:325-328) maps x > L to x - L.
+__host__ __device__ inline void split_coord(double u, int32_t nc, bool periodic, int32_t &cell, float &off,
+                                            float &comp) {
     const double fl = floor(u);
-    off = static_cast<float>(u - fl);
+    const double frac = u - fl;
+    off = static_cast<float>(frac);
     cell = static_cast<int32_t>(fl);
+    comp = static_cast<float>(frac - static_cast<double>(off));
     if (off >= 1.0f) {
         off = 0.0f;
         cell += 1;
+        comp = static_cast<float>(frac - 1.0);
     }
     if (periodic && cell == nc && off == 0.0f) {
         cell = nc - 1;

@@ -205,12 +205,14 @@ __device__ __forceinline__ float mu_f32(float px, float py, float pz, float bx, 
 // the reference's *_step copies (:351-356) or mu (pipeline :1349-1374).
 __device__ __forceinline__ void write_save(int out_kind, int out_f64, void *out_a, void *out_b,
                                            int64_t slot, int64_t n, int64_t p, bool alive,
-                                           const int32_t cell[3], const float off[3],
+                                           const int32_t cell[3], const float off[3], const float comp[3],
                                            const float U[3], const float bt[3],
                                            const SplitGeom &g, const Phys &ph) {
     if (out_kind == PT_OUT_PHASE) {
         for (int c = 0; c < 3; ++c) {
-            const double xv = alive ? (static_cast<double>(cell[c]) + static_cast<double>(off[c])) * g.d[c] : NAN;
+            const double xv = alive ? (static_cast<double>(cell[c]) + static_cast<double>(off[c]) +
+                                       static_cast<double>(comp[c])) * g.d[c]
+                                    : NAN;
             const double pv = alive ? static_cast<double>(U[c]) * ph.pout : NAN;
             const int64_t idx = (slot * 3 + c) * n + p;
             if (out_f64) {
@@ -560,7 +562,7 @@ __host__ inline Cadence make_cadence(int64_t si0, int K, int64_t total_steps, in
 // cell's coefficients in registers.  The coefficients are reloaded only after
 // the particle changes cell (move_axis reports it), so a push costs the
 // six polynomial evaluations and the Boris step.  save(slot, alive, cell,
-// off, U, P2, v) is called at every cadence step (before the push, as the
+// off, comp, U, P2, v) is called at every cadence step (before the push, as the
 // reference's *_step copies); die(si) when an open boundary removes the
 // particle.
 //
@@ -568,7 +570,8 @@ __host__ inline Cadence make_cadence(int64_t si0, int K, int64_t total_steps, in
 // alive = false): the coefficient reload is wave-cooperative (coop_reload)
 // and uses the wave's `lds` area.
 template <int STORE, bool CUBIC, class SaveFn, class DieFn>
-__device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[3], double &P2, bool &alive,
+__device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float comp[3], float U[3], double &P2,
+                                         bool &alive,
                                          const FieldView &fv, const SplitGeom &g, const Phys &ph,
                                          const Cadence &cad, int periodic, ReloadLds *lds, SaveFn save, DieFn die) {
     float a[6][8];
@@ -601,7 +604,7 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
 #pragma unroll
         for (int c = 0; c < 6; ++c) v[c] = eval_poly(a[c], off[0], off[1], off[2]);
         if (do_save) {
-            save(slot, live != 0, cell, off, U, P2, v);
+            save(slot, live != 0, cell, off, comp, U, P2, v);
             ++slot;
         }
         if (!push) continue;
@@ -614,14 +617,25 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float U[
         boris_split(U[0], U[1], U[2], P2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn, renorm);
         // all three offsets first; one rare branch for any cell change.  With
         // cubic cells (CUBIC: g.k equal on every axis) the step factor is one
-        // product, the same value each axis would compute.
+        // product, the same value each axis would compute.  The step is added
+        // with compensated summation (Fast2Sum): comp keeps what the FP32
+        // offset's rounding dropped and feeds it into the next step, so a
+        // step far below the offset's ulp (6e-8 of a cell near 1) still
+        // accumulates at its full value instead of rounding the same way push
+        // after push.
+        auto step_axis = [&](int c, float w) {
+            const float st = fmaf(U[c], w, comp[c]);
+            const float t = off[c] + st;
+            comp[c] = st - (t - off[c]);
+            off[c] = t;
+        };
         if constexpr (CUBIC) {
             const float w = g.k[0] * inv_gn;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) off[c] = fmaf(U[c], w, off[c]);
+            for (int c = 0; c < 3; ++c) step_axis(c, w);
         } else {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) off[c] = fmaf(U[c], g.k[c] * inv_gn, off[c]);
+            for (int c = 0; c < 3; ++c) step_axis(c, g.k[c] * inv_gn);
         }
         // every offset in [0, 1) <=> every bit pattern below that of 1.0f
         // (negative values, -0 and NaN compare above it and take the slow path)

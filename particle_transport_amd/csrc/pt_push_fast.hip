@@ -81,7 +81,12 @@ k_push_plain(SplitState s, uint8_t *__restrict__ alive_arr, const float *__restr
     }
     if (save) {
         const int64_t slot = (si - 1) / stride - save_base;
-        write_save(out_kind, out_f64, out_a, out_b, slot, n, col, alive, cell, off, U, v, g, ph);
+        // the plain path adds each step to the FP32 offset directly, as the
+        // reference adds it to the position; comp stays the residual of the
+        // initial position
+        float comp[3];
+        for (int c = 0; c < 3; ++c) comp[c] = s.comp[c][p];
+        write_save(out_kind, out_f64, out_a, out_b, slot, n, col, alive, cell, off, comp, U, v, g, ph);
     }
     if (!push) return;
     float inv_gn;
@@ -122,19 +127,21 @@ __device__ __forceinline__ void push_fast(SplitState s, uint8_t *__restrict__ al
     const int64_t col = perm ? perm[q] : q;
     bool alive = valid && alive_arr[q] != 0;
     int32_t cell[3];
-    float off[3], U[3];
+    float off[3], comp[3], U[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         cell[c] = s.cell[c][q];
         off[c] = s.off[c][q];
+        comp[c] = s.comp[c][q];
         U[c] = s.P[c][q] * ph.mr;
     }
     const double mr2 = static_cast<double>(ph.mr) * static_cast<double>(ph.mr);
     double P2 = s.P2[q] * mr2;
     run_lane<STORE, CUBIC>(
-        cell, off, U, P2, alive, fv, g, ph, cad, periodic, &lds[threadIdx.x >> 6],
-        [&](int64_t slot, bool alv, const int32_t *ce, const float *of, const float *u, double, const float *v) {
-            if (valid) write_save(out_kind, out_f64, out_a, out_b, slot, n, col, alv, ce, of, u, v, g, ph);
+        cell, off, comp, U, P2, alive, fv, g, ph, cad, periodic, &lds[threadIdx.x >> 6],
+        [&](int64_t slot, bool alv, const int32_t *ce, const float *of, const float *cp, const float *u, double,
+            const float *v) {
+            if (valid) write_save(out_kind, out_f64, out_a, out_b, slot, n, col, alv, ce, of, cp, u, v, g, ph);
         },
         [&](int64_t si) { atomicAdd(&deaths[si - first_step], 1ull); });
     if (!valid) return;
@@ -142,6 +149,7 @@ __device__ __forceinline__ void push_fast(SplitState s, uint8_t *__restrict__ al
     for (int c = 0; c < 3; ++c) {
         s.cell[c][p] = cell[c];
         s.off[c][p] = off[c];
+        s.comp[c][p] = comp[c];
         s.P[c][p] = U[c] / ph.mr;
     }
     s.P2[p] = P2 / mr2;

@@ -271,10 +271,11 @@ __global__ void k_split_from_si(const double *__restrict__ x, const double *__re
     s.P2[i] = p2;
     for (int c = 0; c < 3; ++c) {
         int32_t cell;
-        float off;
-        split_coord(x[c * n + i] / d[c], nc[c], periodic != 0, cell, off);
+        float off, comp;
+        split_coord(x[c * n + i] / d[c], nc[c], periodic != 0, cell, off, comp);
         s.cell[c][i] = cell;
         s.off[c][i] = off;
+        s.comp[c][i] = comp;
         s.P[c][i] = static_cast<float>(p[c * n + i] / pnorm);
     }
 }
@@ -292,7 +293,8 @@ __global__ void k_split_to_si(SplitState s, const int32_t *__restrict__ perm, co
     for (int c = 0; c < 3; ++c) u2 += static_cast<double>(s.P[c][i]) * static_cast<double>(s.P[c][i]);
     const double sc = u2 > 0.0 ? sqrt(s.P2[i] / u2) : 1.0;
     for (int c = 0; c < 3; ++c) {
-        x[c * n + o] = (static_cast<double>(s.cell[c][i]) + static_cast<double>(s.off[c][i])) * d[c];
+        x[c * n + o] = (static_cast<double>(s.cell[c][i]) + static_cast<double>(s.off[c][i]) +
+                        static_cast<double>(s.comp[c][i])) * d[c];
         p[c * n + o] = static_cast<double>(s.P[c][i]) * sc * pnorm;
     }
     alive_out[o] = alive[i];

@@ -35,6 +35,7 @@ using namespace core;
 struct DevReq {
     int32_t *cell[3];
     float *off[3];
+    float *comp[3];         // compensation of the offset sums (SplitState::comp)
     float *U[3];            // momentum / (m c) of this request's species
     double *P2;             // |U|^2 in FP64
     uint8_t *alive;
@@ -74,11 +75,12 @@ __device__ __forceinline__ void serve_tick(const DevReq *__restrict__ reqs, int 
     const int64_t q = valid ? p : 0;
     bool alive = valid && R.alive[q] != 0;
     int32_t cell[3];
-    float off[3], U[3];
+    float off[3], comp[3], U[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         cell[c] = R.cell[c][q];
         off[c] = R.off[c][q];
+        comp[c] = R.comp[c][q];
         U[c] = R.U[c][q];
     }
     double P2 = R.P2[q];
@@ -96,8 +98,9 @@ __device__ __forceinline__ void serve_tick(const DevReq *__restrict__ reqs, int 
     const int periodic = R.periodic;
     unsigned long long *const deaths = R.deaths;
     run_lane<STORE, CUBIC>(
-        cell, off, U, P2, alive, fv, g, ph, cad, periodic, &lds[threadIdx.x >> 6],
-        [&](int64_t slot, bool alv, const int32_t *, const float *, const float *u, double p2, const float *v) {
+        cell, off, comp, U, P2, alive, fv, g, ph, cad, periodic, &lds[threadIdx.x >> 6],
+        [&](int64_t slot, bool alv, const int32_t *, const float *, const float *, const float *u, double p2,
+            const float *v) {
             if (!valid) return;
             if (kind == PT_OUT_MU) {
                 static_cast<float *>(out)[slot * n + p] = alv ? mu_f32(u[0], u[1], u[2], v[0], v[1], v[2], qsign) : NAN;
@@ -114,6 +117,7 @@ __device__ __forceinline__ void serve_tick(const DevReq *__restrict__ reqs, int 
     for (int c = 0; c < 3; ++c) {
         R.cell[c][p] = cell[c];
         R.off[c][p] = off[c];
+        R.comp[c][p] = comp[c];
         R.U[c][p] = U[c];
     }
     R.P2[p] = P2;
@@ -146,10 +150,11 @@ __global__ void k_serve_init(DevReq R, const double *__restrict__ init, int fixe
     if (i >= n) return;
     for (int c = 0; c < 3; ++c) {
         int32_t cell;
-        float off;
-        split_coord(init[c * n + i] / R.g.d[c], R.g.nc[c], R.periodic != 0, cell, off);
+        float off, comp;
+        split_coord(init[c * n + i] / R.g.d[c], R.g.nc[c], R.periodic != 0, cell, off, comp);
         R.cell[c][i] = cell;
         R.off[c][i] = off;
+        R.comp[c][i] = comp;
     }
     if (!fixed_mu) {
         double p2 = 0.0;
@@ -346,7 +351,7 @@ namespace {
 size_t request_bytes(const pt_serve_request &d, int64_t nsave) {
     const size_t n = static_cast<size_t>(d.n);
     const size_t esz = d.out_kind == PT_OUT_PMAG ? 8 : 4;
-    size_t b = align_up(n * 8) + 9 * align_up(n * 4) + align_up(n) + kAlign;
+    size_t b = align_up(n * 8) + 12 * align_up(n * 4) + align_up(n) + kAlign;
     if (d.out_kind != PT_OUT_NONE) b += align_up(static_cast<size_t>(nsave) * n * esz);
     return b;
 }
@@ -368,6 +373,10 @@ void fill_dev(pt_serve *srv, SReq &r) {
     }
     for (int c = 0; c < 3; ++c) {
         D.U[c] = reinterpret_cast<float *>(m);
+        m += align_up(n * 4);
+    }
+    for (int c = 0; c < 3; ++c) {
+        D.comp[c] = reinterpret_cast<float *>(m);
         m += align_up(n * 4);
     }
     D.alive = reinterpret_cast<uint8_t *>(m);

@@ -50,6 +50,7 @@ __global__ void k_gather_state(SplitState src, SplitState dst, const uint8_t *__
     for (int c = 0; c < 3; ++c) {
         dst.cell[c][i] = src.cell[c][j];
         dst.off[c][i] = src.off[c][j];
+        dst.comp[c][i] = src.comp[c][j];
         dst.P[c][i] = src.P[c][j];
     }
     alive_dst[i] = alive_src[j];

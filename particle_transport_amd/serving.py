@@ -313,12 +313,17 @@ def finish_request(server, stats_ctx, rid, req):
 
 
 def run_stream(push_ctx, stats_ctx, fields, reqs, *, budget_bytes, steps_per_tick=200, max_active=1024,
-               box_length_m, time_scale=1.0, keep_results=False, log=None):
+               box_length_m, time_scale=1.0, keep_results=False, keep_products=False, log=None):
     """Replay the stream in real time (arrival times x time_scale).  The loop
     keeps the server stepping; each finished request's product is computed on
     `stats_ctx` by a worker thread (the bindings release the GIL), and the
     request is released once its product is done.  Latency = arrival to
-    product."""
+    product.
+
+    keep_results: True or a set of request indices whose (product, host copy
+    of the cache) are returned in `results`; keep_products: return every
+    request's product in `products`.  `peak_reserved_bytes` is the largest
+    budget the server held at any step."""
     from concurrent.futures import ThreadPoolExecutor
 
     srv = Server(push_ctx, budget_bytes, steps_per_tick, max_active)
@@ -338,6 +343,8 @@ def run_stream(push_ctx, stats_ctx, fields, reqs, *, budget_bytes, steps_per_tic
     outstanding = {}
     finishing = {}
     results = {}
+    products = {}
+    peak = [0]
     lat = []
     pool = ThreadPoolExecutor(max_workers=1)
     t0 = time.perf_counter()
@@ -351,6 +358,7 @@ def run_stream(push_ctx, stats_ctx, fields, reqs, *, budget_bytes, steps_per_tic
                 outstanding[submit_request(srv, rq, box_length_m)] = rq
                 k += 1
             running, queued, done = srv.step()
+            peak[0] = max(peak[0], srv.stats().bytes_reserved)
             for rid in srv.take_done():
                 rq = outstanding.pop(rid)
                 try:
@@ -364,8 +372,10 @@ def run_stream(push_ctx, stats_ctx, fields, reqs, *, budget_bytes, steps_per_tic
                 fut, rq = finishing.pop(rid)
                 res = fut.result()
                 lat.append(time.perf_counter() - t0 - rq.arrival * time_scale)
-                if keep_results:
+                if keep_results is True or (keep_results and rq.i in keep_results):
                     results[rq.i] = (res, srv.copy_cache(rid, cache_dtype(rq.out_kind)))
+                if keep_products:
+                    products[rq.i] = res
                 srv.release(rid)
             if log and now - last_log >= 1.0:
                 last_log = now
@@ -385,4 +395,5 @@ def run_stream(push_ctx, stats_ctx, fields, reqs, *, budget_bytes, steps_per_tic
     return SimpleNamespace(latency_p50_s=float(np.percentile(lat, 50)), latency_p99_s=float(np.percentile(lat, 99)),
                            requests=len(lat), wall_s=wall, particle_steps=st.particle_steps,
                            particle_steps_per_s=st.particle_steps / wall, kernel_ms=st.kernel_ms,
-                           product_busy_s=busy[0], failed=failed, results=results, latencies=lat)
+                           product_busy_s=busy[0], failed=failed, results=results, products=products,
+                           latencies=lat, peak_reserved_bytes=int(peak[0]))
