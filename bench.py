@@ -21,10 +21,13 @@ Beside the headline number the JSON line carries:
   dmumu_check   D_mumu(|mu|) of each format against the FP64 pusher (the
                 oracle-pinned reference-structured kernel) on a 65,536-particle
                 subset;
-  roofline      HBM roofline of the production kernel (algorithmic bytes per
-                particle-step 48*s_f + (36+36)/K + 4/20, K pushes per launch) from
-                HIP-event launch durations on the library's stream, plus the
-                FP32 VALU fraction;
+  roofline      the production kernel against the FP32 vector-ALU roof: FLOP
+                per particle-step from the committed rocprofv3 counter summary
+                (SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_{F32,F64}) times the kernel
+                rate from HIP-event launch durations on the library's stream,
+                over the published 157.3 TFLOP/s; beside it the VALU issue
+                fraction and the HBM convention (algorithmic bytes per
+                particle-step 48*s_f + (56+56)/K + 4/20, K pushes per launch);
   cpu_baseline  the CPU oracle (C restatement of the reference pusher, FP64)
                 on a bounded sample of the same workload in 16 host threads,
                 with a one-core figure beside it.
@@ -44,17 +47,19 @@ sys.path.insert(0, ROOT)
 from particle_transport_amd.injection import M_P as M_P_, Q_E as Q_E_  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# MI355X published FP32 vector peak: 256 CUs x 2.4 GHz x 256 FLOP per clock
+# per CU, i.e. 128 FMA lanes per clock per CU = two wave64 FMA instructions
+# per clock per CU; the VALU issue roof below is that instruction rate.
 FP32_PEAK_TFLOPS = 157.3
-# FP32 operations of one production push, FMA counted as 2: trilinear polynomial
-# 6 x 7 FMA = 84, E = -u x B 12, Boris rotation and kicks ~70, |p| tracking ~20,
-# position move 6 (+4 reciprocal square roots/reciprocals not counted).
-FLOPS_PER_PUSH = 190.0
+VALU_ISSUE_PEAK = FP32_PEAK_TFLOPS * 1e12 / 128.0  # wave64 VALU instructions per second
 # Particle state bytes in the algorithmic-bytes convention (3 int32 cells,
-# 3 FP32 offsets, 3 FP32 momenta), read and written once per launch.
-STATE_BYTES = 36
+# 3 FP32 offsets, 3 FP32 compensation terms, 3 FP32 momenta, FP64 |p|^2),
+# read and written once per launch.
+STATE_BYTES = 56
 FORMATS = ("bf16_brick", "f16_brick", "f32_brick")
 FORMAT_KEY = {"bf16_brick": "bf16", "f16_brick": "f16", "f32_brick": "f32_field"}
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "push_fast_512_traffic.md")
+PROFILE_COUNTERS = os.path.join(ROOT, "profiles", "push_fast_512_valu_counters.md")
 
 
 def log(*a):
@@ -88,6 +93,22 @@ def make_particles(n, L, energy_gev, seed):
     gamma = float(energy_to_gamma(energy_gev))
     p = d * (gamma * M_P * C_LIGHT * math.sqrt(1 - 1 / gamma ** 2))
     return x, p, gamma
+
+
+def counters_from_profiles(fmt):
+    """FP operations and VALU wave-instructions per particle-step of the
+    production kernel on `fmt` at the default workload, from the committed
+    rocprofv3 counter summary (lines "valu <format>: <flop> flop <instr>
+    instructions per particle-step"), or None."""
+    import re
+    if not os.path.exists(PROFILE_COUNTERS):
+        return None
+    with open(PROFILE_COUNTERS) as f:
+        for line in f:
+            m = re.match(r"^valu (\w+): ([0-9.]+) flop ([0-9.]+) instructions per particle-step", line.strip())
+            if m and m.group(1) == fmt:
+                return float(m.group(2)), float(m.group(3))
+    return None
 
 
 def traffic_from_profiles(fmt):
@@ -764,6 +785,9 @@ def main():
     kernel_rate = n * K / avg_launch_s
     profiled = (args.cells, n, S, args.energy_gev) == (512, 4194304, 200, 100.0)
     traffic = traffic_from_profiles(head) if profiled else None
+    counters = counters_from_profiles(head) if profiled else None
+    flop_ps, valu_ps = counters if counters else (None, None)
+    flops = flop_ps * kernel_rate / 1e12 if counters else None
     out = {
         "metric": "particle-steps/s (FP32 vs BF16 field) at D_mumu err<=1%; p99 campaign latency",
         "value": value if passing else None, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -779,16 +803,26 @@ def main():
                                        "fastest format's figures are reported for reference only"),
                    "no_format_passed": not passing,
                    "global_batch": n, "seq_len": S, "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_particle_step": b_alg,
-                     "measured_hbm_bytes_per_particle_step": traffic / (n * K) if traffic else None,
+        "roofline": {"bound": "valu", "achieved": flops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": flops / FP32_PEAK_TFLOPS if counters else None, "traffic": traffic,
+                     "flop_per_particle_step": flop_ps,
+                     "valu_instructions_per_particle_step": valu_ps,
+                     "valu_issue_frac": valu_ps * kernel_rate / VALU_ISSUE_PEAK if counters else None,
+                     "counter_source": os.path.relpath(PROFILE_COUNTERS, ROOT) if counters else None,
                      "kernel_particle_steps_per_s": kernel_rate, "avg_launch_ms": avg_launch_s * 1e3,
-                     "fp32_valu_frac": kernel_rate * FLOPS_PER_PUSH / (FP32_PEAK_TFLOPS * 1e12),
-                     "note": "B_alg counts the 48 corner values every push as the reference gathers them; the "
-                             "kernel keeps the current cell's interpolation coefficients in registers and reloads "
-                             "them only on a cell change, so frac exceeds 1 and the FP32 ALU fraction is the "
-                             "binding roof"},
+                     "hbm_convention": {
+                         "bytes_per_particle_step": b_alg, "achieved_gbs": achieved, "peak_gbs": HBM_PEAK_GBS,
+                         "frac": achieved / HBM_PEAK_GBS,
+                         "measured_hbm_bytes_per_particle_step": traffic / (n * K) if traffic else None,
+                         "note": "B_alg counts the 48 corner values every push as the reference gathers them; "
+                                 "the kernel keeps the current cell's interpolation coefficients in registers and "
+                                 "reloads them only on a cell change, so this ratio exceeds 1 and is not the "
+                                 "binding roof"},
+                     "note": "FLOP and VALU instructions per particle-step are rocprofv3 SQ counters of this "
+                             "kernel and workload (every issued wave64 instruction, helpers and idle lanes "
+                             "included) read from counter_source; achieved = that FLOP count x the kernel "
+                             "rate measured here; valu_issue_frac = VALU instructions issued over the "
+                             "two-per-clock-per-CU rate the published FP32 peak implies"},
         "formats": {f: {"value": timing[f]["value"], "avg_launch_ms": timing[f]["avg_launch_s"] * 1e3,
                         "dmumu_pass": None if check is None else check[FORMAT_KEY[f]]["pass"]} for f in formats},
         "plain_f32": {"value": plain_rate, "kernel_particle_steps_per_s": plain_kernel,

@@ -58,17 +58,19 @@ struct Phys {
 
 // One Boris push in folded units with the momentum magnitude carried in FP64.
 // U (FP32 vector, units m c) and P2 = |U|^2 (FP64): the two half E-kicks
-// are the only steps that change |p|, so P2 is advanced through them exactly
-// in FP64 (|U+e|^2 = |U|^2 + 2 U.e + e.e), the rotation is applied to the FP32
-// vector, and the result is rescaled to |U| = sqrt(P2).  Without this the FP32
-// rotation's rounding drifts |p| by ~5e-8 per push, systematically.
+// are the only steps that change |p|.  With U- = U + e, U' its rotation
+// (|U'| = |U-|) and U+ = U' + e, |U+|^2 - |U|^2 = 2 (U + U+).e exactly, so
+// P2 is advanced by that FP32 dot product (about 1e-5 of |U|^2) in one FP64
+// add, the rotation is applied to the FP32 vector, and the result is
+// rescaled to |U| = sqrt(P2).  Without this the FP32 rotation's rounding
+// drifts |p| by ~5e-8 per push, systematically.
 //
 // `renorm` (uniform across the wave) applies the rescale; between rescales
 // |U| is off by at most the rotation drift of the pushes since the last one.
 // The Lorentz factors come from the FP32 vector (1/gamma- from 1 + |U-|^2,
-// 1/gamma+ from it and the second kick), so they are within ~2e-7 of the FP64
-// values and depend only on the particle's state, never on how the steps are
-// split into launches.  inv_gn returns 1/gamma of the new momentum.
+// 1/gamma+ from 1 + |U+|^2 before the rescale, both by the hardware
+// reciprocal square root), so they depend only on the particle's state,
+// never on how the steps are split into launches.  inv_gn returns 1/gamma+.
 __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, double &P2, float bx, float by,
                                             float bz, float ux, float uy, float uz, float &inv_gn,
                                             bool renorm = true) {
@@ -76,9 +78,6 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
     const float ex = -(uy * bz - uz * by);
     const float ey = -(uz * bx - ux * bz);
     const float ez = -(ux * by - uy * bx);
-    // kick terms are ~1e-5 of |U|^2: FP32 for them, one FP64 add to accumulate
-    const float ee = ex * ex + ey * ey + ez * ez;
-    const float k1 = fmaf(2.0f, Ux * ex + Uy * ey + Uz * ez, ee);
     const float pmx = Ux + ex, pmy = Uy + ey, pmz = Uz + ez;
     // 1/gamma- from the hardware reciprocal square root of 1 + |U-|^2
     const float r = __builtin_amdgcn_rsqf(fmaf(pmx, pmx, fmaf(pmy, pmy, fmaf(pmz, pmz, 1.0f))));
@@ -106,11 +105,13 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
     const float qx = fmaf(ppy, sz, fmaf(-ppz, sy, pmx));
     const float qy = fmaf(ppz, sx, fmaf(-ppx, sz, pmy));
     const float qz = fmaf(ppx, sy, fmaf(-ppy, sx, pmz));
-    const float k2 = fmaf(2.0f, qx * ex + qy * ey + qz * ez, ee);
-    const double P2n = P2 + static_cast<double>(k1 + k2);
-    Ux = qx + ex;
-    Uy = qy + ey;
-    Uz = qz + ez;
+    const float nx = qx + ex, ny = qy + ey, nz = qz + ez;
+    const float dk = (Ux + nx) * ex + (Uy + ny) * ey + (Uz + nz) * ez;
+    const double P2n = fma(2.0, static_cast<double>(dk), P2);
+    inv_gn = __builtin_amdgcn_rsqf(fmaf(nx, nx, fmaf(ny, ny, fmaf(nz, nz, 1.0f))));
+    Ux = nx;
+    Uy = ny;
+    Uz = nz;
     if (renorm) {
         // rescale to sqrt(P2n): the ratio is 1 + O(1e-6), so sqrt(1+d) = 1 + d/2 - d^2/8
         const float u2 = Ux * Ux + Uy * Uy + Uz * Uz;
@@ -121,10 +122,6 @@ __device__ __forceinline__ void boris_split(float &Ux, float &Uy, float &Uz, dou
         Uz *= sc;
     }
     P2 = P2n;
-    // The rotation keeps |U-|, so gamma+^2 = gamma-^2 + k2 and
-    // 1/gamma+ = r (1 + x)^(-1/2) with x = k2 r^2 ~ 1e-5: two Taylor terms.
-    const float x = k2 * (r * r);
-    inv_gn = r * fmaf(x, fmaf(0.375f, x, -0.5f), 1.0f);
 }
 
 // Bring an offset that left [0, 1) back into its cell range (periodic wrap
@@ -151,6 +148,51 @@ __device__ __forceinline__ void move_axis(int32_t &cell, float &off, float u, fl
                                           bool periodic, bool &moved) {
     off = fmaf(u, step, off);
     wrap_axis(cell, off, nc, periodic, moved);
+}
+
+// One axis of the compensated position after a push that took its
+// coordinate x = off + comp out of [0, 1): fold comp into the offset
+// (Fast2Sum), move the integer part into the cell (periodic wrap of the cell
+// index when `periodic`) and recompute x.  `moved` is set.
+__device__ __forceinline__ void cross_axis(int32_t &cell, float &off, float &comp, float &x, int32_t nc,
+                                           bool periodic, bool &moved) {
+    if (!(x >= 0.0f && x < 1.0f)) {
+        moved = true;
+        const float t = x;
+        comp -= t - off;
+        off = t;
+        const float fl = floorf(off);
+        cell += static_cast<int32_t>(fl);
+        off -= fl;
+        if (off >= 1.0f) {  // -tiny + 1 rounds to 1: stay put at offset 0
+            off -= 1.0f;
+            cell += 1;
+        }
+        if (periodic) {
+            // a push moves a particle by less than a box length, so one
+            // correction almost always suffices; the division is the fallback
+            if (cell < 0) {
+                cell += nc;
+            } else if (cell >= nc) {
+                cell -= nc;
+            }
+            if (__builtin_expect(cell < 0 || cell >= nc, 0)) {
+                cell %= nc;
+                if (cell < 0) cell += nc;
+            }
+        }
+        x = off + comp;
+    }
+}
+
+// x == L on an open box (cell nc, offset 0) as the last cell at offset 1:
+// the same point, gathered from the same cell, with a coordinate that keeps
+// the particle on the boundary path.
+__device__ __forceinline__ void to_last_cell(int32_t &cell, float &off, int32_t nc) {
+    if (cell == nc) {
+        cell = nc - 1;
+        off += 1.0f;
+    }
 }
 
 // x > L or x < 0 kills (phase_space_gpu_runner.jl:385-387); x == L is cell nc, offset 0.
@@ -240,6 +282,25 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblk) {
     return xcd * q + (xcd < r ? xcd : r) + (b >> 3);
 }
 
+// Brick record of node (gi, gj, gk) in 32-bit arithmetic (every record
+// index of a brick field fits in 31 bits) and the record steps to its +1 neighbours: 1, 8 and 64
+// records inside an 8^3 brick, a jump to the next brick from its last node
+// along an axis.
+struct CornerRecords {
+    uint32_t base, sx, sy, sz;
+};
+__device__ __forceinline__ CornerRecords corner_records(const FieldView &fv, int32_t gi, int32_t gj, int32_t gk) {
+    const uint32_t nb0 = static_cast<uint32_t>(fv.nb[0]), nb1 = static_cast<uint32_t>(fv.nb[1]);
+    const uint32_t i = static_cast<uint32_t>(gi), j = static_cast<uint32_t>(gj), k = static_cast<uint32_t>(gk);
+    const uint32_t brick = ((k >> 3) * nb1 + (j >> 3)) * nb0 + (i >> 3);
+    CornerRecords r;
+    r.base = (brick << 9) | ((k & 7) << 6) | ((j & 7) << 3) | (i & 7);
+    r.sx = (i & 7) == 7 ? 512u - 7u : 1u;
+    r.sy = (j & 7) == 7 ? nb0 * 512u - 56u : 8u;
+    r.sz = (k & 7) == 7 ? nb0 * nb1 * 512u - 448u : 64u;
+    return r;
+}
+
 // Corner values of the six components at the 8 nodes of cell (gi,gj,gk),
 // corner index ci = dx + 2 dy + 4 dz, in storage units.
 template <int STORE>
@@ -261,17 +322,14 @@ __device__ __forceinline__ void load_corners(const FieldView &fv, int32_t gi, in
         // node of a brick they jump to the next brick along that axis.  Every
         // record index fits in 32 bits (< 2^31 / 16 bytes for any field the
         // library accepts).
-        const int32_t nb0 = static_cast<int32_t>(fv.nb[0]), nb1 = static_cast<int32_t>(fv.nb[1]);
-        const int32_t base = static_cast<int32_t>(brick_node_index(fv.nb, gi, gj, gk));
-        const int32_t sx = (gi & 7) == 7 ? 512 - 7 : 1;
-        const int32_t sy = (gj & 7) == 7 ? nb0 * 512 - 56 : 8;
-        const int32_t sz = (gk & 7) == 7 ? nb0 * nb1 * 512 - 448 : 64;
-        const int32_t rec[8] = {base, base + sx, base + sy, base + sy + sx,
-                                base + sz, base + sz + sx, base + sz + sy, base + sz + sy + sx};
+        const CornerRecords cr = corner_records(fv, gi, gj, gk);
+        const uint32_t rec[8] = {cr.base, cr.base + cr.sx, cr.base + cr.sy, cr.base + cr.sy + cr.sx,
+                                 cr.base + cr.sz, cr.base + cr.sz + cr.sx, cr.base + cr.sz + cr.sy,
+                                 cr.base + cr.sz + cr.sy + cr.sx};
 #pragma unroll
         for (int ci = 0; ci < 8; ++ci) {
             if constexpr (STORE == PT_STORE_F32_BRICK) {
-                const float4 *R = static_cast<const float4 *>(fv.data) + static_cast<int64_t>(rec[ci]) * 2;
+                const float4 *R = static_cast<const float4 *>(fv.data) + static_cast<uint64_t>(rec[ci]) * 2u;
                 const float4 a = R[0];
                 const float4 b = R[1];
                 s[0][ci] = a.x;
@@ -392,11 +450,7 @@ __device__ __forceinline__ void load_component_corners(const FieldView &fv, int3
             s[ci] = F[i + fv.n[0] * (j + fv.n[1] * k)];
         }
     } else {
-        const int32_t nb0 = static_cast<int32_t>(fv.nb[0]), nb1 = static_cast<int32_t>(fv.nb[1]);
-        const int32_t base = static_cast<int32_t>(brick_node_index(fv.nb, gi, gj, gk));
-        const int32_t sx = (gi & 7) == 7 ? 512 - 7 : 1;
-        const int32_t sy = (gj & 7) == 7 ? nb0 * 512 - 56 : 8;
-        const int32_t sz = (gk & 7) == 7 ? nb0 * nb1 * 512 - 448 : 64;
+        const CornerRecords cr = corner_records(fv, gi, gj, gk);
         // This component's half of a 32-bit word, taken per lane without a
         // select between the two halves (with a select the compiler waited
         // on each of the eight loads in turn): FP16 by a shift, BF16 by one
@@ -407,10 +461,11 @@ __device__ __forceinline__ void load_component_corners(const FieldView &fv, int3
             // the eight loads first, then the decodes: the scheduler keeps
             // them apart, so all eight are in flight together
             uint32_t w[8];
+            const uint32_t *src = static_cast<const uint32_t *>(fv.data) + (comp >> 1);
 #pragma unroll
             for (int ci = 0; ci < 8; ++ci) {
-                const int64_t rec = base + ((ci & 1) ? sx : 0) + ((ci & 2) ? sy : 0) + ((ci & 4) ? sz : 0);
-                w[ci] = static_cast<const uint32_t *>(fv.data)[rec * 4 + (comp >> 1)];
+                const uint32_t rec = cr.base + ((ci & 1) ? cr.sx : 0u) + ((ci & 2) ? cr.sy : 0u) + ((ci & 4) ? cr.sz : 0u);
+                w[ci] = src[static_cast<uint64_t>(rec) * 4u];
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -418,11 +473,11 @@ __device__ __forceinline__ void load_component_corners(const FieldView &fv, int3
         } else {
 #pragma unroll
             for (int ci = 0; ci < 8; ++ci) {
-                const int64_t rec = base + ((ci & 1) ? sx : 0) + ((ci & 2) ? sy : 0) + ((ci & 4) ? sz : 0);
+                const uint32_t rec = cr.base + ((ci & 1) ? cr.sx : 0u) + ((ci & 2) ? cr.sy : 0u) + ((ci & 4) ? cr.sz : 0u);
                 if constexpr (STORE == PT_STORE_F32_BRICK) {
-                    s[ci] = static_cast<const float *>(fv.data)[rec * 8 + comp];
+                    s[ci] = (static_cast<const float *>(fv.data) + comp)[static_cast<uint64_t>(rec) * 8u];
                 } else {
-                    const uint32_t w = static_cast<const uint32_t *>(fv.data)[rec * 4 + (comp >> 1)];
+                    const uint32_t w = (static_cast<const uint32_t *>(fv.data) + (comp >> 1))[static_cast<uint64_t>(rec) * 4u];
                     s[ci] = f16_bits_to_float(static_cast<uint16_t>((w >> sh) & 0xffffu));
                 }
             }
@@ -441,6 +496,7 @@ constexpr int kReloadBatch = 10;
 struct ReloadLds {
     int4 slot[kReloadBatch];
     float coef[kReloadBatch][6][8];
+    float scale[6];  // per component: decode scale times the physical factor
 };
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -449,22 +505,26 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-struct ReloadLane {
-    int lane, entry, comp;  // helper role: posted cell `entry`, component `comp`
-    float scale;            // decode of `comp` times its physical factor
-};
+// Lane index within the wave, and the number of lanes below it whose bit
+// is set in `mask` (hardware bit counts, no per-lane constants kept).
+__device__ __forceinline__ int lane_id() {
+    return static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));
+}
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
+}
 
-__device__ __forceinline__ ReloadLane reload_lane(const FieldView &fv, const Phys &ph) {
-    ReloadLane r;
-    r.lane = static_cast<int>(threadIdx.x & 63);
-    r.entry = r.lane / 6;
-    r.comp = r.lane - 6 * r.entry;
-    r.scale = fv.scale[0] * ph.fac[0];
+// The per-component scales the helper lanes apply, written once into the
+// wave's LDS area (they are read only on a reload pass, so no lane keeps
+// them in a register across the step loop).
+__device__ __forceinline__ void reload_init(const FieldView &fv, const Phys &ph, ReloadLds *lds) {
+    const int lane = lane_id();
 #pragma unroll
-    for (int c = 1; c < 6; ++c) {
-        if (r.comp == c) r.scale = fv.scale[c] * ph.fac[c];
+    for (int c = 0; c < 6; ++c) {
+        if (lane == c) lds->scale[c] = fv.scale[c] * ph.fac[c];
     }
-    return r;
+    wave_lds_sync();
 }
 
 // Per-lane reload of the coefficients of the particle's own cell, all loads
@@ -489,11 +549,10 @@ __device__ __forceinline__ void lane_reload(const int32_t cell[3], const FieldVi
 
 template <int STORE>
 __device__ __forceinline__ void coop_reload(int &need, const int32_t cell[3], const FieldView &fv,
-                                            const SplitGeom &g, const ReloadLane &rl, ReloadLds *lds,
-                                            float a[6][8]) {
+                                            const SplitGeom &g, ReloadLds *lds, float a[6][8]) {
     uint64_t pending = __ballot(need != 0);
     while (pending) {
-        const int rank = __popcll(pending & ((uint64_t{1} << rl.lane) - 1));
+        const int rank = lanes_below(pending);
         const bool mine = need != 0 && rank < kReloadBatch;
         const int m = __popcll(pending) < kReloadBatch ? __popcll(pending) : kReloadBatch;
         if (mine) {
@@ -506,13 +565,18 @@ __device__ __forceinline__ void coop_reload(int &need, const int32_t cell[3], co
             lds->slot[rank] = make_int4(gi, gj, gk, bias);
         }
         wave_lds_sync();
-        if (rl.entry < m) {
-            const int4 t = lds->slot[rl.entry];
+        // helper role: lane = 6 entry + comp serves component comp of
+        // posted cell `entry` ((lane * 43) >> 8 == lane / 6 for lane < 64)
+        const int lane = lane_id();
+        const int entry = (lane * 43) >> 8;
+        const int comp = lane - 6 * entry;
+        if (entry < m) {
+            const int4 t = lds->slot[entry];
             float sv[8], c8[8];
-            load_component_corners<STORE>(fv, t.x, t.y, t.z, rl.comp, sv);
-            cell_coeffs<STORE>(fv, t.x, t.y, t.z, rl.comp, sv, rl.scale, c8);
+            load_component_corners<STORE>(fv, t.x, t.y, t.z, comp, sv);
+            cell_coeffs<STORE>(fv, t.x, t.y, t.z, comp, sv, lds->scale[comp], c8);
             if (t.w) shift_coeffs(c8, (t.w & 1) ? 1.0f : 0.0f, (t.w & 2) ? 1.0f : 0.0f, (t.w & 4) ? 1.0f : 0.0f);
-            float4 *dst = reinterpret_cast<float4 *>(lds->coef[rl.entry][rl.comp]);
+            float4 *dst = reinterpret_cast<float4 *>(lds->coef[entry][comp]);
             dst[0] = make_float4(c8[0], c8[1], c8[2], c8[3]);
             dst[1] = make_float4(c8[4], c8[5], c8[6], c8[7]);
         }
@@ -560,11 +624,21 @@ __host__ inline Cadence make_cadence(int64_t si0, int K, int64_t total_steps, in
 
 // The production step loop of one lane: K steps with state and the current
 // cell's coefficients in registers.  The coefficients are reloaded only after
-// the particle changes cell (move_axis reports it), so a push costs the
-// six polynomial evaluations and the Boris step.  save(slot, alive, cell,
-// off, comp, U, P2, v) is called at every cadence step (before the push, as the
+// the particle changes cell, so a push costs the six polynomial evaluations,
+// the Boris step and the position update.  save(slot, alive, cell, off,
+// comp, U, P2, v) is called at every cadence step (before the push, as the
 // reference's *_step copies); die(si) when an open boundary removes the
 // particle.
+//
+// Position: cell + off + comp.  Each push adds its step u * k / gamma to the
+// small term comp (one rounding at the scale of comp, not of the offset), and
+// the polynomial is evaluated at x = off + comp.  Every fourth global step
+// (and whenever x leaves [0, 1)) comp is folded into the offset with
+// compensated summation (Fast2Sum: off = x, comp keeps what that addition
+// dropped), so a step far below the offset's ulp (6e-8 of a cell near 1)
+// still accumulates at its full value instead of rounding the same way push
+// after push.  The fold cadence follows the global step index, so the result
+// does not depend on how the steps are split into launches.
 //
 // All 64 lanes of the wave must run the loop (lanes without a particle with
 // alive = false): the coefficient reload is wave-cooperative (coop_reload)
@@ -574,6 +648,14 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float co
                                          bool &alive,
                                          const FieldView &fv, const SplitGeom &g, const Phys &ph,
                                          const Cadence &cad, int periodic, ReloadLds *lds, SaveFn save, DieFn die) {
+    // A particle on an upper face of an open box (x == L: cell nc, offset
+    // 0) is held as the last cell at offset 1, so its coordinate is not
+    // below 1 and every push takes the boundary path until it moves inwards
+    // (or leaves).
+    if (!periodic) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) to_last_cell(cell[c], off[c], g.nc[c]);
+    }
     float a[6][8];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
@@ -585,24 +667,32 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float co
     // push (never set on a dead lane).
     int live = alive ? 1 : 0;
     int reload = 0;
+    float x[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[c] = off[c] + comp[c];
     const int32_t si_mod4 = static_cast<int32_t>(cad.si0 & 3);
-    const ReloadLane rl = reload_lane(fv, ph);
+    reload_init(fv, ph, lds);
     int32_t phase = cad.phase0;
     int64_t slot = cad.slot0;
     // pushes happen while si < total_steps: at most n_push of the K steps
     const int64_t room = cad.total_steps - cad.si0;
     const int32_t n_push = room <= 0 ? 0 : (room < cad.K ? static_cast<int32_t>(room) : cad.K);
+    auto fold = [&](int c) {
+        const float t = x[c];
+        comp[c] -= t - off[c];
+        off[c] = t;
+    };
     for (int32_t k = 0; k < cad.K; ++k) {
         const bool do_save = cad.want_save && phase == 0;
         phase = phase + 1 == cad.stride ? 0 : phase + 1;
         const bool push = live != 0 && k < n_push;
-        coop_reload<STORE>(reload, cell, fv, g, rl, lds, a);  // every lane of the wave, converged here
+        coop_reload<STORE>(reload, cell, fv, g, lds, a);  // every lane of the wave, converged here
         if (!do_save && !push) continue;
         // dead lanes evaluate stale coefficients only to keep the loop
         // uniform; save() writes NaN for them
         float v[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) v[c] = eval_poly(a[c], off[0], off[1], off[2]);
+        for (int c = 0; c < 6; ++c) v[c] = eval_poly(a[c], x[0], x[1], x[2]);
         if (do_save) {
             save(slot, live != 0, cell, off, comp, U, P2, v);
             ++slot;
@@ -612,46 +702,50 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float co
         // before each save (phase == 0: the next step saves), so every output
         // carries the rescaled magnitude and the result does not depend on
         // how the steps are split into launches
-        const bool renorm = (((si_mod4 + k) & 3) == 0) | (phase == 0);
+        const bool fourth = ((si_mod4 + k) & 3) == 0;
         float inv_gn;
-        boris_split(U[0], U[1], U[2], P2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn, renorm);
-        // all three offsets first; one rare branch for any cell change.  With
-        // cubic cells (CUBIC: g.k equal on every axis) the step factor is one
-        // product, the same value each axis would compute.  The step is added
-        // with compensated summation (Fast2Sum): comp keeps what the FP32
-        // offset's rounding dropped and feeds it into the next step, so a
-        // step far below the offset's ulp (6e-8 of a cell near 1) still
-        // accumulates at its full value instead of rounding the same way push
-        // after push.
-        auto step_axis = [&](int c, float w) {
-            const float st = fmaf(U[c], w, comp[c]);
-            const float t = off[c] + st;
-            comp[c] = st - (t - off[c]);
-            off[c] = t;
-        };
+        boris_split(U[0], U[1], U[2], P2, v[0], v[1], v[2], v[3], v[4], v[5], inv_gn, fourth | (phase == 0));
+        // With cubic cells (CUBIC: g.k equal on every axis) the step factor
+        // is one product, the same value each axis would compute.
         if constexpr (CUBIC) {
             const float w = g.k[0] * inv_gn;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) step_axis(c, w);
+            for (int c = 0; c < 3; ++c) comp[c] = fmaf(U[c], w, comp[c]);
         } else {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) step_axis(c, g.k[c] * inv_gn);
+            for (int c = 0; c < 3; ++c) comp[c] = fmaf(U[c], g.k[c] * inv_gn, comp[c]);
         }
-        // every offset in [0, 1) <=> every bit pattern below that of 1.0f
-        // (negative values, -0 and NaN compare above it and take the slow path)
-        const uint32_t top = max(max(__float_as_uint(off[0]), __float_as_uint(off[1])), __float_as_uint(off[2]));
-        const bool inside = top < 0x3f800000u;
-        if (!inside) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[c] = off[c] + comp[c];
+        if (fourth) {
+            // a branch, not selects: the empty asm keeps the compiler from
+            // flattening the fold into every push
+            asm volatile("");
+#pragma unroll
+            for (int c = 0; c < 3; ++c) fold(c);
+        }
+        // every coordinate in [0, 1) <=> every bit pattern below that of
+        // 1.0f (negative values, -0 and NaN compare above it)
+        const uint32_t top = max(max(__float_as_uint(x[0]), __float_as_uint(x[1])), __float_as_uint(x[2]));
+        if (__builtin_expect(!(top < 0x3f800000u), 0)) {
             bool moved = false;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) wrap_axis(cell[c], off[c], g.nc[c], periodic, moved);
-            if (moved) reload = 1;
+            for (int c = 0; c < 3; ++c) cross_axis(cell[c], off[c], comp[c], x[c], g.nc[c], periodic, moved);
+            reload |= moved ? 1 : 0;
+            if (!periodic) {
+                if (outside(cell[0], off[0], g.nc[0]) | outside(cell[1], off[1], g.nc[1]) |
+                    outside(cell[2], off[2], g.nc[2])) {
+                    die(cad.si0 + k);
+                    live = 0;
+                    reload = 0;
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    to_last_cell(cell[c], off[c], g.nc[c]);
+                    x[c] = off[c] + comp[c];
+                }
+            }
         }
-        const bool dies = !periodic && (outside(cell[0], off[0], g.nc[0]) | outside(cell[1], off[1], g.nc[1]) |
-                                        outside(cell[2], off[2], g.nc[2]));
-        if (dies) die(cad.si0 + k);
-        live = dies ? 0 : live;
-        reload = dies ? 0 : reload;
     }
     alive = live != 0;
 }
