@@ -388,6 +388,7 @@ def stats_rates(ctx, F32, L, x, p, dt, log_fn, cpu=True):
     ens.advance(dt=dt, first_step=1, n_steps=nsave, total_steps=nsave, stride=1, out_kind="phase", out_a=pos,
                 out_b=mom, steps_per_launch=200)
     ens.close()
+    out["dmumu_from_phase"] = dmumu_from_phase_split(ctx, F32, L, pos, mom, lags, log_fn)
     pos.free()
     p0 = float(np.linalg.norm(p[:, 0]))
     for rep in range(2):
@@ -415,6 +416,43 @@ def stats_rates(ctx, F32, L, x, p, dt, log_fn, cpu=True):
     mom.free()
     fld.close()
     return out
+
+
+def dmumu_from_phase_split(ctx, F32, L, pos, mom, lags, log_fn):
+    """D_mumu from a device-resident FP32 phase-space cache (positions and
+    momenta, (nsave, 3, n)), as run_combined_full does per batch: mu of every
+    sample reconstructed into a device buffer (reconstruct_mu_chunk, B
+    trilinear from the FP32 planar field), then the pair walk over it.  Times
+    both steps and sets the buffer's HBM round trip (one write, one read)
+    beside them: the most that computing mu inside the pair walk could save."""
+    import numpy as np
+
+    from particle_transport_amd import dmumu
+    from particle_transport_amd._lib import DeviceBuffer
+    from particle_transport_amd.runner import Field
+    nsave, _, n = pos.shape
+    fld = Field(ctx, F32, L, "f32_planar")
+    mu = DeviceBuffer(ctx, (n, nsave), np.float32)
+    edges = np.linspace(0.0, 1.0, 41)
+    for rep in range(2):  # the first pass also builds device tables
+        t0 = time.perf_counter()
+        dmumu.reconstruct_mu_chunk(ctx, fld, pos, mom, out=mu)
+        rec_s = time.perf_counter() - t0
+        acc = dmumu.new_accumulators(len(lags), 40)
+        t0 = time.perf_counter()
+        dmumu.process_mu_chunk_dmumu(ctx, mu, lags, edges, acc, "sliding", True, layout="time_fastest")
+        walk_s = time.perf_counter() - t0
+    fld.close()
+    mu.free()
+    buf = n * nsave * 4
+    rt_s = 2 * buf / (HBM_PEAK_GBS * 1e9)
+    res = {"shape": f"{n} particles x {nsave} samples FP32 phase space, {len(lags)} lags, 40 bins",
+           "reconstruct_s": rec_s, "walk_s": walk_s, "mu_buffer_bytes": buf,
+           "round_trip_s_at_hbm_peak": rt_s, "round_trip_fraction": rt_s / (rec_s + walk_s),
+           "phase_space_bytes_read": 2 * 3 * n * nsave * 4}
+    log_fn(f"[bench] D_mumu from phase space: reconstruct {rec_s * 1e3:.2f} ms, walk {walk_s * 1e3:.2f} ms, "
+           f"mu round trip {rt_s * 1e3:.3f} ms at the HBM peak")
+    return res
 
 
 def injection_rates(ctx, F32, L, energy_gev, log_fn, n=4194304):

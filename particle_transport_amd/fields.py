@@ -16,12 +16,20 @@ and rescaled to the requested vector rms.  Fields are generated periodic on
 node grid range(0, L, length=n) is seamless across the periodic wrap.
 Values are in the reference's input units: B in microgauss, u in km/s.
 
-``mode_decompose=True`` also returns Alfven/fast/slow parts.  Per k-mode the
-solenoidal fluctuation is split on the orthonormal pair e_A = k x z / |k x z|
-and e_P = k x e_A (the k-z plane): Alfven = (f.e_A) e_A; the e_P part is
-shared as slow = cos^2(theta) (f.e_P) e_P and fast = sin^2(theta) (f.e_P) e_P,
-theta the angle between k and z.  Each part is divergence-free and the parts
-sum to the total fluctuation; each mode field is B0 z + its own part.
+``mode_decompose=True`` builds the mode-decomposed set instead (Cho &
+Lazarian 2002, 2003): one seeded Gaussian vector field V with the same k^-5/3
+spectrum (not made solenoidal) is projected, per k-mode, onto the
+eigen-directions of the linear ideal-MHD wave operator with B0 along z and
+c_s^2 / V_A^2 = alpha = (5/3) beta / 2: the Alfven direction xi_A = k x z / |k
+x z| and the fast and slow directions xi_f, xi_s in the plane of k and z
+(mhd_eigendirections).  Each projection V_m = (V.xi_m) xi_m is that mode's
+displacement; its flow is u_m ~ V_m and its magnetic fluctuation the
+displacement's field-line signature dB_m ~ i k x (V_m x z) / |k|, which is
+perpendicular to k, so every magnetic part is divergence-free.  The parts sum
+to the total (V and i k x (V x z) / |k|), both fluctuations are rescaled by one
+factor each to the requested rms, and each mode field is B0 z + its own parts.
+The total of a mode-decomposed set is therefore not the plain recipe's total
+for the same seed.
 """
 from __future__ import annotations
 
@@ -127,9 +135,10 @@ def _wavenumbers(M: int, xp, device=None):
     return KX, KY, KZ
 
 
-def _solenoidal_spectrum(M, kmin, kmax, rng, xp, device):
-    """Three rfft components of a divergence-free Gaussian field with 3-D power
-    ~ k^(-11/3) on the shell band [kmin, kmax]."""
+def _solenoidal_spectrum(M, kmin, kmax, rng, xp, device, solenoidal=True):
+    """Three rfft components of a divergence-free (or, solenoidal=False,
+    unprojected) Gaussian field with 3-D power ~ k^(-11/3) on the shell band
+    [kmin, kmax]."""
     if xp is np:
         noise = rng.standard_normal((3, M, M, M))
         F = np.fft.rfftn(noise, axes=(1, 2, 3))
@@ -145,34 +154,77 @@ def _solenoidal_spectrum(M, kmin, kmax, rng, xp, device):
     safe = xp.where(k2 > 0, k2, xp.ones_like(k2))
     amp = xp.where(band, safe ** (-11.0 / 12.0), xp.zeros_like(k2))
     F = F * amp
-    kdotF = (KX * F[0] + KY * F[1] + KZ * F[2]) / safe
-    F = xp.stack([F[0] - KX * kdotF, F[1] - KY * kdotF, F[2] - KZ * kdotF])
+    if solenoidal:
+        kdotF = (KX * F[0] + KY * F[1] + KZ * F[2]) / safe
+        F = xp.stack([F[0] - KX * kdotF, F[1] - KY * kdotF, F[2] - KZ * kdotF])
     return F, (KX, KY, KZ, k2)
 
 
-def _mode_split(F, K, xp):
+def mhd_eigendirections(KX, KY, KZ, alpha, xp=np):
+    """Unit displacement directions of the three linear ideal-MHD waves for
+    wavevectors (KX, KY, KZ) with B0 along z and c_s^2 / V_A^2 = alpha: the
+    Alfven direction xi_A = k x z / |k x z| and the fast and slow directions
+    in the plane of k and z, from Cho & Lazarian (2003, MNRAS 345, 325, eq. 3-4):
+    xi_f ~ (1 + a + sqrt(D)) k_perp e_perp + (-1 + a + sqrt(D)) k_par z and
+    xi_s ~ (1 + a - sqrt(D)) k_perp e_perp + (-1 + a - sqrt(D)) k_par z,
+    D = (1 + a)^2 - 4 a cos^2(theta), with k_perp, k_par the components of k/|k|.
+    Where one of the two vanishes (k along z) its direction is taken as the
+    in-plane perpendicular of the other (where both do, k along z at alpha =
+    1, fast = e_perp and slow = z); k along z also takes xi_A = x and e_perp
+    = y.  Returns three (x, y, z) component tuples; the sign of each
+    direction is arbitrary (projections do not depend on it)."""
+    k2 = KX * KX + KY * KY + KZ * KZ
+    one = xp.ones_like(k2)
+    zero = xp.zeros_like(k2)
+    kmag = xp.sqrt(xp.where(k2 > 0, k2, one))
+    kperp_abs = xp.sqrt(KX * KX + KY * KY)
+    has_perp = kperp_abs > 0
+    kp_safe = xp.where(has_perp, kperp_abs, one)
+    eA = (xp.where(has_perp, KY / kp_safe, one), xp.where(has_perp, -KX / kp_safe, zero), zero)
+    ep = (xp.where(has_perp, KX / kp_safe, zero), xp.where(has_perp, KY / kp_safe, one), zero)
+    kperp = kperp_abs / kmag
+    kpar = KZ / kmag
+    sqD = xp.sqrt(xp.clip((1 + alpha) ** 2 - 4 * alpha * kpar * kpar, 0, None))
+    # in-plane coordinates (along e_perp, along z) of the two directions
+    fp, fz = (1 + alpha + sqD) * kperp, (-1 + alpha + sqD) * kpar
+    sp, sz = (1 + alpha - sqD) * kperp, (-1 + alpha - sqD) * kpar
+    fn = xp.sqrt(fp * fp + fz * fz)
+    sn = xp.sqrt(sp * sp + sz * sz)
+    use_f = fn >= sn
+    # normalise the larger one, take the other as its in-plane perpendicular
+    big_p = xp.where(use_f, fp, sp)
+    big_z = xp.where(use_f, fz, sz)
+    bn = xp.sqrt(big_p * big_p + big_z * big_z)
+    # both vanish only for k along z at alpha = 1, where the two waves are
+    # degenerate: fast transverse (e_perp), slow along z
+    degenerate = bn <= 0
+    big_p = xp.where(degenerate, one, big_p / xp.where(degenerate, one, bn))
+    big_z = xp.where(degenerate, zero, big_z / xp.where(degenerate, one, bn))
+    use_f = use_f | degenerate
+    fp_u = xp.where(use_f, big_p, big_z)
+    fz_u = xp.where(use_f, big_z, -big_p)
+    sp_u = xp.where(use_f, -big_z, big_p)
+    sz_u = xp.where(use_f, big_p, big_z)
+    xi_f = (fp_u * ep[0], fp_u * ep[1], fz_u)
+    xi_s = (sp_u * ep[0], sp_u * ep[1], sz_u)
+    return eA, xi_f, xi_s
+
+
+def _mode_parts(V, K, xp, alpha):
+    """Per mode (alfven, fast, slow): the flow part V_m = (V.xi_m) xi_m and the
+    magnetic part i k x (V_m x z) / |k| = i (V_m k_z - z (k.V_m)) / |k|."""
     KX, KY, KZ, k2 = K
-    kperp = xp.sqrt(KX * KX + KY * KY)
-    kmag = xp.sqrt(k2)
-    one = xp.ones_like(kperp)
-    kp_safe = xp.where(kperp > 0, kperp, one)
-    km_safe = xp.where(kmag > 0, kmag, one)
-    # e_A = k x z / |k x z| = (ky, -kx, 0)/kperp ; k parallel to z -> e_A = x
-    eAx = xp.where(kperp > 0, KY / kp_safe, one)
-    eAy = xp.where(kperp > 0, -KX / kp_safe, xp.zeros_like(kperp))
-    eAz = xp.zeros_like(kperp)
-    khx, khy, khz = KX / km_safe, KY / km_safe, KZ / km_safe
-    # e_P = khat x e_A
-    ePx = khy * eAz - khz * eAy
-    ePy = khz * eAx - khx * eAz
-    ePz = khx * eAy - khy * eAx
-    a = F[0] * eAx + F[1] * eAy + F[2] * eAz
-    p = F[0] * ePx + F[1] * ePy + F[2] * ePz
-    cos2 = xp.where(kmag > 0, (KZ / km_safe) ** 2, one)
-    alf = xp.stack([a * eAx, a * eAy, a * eAz])
-    slow = xp.stack([cos2 * p * ePx, cos2 * p * ePy, cos2 * p * ePz])
-    fast = xp.stack([(1 - cos2) * p * ePx, (1 - cos2) * p * ePy, (1 - cos2) * p * ePz])
-    return {"alfven": alf, "fast": fast, "slow": slow}
+    kmag = xp.sqrt(xp.where(k2 > 0, k2, xp.ones_like(k2)))
+    khx, khy, khz = KX / kmag, KY / kmag, KZ / kmag
+    dirs = dict(zip(("alfven", "fast", "slow"), mhd_eigendirections(KX, KY, KZ, alpha, xp)))
+    out = {}
+    for name, e in dirs.items():
+        a = V[0] * e[0] + V[1] * e[1] + V[2] * e[2]
+        flow = xp.stack([a * e[0], a * e[1], a * e[2]])
+        kdot = khx * flow[0] + khy * flow[1] + khz * flow[2]
+        mag = xp.stack([flow[0] * khz, flow[1] * khz, flow[2] * khz - kdot]) * 1j
+        out[name] = (mag, flow)
+    return out
 
 
 def _to_real(F, M, xp):
@@ -194,13 +246,14 @@ def _with_seam(r, xp):
 
 def generate_turbulence(n_nodes: int, *, B0_uG: float = 5.0, dB_over_B0: float = 0.5, u_rms_kms: float = 20.0,
                         kmin: float = 1.0, kmax: float = 16.0, seed: int = 1234, device=None,
-                        dtype="float64", mode_decompose: bool = False):
+                        dtype="float64", mode_decompose: bool = False, mode_beta: float = 1.0):
     """Synthetic B (microgauss) and u (km/s) on an n_nodes^3 node grid.
 
     Returns a dict with key "total" (and "alfven"/"fast"/"slow" when
     mode_decompose) mapping to a 6-tuple (Bx, By, Bz, ux, uy, uz), each shaped
     (nx, ny, nz) with x fastest in memory (the Julia layout).  device=None
     uses numpy; a torch device string ("cuda", "cpu") uses torch.fft there.
+    mode_beta: plasma beta of the mode decomposition (c_s^2/V_A^2 = 5/6 beta).
     """
     M = int(n_nodes) - 1
     if M < 2:
@@ -212,14 +265,23 @@ def generate_turbulence(n_nodes: int, *, B0_uG: float = 5.0, dB_over_B0: float =
         xp = torch
         rng = torch.Generator(device=device)
         rng.manual_seed(int(seed))
-    FB, K = _solenoidal_spectrum(M, kmin, kmax, rng, xp, device)
-    FU, _ = _solenoidal_spectrum(M, kmin, kmax, rng, xp, device)
-
     def realise(Fc):
         return _to_real(Fc, M, xp)
 
-    rB = realise(FB)
-    rU = realise(FU)
+    if mode_decompose:
+        V, K = _solenoidal_spectrum(M, kmin, kmax, rng, xp, device, solenoidal=False)
+        parts = _mode_parts(V, K, xp, 5.0 / 6.0 * float(mode_beta))
+        del V
+        real_parts = {name: (realise(mag), realise(flow)) for name, (mag, flow) in parts.items()}
+        del parts
+        rB = sum(b for b, _ in real_parts.values())
+        rU = sum(u for _, u in real_parts.values())
+    else:
+        FB, K = _solenoidal_spectrum(M, kmin, kmax, rng, xp, device)
+        FU, _ = _solenoidal_spectrum(M, kmin, kmax, rng, xp, device)
+        rB = realise(FB)
+        rU = realise(FU)
+        del FB, FU
     sB = float(dB_over_B0 * B0_uG) / float(xp.sqrt((rB * rB).sum(0).mean()))
     sU = float(u_rms_kms) / float(xp.sqrt((rU * rU).sum(0).mean()))
 
@@ -239,10 +301,8 @@ def generate_turbulence(n_nodes: int, *, B0_uG: float = 5.0, dB_over_B0: float =
 
     result = {"total": pack(rB, rU)}
     if mode_decompose:
-        mb = _mode_split(FB, K, xp)
-        mu = _mode_split(FU, K, xp)
         for name in ("alfven", "fast", "slow"):
-            result[name] = pack(realise(mb[name]), realise(mu[name]))
+            result[name] = pack(*real_parts[name])
     return result
 
 

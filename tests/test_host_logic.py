@@ -194,12 +194,67 @@ def test_generator_invariants():
     kk = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
     power = sum(np.abs(f) ** 2 for f in Fs)
     assert power[(kk > 5) | (kk < 1)].max() <= 1e-20 * power.max()
-    # modes sum to the total fluctuation and are each divergence-free
-    s = sum(core(out[m][0]) for m in ("alfven", "fast", "slow"))
-    assert np.allclose(s, core(Bx), atol=1e-12)
+    # modes sum to the total fluctuation (B and u); every magnetic part is
+    # divergence-free
+    for c in range(6):
+        s = sum(core(out[m][c]) - (5.0 if c == 2 else 0.0) for m in ("alfven", "fast", "slow"))
+        assert np.allclose(s, core(out["total"][c]) - (5.0 if c == 2 else 0.0), atol=1e-12)
     for m in ("alfven", "fast", "slow"):
         Fm = [np.fft.fftn(core(out[m][i]) - (5.0 if i == 2 else 0.0)) for i in range(3)]
         assert np.abs(KX * Fm[0] + KY * Fm[1] + KZ * Fm[2]).max() <= 1e-9 * max(np.abs(f).max() for f in Fs)
+    # Alfven parts are transverse to B0 (no z component in B or u); the flow
+    # of the fast and slow parts is orthogonal mode by mode
+    Fa = [np.fft.fftn(core(out["alfven"][i]) - (5.0 if i == 2 else 0.0)) for i in range(6)]
+    assert np.abs(Fa[2]).max() <= 1e-9 * np.abs(Fa[0]).max() and np.abs(Fa[5]).max() <= 1e-9 * np.abs(Fa[3]).max()
+    Ff = [np.fft.fftn(core(out["fast"][i])) for i in range(3, 6)]
+    Fsl = [np.fft.fftn(core(out["slow"][i])) for i in range(3, 6)]
+    dot = sum(Ff[i] * np.conj(Fsl[i]) for i in range(3))
+    assert np.abs(dot).max() <= 1e-9 * max(np.abs(f).max() for f in Ff) * max(np.abs(f).max() for f in Fsl)
+
+
+def _mhd_operator(k, alpha):
+    """Linear ideal-MHD wave operator on the displacement for B0 along z (V_A =
+    1, c_s^2 = alpha): omega^2 xi = K xi with K = alpha k k^T + (k.z)^2 I
+    + k k^T - (k.z)(k z^T + z k^T) (force of the perturbed pressure and
+    Lorentz force, textbook form)."""
+    z = np.array([0.0, 0.0, 1.0])
+    kz = k @ z
+    return alpha * np.outer(k, k) + kz * kz * np.eye(3) + np.outer(k, k) - kz * (np.outer(k, z) + np.outer(z, k))
+
+
+@pytest.mark.parametrize("alpha", [0.05, 5.0 / 6.0, 1.0, 4.0])
+def test_mode_directions_are_mhd_eigenvectors(alpha):
+    """The generator's Alfven / fast / slow directions are orthonormal
+    eigenvectors of the MHD wave operator, with the fast eigenvalue (omega^2)
+    the largest and the slow one the smallest, for random k and along the
+    axes."""
+    rng = np.random.default_rng(11)
+    ks = list(rng.standard_normal((40, 3))) + [np.array(v, float) for v in ((0, 0, 1), (0, 0, -2), (1, 0, 0),
+                                                                              (0, 3, 0), (1, 1, 0), (1, 0, 1e-3))]
+    for k in ks:
+        k = k / np.linalg.norm(k)
+        (eA, ef, es) = [np.array([float(c) for c in d]) for d in fields.mhd_eigendirections(
+            np.array(k[0]), np.array(k[1]), np.array(k[2]), alpha)]
+        Q = np.stack([eA, ef, es])
+        assert np.allclose(Q @ Q.T, np.eye(3), atol=1e-12)
+        Kop = _mhd_operator(k, alpha)
+        lam = [float(e @ Kop @ e) for e in (eA, ef, es)]
+        for e, l in zip((eA, ef, es), lam):
+            assert np.allclose(Kop @ e, l * e, atol=1e-12)
+        assert lam[0] == pytest.approx(k[2] ** 2, abs=1e-12)  # Alfven: omega = k_par V_A
+        assert lam[1] >= lam[0] - 1e-12 and lam[2] <= lam[0] + 1e-12
+
+
+def test_slow_mode_carries_no_field_at_low_beta():
+    """At beta -> 0 the slow wave is sound along B0: its displacement is
+    along z and it bends no field line, so the slow magnetic part vanishes
+    while its flow does not."""
+    out = fields.generate_turbulence(17, kmin=1, kmax=5, seed=3, mode_decompose=True, mode_beta=0.0)
+    core = lambda a: a[:-1, :-1, :-1]  # noqa: E731
+    slow_b = max(np.abs(core(out["slow"][i]) - (5.0 if i == 2 else 0.0)).max() for i in range(3))
+    fast_b = max(np.abs(core(out["fast"][i]) - (5.0 if i == 2 else 0.0)).max() for i in range(3))
+    slow_u = max(np.abs(core(out["slow"][i])).max() for i in range(3, 6))
+    assert slow_b <= 1e-12 * fast_b and slow_u > 1.0
 
 
 def test_generator_spectrum_slope():
