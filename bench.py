@@ -561,7 +561,8 @@ def serving_run(args, log_fn, fmt="bf16_brick"):
     format the D_mumu check selected): Poisson arrivals at 20/s, per-request
     field/energy/count/duration/injection/output drawn as the serving
     configuration specifies, 200 GB budget.  Runs the first
-    args.serving_requests requests of the stream in real time."""
+    args.serving_requests requests of the stream in real time.  With
+    args.cell_tables the fields carry cell tables (results bit-identical)."""
     from particle_transport_amd import serving
     from particle_transport_amd.fields import box_length_m
     from particle_transport_amd.runner import Context, Field
@@ -571,7 +572,7 @@ def serving_run(args, log_fn, fmt="bf16_brick"):
     for k in range(4):
         log_fn(f"[serve] field {k}")
         F = make_fields(args.cells + 1, 1.0, 64.0, seed=3000 + k)[0]
-        flds.append(Field(push_ctx, F, L, fmt))
+        flds.append(Field(push_ctx, F, L, fmt, cell_table=args.cell_tables))
         del F
     B0s = [f.B0 for f in flds]
     reqs = serving.make_stream(args.serving_requests, 4, B0s, rate_hz=20.0, seed=77)
@@ -586,9 +587,46 @@ def serving_run(args, log_fn, fmt="bf16_brick"):
             "wall_s": res.wall_s, "tick_kernel_s": res.kernel_ms / 1e3,
             "tick_kernel_particle_steps_per_s": res.particle_steps / max(res.kernel_ms / 1e3, 1e-12),
             "product_thread_busy_s": res.product_busy_s, "tick_steps": args.serving_tick_steps,
-            "failed_requests": len(res.failed), "field_format": fmt,
+            "failed_requests": len(res.failed), "field_format": fmt, "cell_tables": bool(args.cell_tables),
+            "peak_reserved_bytes": res.peak_reserved_bytes,
             "stream": f"Poisson 20/s, 4 resident {fmt} fields, 200 GB budget, "
             "energy log-U[1 GeV,100 TeV], n log-U[1e4,1e6], duration U[10,500] T_g0, mu+D_mumu(64x20) or D_pp"}
+
+
+def reload_regime(ctx, F32, L, log_fn, n=1048576, launches=20, S=200):
+    """The production pusher where particles change cell often: 1,048,576
+    protons at 10 and 100 TeV in the 512^3 field, BF16 and FP32 bricks, with
+    the stored nodes (wave-cooperative reload) and with a cell table, 1 + 20
+    launches of 200 pushes each (mu every 20)."""
+    import numpy as np
+
+    from particle_transport_amd._lib import DeviceBuffer
+    from particle_transport_amd.injection import M_P, Q_E
+    from particle_transport_amd.runner import Ensemble, Field
+    out = {"shape": f"{n} protons, {launches} launches of {S} pushes, 512^3 cells", "rates": {}}
+    for E in (1e4, 1e5):
+        x, p, gamma = make_particles(n, L, E, seed=99)
+        for fmt in ("bf16_brick", "f32_brick"):
+            fld = Field(ctx, F32, L, fmt)
+            dt = (2 * math.pi / S) / (Q_E * fld.B0 / (gamma * M_P))
+            for table in (False, True):
+                fld.set_cell_table(table)
+                ens = Ensemble(ctx, fld, x, p, pusher="fast")
+                mu = DeviceBuffer(ctx, (S // 20, n), np.float32)
+                total = (launches + 1) * S + 1
+                kms = 0.0
+                for i in range(launches + 1):
+                    r = ens.advance(dt=dt, first_step=1 + i * S, n_steps=S, total_steps=total, stride=20,
+                                    out_kind="mu", out_a=mu, steps_per_launch=S)
+                    if i:
+                        kms += r.kernel_ms
+                ens.close()
+                mu.free()
+                key = f"{E:g}GeV/{fmt}/{'cell_table' if table else 'nodes'}"
+                out["rates"][key] = {"avg_launch_ms": kms / launches, "particle_steps_per_s": n * S * launches / (kms / 1e3)}
+                log_fn(f"[bench] reload regime {key}: {kms / launches:.3f} ms per launch")
+            fld.close()
+    return out
 
 
 def modes_run(args, log_fn):
@@ -604,7 +642,7 @@ def modes_run(args, log_fn):
     L = fields.box_length_m(10.0)
     ctx, sctx = Context(0), Context(0)
     gen = make_fields(args.mode_cells + 1, 0.5, 16.0, seed=555, modes=True)
-    flds = [Field(ctx, F, L, "f32_brick") for F in gen]
+    flds = [Field(ctx, F, L, "f32_brick", cell_table=args.cell_tables) for F in gen]
     del gen
     B0 = flds[0].B0  # shared total-field clock
     rng = np.random.default_rng(96)
@@ -628,6 +666,7 @@ def modes_run(args, log_fn):
     sctx.close()
     ctx.close()
     return {"campaigns": res.requests, "particle_steps_per_s": res.particle_steps_per_s, "wall_s": res.wall_s,
+            "cell_tables": bool(args.cell_tables),
             "latency_p50_s": res.latency_p50_s, "latency_p99_s": res.latency_p99_s,
             "config": f"4 x {args.mode_cells}^3 FP32 fields (total/alfven/fast/slow), 6 energies x 4 injections, "
                       f"{args.mode_gp} T_g0 each"}
@@ -658,6 +697,10 @@ def main():
     ap.add_argument("--serving-tick-steps", type=int, default=200,
                     help="pushes per serving tick (one fused launch over all running requests)")
     ap.add_argument("--no-modes", action="store_true", help="skip the 96-campaign mode-decomposed batch")
+    ap.add_argument("--no-reload-regime", action="store_true",
+                    help="skip the 10 / 100 TeV legs with and without cell tables")
+    ap.add_argument("--cell-tables", action="store_true",
+                    help="serve (and run the mode batch) on fields with cell tables")
     ap.add_argument("--mode-cells", type=int, default=256)
     ap.add_argument("--mode-gp", type=int, default=10)
     ap.add_argument("--pmc-probe", action="store_true",
@@ -770,6 +813,7 @@ def main():
         log("[bench] D_mumu check on 65,536 particles")
         check = dmumu_check(ctx, F32, L, x, p, dt, 65536, 4001, stride)
 
+    reload = None if args.no_reload_regime else reload_regime(ctx, F32, L, log)
     stats = None if args.no_stats else stats_rates(ctx, F32, L, x, p, dt, log, cpu=not args.no_cpu)
     inject = None if args.no_stats else injection_rates(ctx, F32, L, args.energy_gev, log)
     pcie = None if args.no_pcie else pcie_inclusive(F32, L, x, p, args.energy_gev, log)
@@ -867,6 +911,7 @@ def main():
                       "speedup_of_headline": value / plain_rate,
                       "speedup_of_bf16": timing["bf16_brick"]["value"] / plain_rate if "bf16_brick" in timing else None},
         "dmumu_check": check,
+        "reload_regime": reload,
         "statistics": stats,
         "injection": inject,
         "pcie_inclusive": pcie,

@@ -116,6 +116,19 @@ int pt_field_create(pt_ctx *ctx, int64_t nx, int64_t ny, int64_t nz, double box_
                     pt_field **out);
 int pt_field_destroy(pt_field *field);
 
+/* Cell table (no counterpart in the reference, which samples the field's
+ * nodes every push): enable = 1 builds, for every cell of an FP32 planar or
+ * brick field, the 6 x 8 FP32 trilinear coefficients that the production
+ * pushers (PT_PUSH_FAST and the serving tick) otherwise rebuild from the
+ * stored nodes whenever a particle changes cell, 192 bytes per cell in 8^3
+ * cell bricks (25.8 GB for a 512^3-cell field); those pushers then load a
+ * cell's record instead.  Trajectories are bit-identical with and without the
+ * table; it pays where particles change cell every few tens of pushes
+ * (10-100 TeV protons in the benchmark fields).  enable = 0 frees it;
+ * pt_field_destroy frees it too.  A server's fields must all have one or all
+ * have none. */
+int pt_field_cell_table(pt_field *field, int enable);
+
 /* reference_B0_T (time_units.jl:18-20): mean over all nodes of
  * sqrt(Bx^2+By^2+Bz^2), summed in FP64 on the device. */
 int pt_field_mean_abs_b(const pt_field *field, double *B0_T);

@@ -143,6 +143,11 @@ function field_create(ctx, fields, box_length_m::Real; storage=STORE_F64_PLANAR)
     return h[]
 end
 field_destroy(field) = check(ccall((:pt_field_destroy, LIB), Cint, (Ptr{Cvoid},), field))
+# Optional cell table (192 bytes per cell): the production pushers then load a
+# cell's coefficients on a cell change instead of rebuilding them; results are
+# bit-identical.  No reference counterpart.
+field_cell_table(field, enable::Bool) =
+    check(ccall((:pt_field_cell_table, LIB), Cint, (Ptr{Cvoid}, Cint), field, enable ? 1 : 0))
 
 function field_mean_abs_b(field)
     b = Ref{Float64}(0.0)

@@ -24,7 +24,7 @@ OUT = {"none": 0, "phase": 1, "phase-space": 1, "mu": 2}
 # Every symbol include/pt_hip.h declares.
 EXPORTS = (
     "pt_abi_version", "pt_last_error", "pt_ctx_create", "pt_ctx_destroy", "pt_ctx_bytes_in_use", "pt_device_memory",
-    "pt_field_create", "pt_field_destroy", "pt_field_mean_abs_b", "pt_mean_abs_b",
+    "pt_field_create", "pt_field_destroy", "pt_field_cell_table", "pt_field_mean_abs_b", "pt_mean_abs_b",
     "pt_ens_create", "pt_ens_create_injected", "pt_inject", "pt_ens_destroy", "pt_ens_advance",
     "pt_ens_get_state", "pt_ens_last_timing", "pt_ens_sort",
     "pt_mu_from_phase", "pt_dmumu_accumulate", "pt_dmumu_accumulate_noalloc", "pt_dpp_accumulate",
@@ -85,6 +85,7 @@ def load():
         "pt_field_create": (_int, [_vp, _i64, _i64, _i64, ctypes.c_double, _int, _int, _int,
                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
         "pt_field_destroy": (_int, [_vp]),
+        "pt_field_cell_table": (_int, [_vp, _int]),
         "pt_field_mean_abs_b": (_int, [_vp, _dp]),
         "pt_mean_abs_b": (_int, [_vp, _i64, _int, _int, _vp, _vp, _vp, _dp]),
         "pt_ens_create": (_int, [_vp, _vp, _int, _i64, _int, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),

@@ -40,6 +40,7 @@ SOURCES = {
     "pt_serve.hip": [],
     "pt_sort.hip": [],
     "pt_inject.hip": ["-ffp-contract=off"],
+    "pt_cells.hip": [],
 }
 
 

@@ -107,6 +107,12 @@ struct pt_field {
     size_t coarse_bytes = 0;
     int32_t cn[3] = {0, 0, 0};   // coarse cells per axis
     double B0 = 0.0;             // FP64 mean |B| of the source values
+    // Optional cell table (pt_field_cell_table): per cell, the 6 x 8 FP32
+    // polynomial coefficients the pushers would compute from the stored
+    // nodes, in 8^3-cell bricks (cell_record_index); null when not built.
+    float *cells = nullptr;
+    size_t cells_bytes = 0;
+    int32_t cb[3] = {0, 0, 0};   // cell bricks per axis
 };
 
 struct pt_ens {
@@ -224,6 +230,17 @@ void *dev_alloc(pt_ctx *ctx, size_t bytes);
 void dev_free(pt_ctx *ctx, void *p, size_t bytes);
 
 // -------------------------------------------------------- device helpers ---
+// Record of cell (i, j, k) in a cell table: cells grouped in 8^3 bricks
+// (cb bricks per axis), 48 floats per record ([component][coefficient]).
+__host__ __device__ inline int64_t cell_record_index(const int32_t cb[3], int64_t i, int64_t j, int64_t k) {
+    const int64_t brick = ((k >> 3) * cb[1] + (j >> 3)) * cb[0] + (i >> 3);
+    return (brick << 9) + (((k & 7) << 6) | ((j & 7) << 3) | (i & 7));
+}
+constexpr int kCellFloats = 48;
+// Template tag of the pusher kernels that read a cell table instead of the
+// stored nodes (not a storage kind of the C ABI).
+constexpr int kStoreCells = 16;
+
 __host__ __device__ inline int64_t brick_node_index(const int64_t nb[3], int64_t i, int64_t j,
                                                     int64_t k) {
     int64_t brick = ((k >> 3) * nb[1] + (j >> 3)) * nb[0] + (i >> 3);

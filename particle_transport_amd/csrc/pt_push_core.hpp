@@ -23,6 +23,8 @@ struct FieldView {
     float scale[6];       // decoded value = stored * scale (+ the coarse part, 16-bit bricks)
     const float *coarse;  // 16-bit bricks: coarse-part coefficients [coarse cell][6][8]
     int32_t cn[3];        // coarse cells per axis
+    const float *cells;   // cell table (kStoreCells kernels), else null
+    int32_t cb[3];        // cell bricks per axis
 };
 
 __host__ inline FieldView make_field_view(const pt_field *f) {
@@ -35,6 +37,8 @@ __host__ inline FieldView make_field_view(const pt_field *f) {
     }
     for (int c = 0; c < 6; ++c) fv.scale[c] = f->sc.scale[c];
     fv.coarse = f->coarse;
+    fv.cells = f->cells;
+    for (int c = 0; c < 3; ++c) fv.cb[c] = f->cb[c];
     return fv;
 }
 
@@ -496,7 +500,8 @@ constexpr int kReloadBatch = 10;
 struct ReloadLds {
     int4 slot[kReloadBatch];
     float coef[kReloadBatch][6][8];
-    float scale[6];  // per component: decode scale times the physical factor
+    float scale[6];  // per component: decode scale (times the physical factor unless 16-bit)
+    float fac[6];    // per component: physical factor (16-bit bricks)
 };
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -515,16 +520,48 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
                                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
 }
 
-// The per-component scales the helper lanes apply, written once into the
-// wave's LDS area (they are read only on a reload pass, so no lane keeps
-// them in a register across the step loop).
+// The per-component scales and factors the helper lanes apply, written once
+// into the wave's LDS area (they are read only on a reload pass, so no lane
+// keeps them in a register across the step loop).
+template <int STORE>
 __device__ __forceinline__ void reload_init(const FieldView &fv, const Phys &ph, ReloadLds *lds) {
     const int lane = lane_id();
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-        if (lane == c) lds->scale[c] = fv.scale[c] * ph.fac[c];
+        if (lane == c) {
+            lds->scale[c] = kHierarchical<STORE> ? fv.scale[c] : fv.scale[c] * ph.fac[c];
+            lds->fac[c] = ph.fac[c];
+        }
     }
     wave_lds_sync();
+}
+
+// A particle's coefficients are the cell's field-unit coefficients (what a
+// cell table holds) times the component's physical factor.  FP32 storages
+// have scale 1, so folding the factor into the decode scale gives the same
+// product; 16-bit bricks form the field-unit values (residual plus coarse
+// part) first and multiply after.  Every reload path, with or without a cell
+// table, therefore gives the same values bit for bit.
+__device__ __forceinline__ void apply_factor(float a[8], float fac) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] *= fac;
+}
+
+// The 48 coefficients of gather cell (gi, gj, gk) from the field's cell table.
+__device__ __forceinline__ void table_coeffs(const FieldView &fv, int32_t gi, int32_t gj, int32_t gk,
+                                             const Phys &ph, float a[6][8]) {
+    const float4 *src = reinterpret_cast<const float4 *>(fv.cells) + cell_record_index(fv.cb, gi, gj, gk) * 12;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+        const float4 v = src[q];
+        float *d = &a[q >> 1][(q & 1) * 4];
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) apply_factor(a[c], ph.fac[c]);
 }
 
 // Per-lane reload of the coefficients of the particle's own cell, all loads
@@ -536,11 +573,21 @@ __device__ __forceinline__ void lane_reload(const int32_t cell[3], const FieldVi
     const int32_t gi = gather_cell(cell[0], g.nc[0], b[0]);
     const int32_t gj = gather_cell(cell[1], g.nc[1], b[1]);
     const int32_t gk = gather_cell(cell[2], g.nc[2], b[2]);
-    float sv[6][8];
-    load_corners<STORE>(fv, gi, gj, gk, sv);
+    if constexpr (STORE == kStoreCells) {
+        table_coeffs(fv, gi, gj, gk, ph, a);
+    } else {
+        float sv[6][8];
+        load_corners<STORE>(fv, gi, gj, gk, sv);
 #pragma unroll
-    for (int c = 0; c < 6; ++c)
-        cell_coeffs<STORE>(fv, gi, gj, gk, c, sv[c], fv.scale[c] * ph.fac[c], a[c]);
+        for (int c = 0; c < 6; ++c) {
+            if constexpr (kHierarchical<STORE>) {
+                cell_coeffs<STORE>(fv, gi, gj, gk, c, sv[c], fv.scale[c], a[c]);
+                apply_factor(a[c], ph.fac[c]);
+            } else {
+                cell_coeffs<STORE>(fv, gi, gj, gk, c, sv[c], fv.scale[c] * ph.fac[c], a[c]);
+            }
+        }
+    }
     if (b[0] != 0.0f || b[1] != 0.0f || b[2] != 0.0f) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) shift_coeffs(a[c], b[0], b[1], b[2]);
@@ -549,7 +596,18 @@ __device__ __forceinline__ void lane_reload(const int32_t cell[3], const FieldVi
 
 template <int STORE>
 __device__ __forceinline__ void coop_reload(int &need, const int32_t cell[3], const FieldView &fv,
-                                            const SplitGeom &g, ReloadLds *lds, float a[6][8]) {
+                                            const SplitGeom &g, const Phys &ph, ReloadLds *lds, float a[6][8]) {
+    if constexpr (STORE == kStoreCells) {
+        // a cell table needs no helpers: each lane that changed cell loads
+        // its own 48 coefficients
+        if (__ballot(need != 0)) {
+            if (need) {
+                lane_reload<STORE>(cell, fv, g, ph, a);
+                need = 0;
+            }
+        }
+        return;
+    }
     uint64_t pending = __ballot(need != 0);
     while (pending) {
         const int rank = lanes_below(pending);
@@ -575,6 +633,7 @@ __device__ __forceinline__ void coop_reload(int &need, const int32_t cell[3], co
             float sv[8], c8[8];
             load_component_corners<STORE>(fv, t.x, t.y, t.z, comp, sv);
             cell_coeffs<STORE>(fv, t.x, t.y, t.z, comp, sv, lds->scale[comp], c8);
+            if constexpr (kHierarchical<STORE>) apply_factor(c8, lds->fac[comp]);
             if (t.w) shift_coeffs(c8, (t.w & 1) ? 1.0f : 0.0f, (t.w & 2) ? 1.0f : 0.0f, (t.w & 4) ? 1.0f : 0.0f);
             float4 *dst = reinterpret_cast<float4 *>(lds->coef[entry][comp]);
             dst[0] = make_float4(c8[0], c8[1], c8[2], c8[3]);
@@ -671,7 +730,7 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float co
 #pragma unroll
     for (int c = 0; c < 3; ++c) x[c] = off[c] + comp[c];
     const int32_t si_mod4 = static_cast<int32_t>(cad.si0 & 3);
-    reload_init(fv, ph, lds);
+    reload_init<STORE>(fv, ph, lds);
     int32_t phase = cad.phase0;
     int64_t slot = cad.slot0;
     // pushes happen while si < total_steps: at most n_push of the K steps
@@ -686,7 +745,7 @@ __device__ __forceinline__ void run_lane(int32_t cell[3], float off[3], float co
         const bool do_save = cad.want_save && phase == 0;
         phase = phase + 1 == cad.stride ? 0 : phase + 1;
         const bool push = live != 0 && k < n_push;
-        coop_reload<STORE>(reload, cell, fv, g, lds, a);  // every lane of the wave, converged here
+        coop_reload<STORE>(reload, cell, fv, g, ph, lds, a);  // every lane of the wave, converged here
         if (!do_save && !push) continue;
         // dead lanes evaluate stale coefficients only to keep the loop
         // uniform; save() writes NaN for them

@@ -248,6 +248,13 @@ int64_t launch_push_fast(pt_ens *e, const pt_field *f, const PushArgs &a) {
         auto *dl = reinterpret_cast<unsigned long long *>(e->deaths);
         const int64_t si0 = a.first_step + done;
         const Cadence cad = make_cadence(si0, k, a.total_steps, a.stride, a.out_kind != PT_OUT_NONE, a.save_base);
+        if (f->cells) {
+            launch_fast<kStoreCells>(g, blocks, e->ctx->stream, split_state(e), e->alive, fv, g, ph, e->n, cad,
+                                     a.out_kind, a.out_f64, a.out_a, a.out_b, e->periodic, a.death_base, dl, e->perm);
+            PT_HIP_CHECK(hipGetLastError());
+            ++launches;
+            continue;
+        }
         switch (f->storage) {
 #define PT_LAUNCH_FAST(ST)                                                                                  \
     case ST:                                                                                                \

@@ -719,6 +719,7 @@ int pt_field_destroy(pt_field *field) {
         (void)hipStreamSynchronize(field->ctx->stream);
         dev_free(field->ctx, field->data, field->bytes);
         if (field->coarse) dev_free(field->ctx, field->coarse, field->coarse_bytes);
+        if (field->cells) dev_free(field->ctx, field->cells, field->cells_bytes);
         delete field;
     });
 }

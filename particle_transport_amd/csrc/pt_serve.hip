@@ -535,11 +535,13 @@ void enqueue_tick(pt_serve *srv) {
     }
     int64_t blocks = 0;
     bool cubic = true;
+    int with_cells = 0;
     t.work.clear();
     for (int i = 0; i < nreq; ++i) {
         SReq &r = *work[static_cast<size_t>(i)];
         const float *k = r.dev.g.k;
         cubic = cubic && k[0] == k[1] && k[0] == k[2];
+        with_cells += r.dev.fv.cells ? 1 : 0;
         const int64_t K = std::min<int64_t>(srv->steps_per_tick, r.d.total_steps - r.si_planned + 1);
         r.dev.cad = make_cadence(r.si_planned, static_cast<int>(K), r.d.total_steps, r.d.save_stride,
                                  r.d.out_kind != PT_OUT_NONE, 0);
@@ -553,6 +555,17 @@ void enqueue_tick(pt_serve *srv) {
     hipStream_t st = srv->ctx->stream;
     PT_HIP_CHECK(hipEventRecord(t.ev0, st));
     PT_HIP_CHECK(hipMemcpyAsync(t.d_table, t.h_table, sizeof(DevReq) * nreq, hipMemcpyHostToDevice, st));
+    // One kernel serves every request of the tick: all of them read cell
+    // tables or none does (a table built or freed while its field serves
+    // requests is refused here).
+    PT_REQUIRE(with_cells == 0 || with_cells == nreq,
+               "pt_serve_step: the running requests' fields differ in having a cell table");
+    if (with_cells) {
+        launch_tick<kStoreCells>(srv, t.d_table, nreq, blocks, cubic);
+        PT_HIP_CHECK(hipEventRecord(t.ev1, st));
+        srv->inflight.push_back(slot);
+        return;
+    }
     switch (srv->storage) {
         case PT_STORE_F32_BRICK: launch_tick<PT_STORE_F32_BRICK>(srv, t.d_table, nreq, blocks, cubic); break;
         case PT_STORE_BF16_BRICK: launch_tick<PT_STORE_BF16_BRICK>(srv, t.d_table, nreq, blocks, cubic); break;
@@ -638,6 +651,8 @@ int pt_serve_add_field(pt_serve *srv, const pt_field *field, int32_t *field_id) 
                    "pt_serve_add_field: serving needs a BRICK field");
         PT_REQUIRE(srv->storage < 0 || srv->storage == field->storage,
                    "pt_serve_add_field: all fields of a server share one storage kind");
+        PT_REQUIRE(srv->fields.empty() || (srv->fields.front()->cells != nullptr) == (field->cells != nullptr),
+                   "pt_serve_add_field: all fields of a server have a cell table or none does");
         PT_REQUIRE(field->ctx == srv->ctx, "pt_serve_add_field: field belongs to another context");
         srv->storage = field->storage;
         srv->fields.push_back(field);

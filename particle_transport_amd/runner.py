@@ -102,9 +102,15 @@ class Context:
 
 
 class Field:
-    """The six field components on the device in one storage layout."""
+    """The six field components on the device in one storage layout.
 
-    def __init__(self, ctx: Context, comps, box_length_m: float, storage: str = "f64_planar"):
+    cell_table=True also builds the cell table (pt_field_cell_table: 192
+    bytes per cell of precomputed FP32 polynomial coefficients) that the
+    production pushers then read on every cell change; results are
+    bit-identical with and without it."""
+
+    def __init__(self, ctx: Context, comps, box_length_m: float, storage: str = "f64_planar", *,
+                 cell_table: bool = False):
         keep = []
         ptrs, dts, devs, shapes = zip(*[_julia_layout_pointer(c, keep) for c in comps])
         if len(set(dts)) != 1 or len(set(devs)) != 1 or len(set(shapes)) != 1:
@@ -119,6 +125,14 @@ class Field:
         self.shape = (nx, ny, nz)
         self.storage = storage
         self.box_length_m = float(box_length_m)
+        self.cell_table = False
+        if cell_table:
+            self.set_cell_table(True)
+
+    def set_cell_table(self, enable: bool):
+        """Build (True) or free (False) the field's cell table."""
+        check(_lib.load().pt_field_cell_table(self.h, 1 if enable else 0))
+        self.cell_table = bool(enable)
 
     @property
     def B0(self) -> float:
