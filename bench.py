@@ -126,10 +126,11 @@ def traffic_from_profiles(fmt):
     return None
 
 
-def pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total, formats=FORMATS, sort_every=0):
+def pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total, formats=FORMATS, sort_every=0, cell_tables=False):
     """The production kernel at the bench shape, three launches per field
-    format, then three 1 GiB device-to-device copies whose FETCH_SIZE and
-    WRITE_SIZE calibrate the counters (run under rocprofv3 --pmc)."""
+    format (on cell tables with cell_tables), then three 1 GiB
+    device-to-device copies whose FETCH_SIZE and WRITE_SIZE calibrate the
+    counters (run under rocprofv3 --pmc)."""
     import ctypes
 
     import numpy as np
@@ -140,7 +141,7 @@ def pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total, formats=FORMATS, sort_
     from particle_transport_amd.runner import Ensemble, Field
     n = x.shape[1]
     for fmt in formats:
-        fld = Field(ctx, F32, L, fmt)
+        fld = Field(ctx, F32, L, fmt, cell_table=cell_tables)
         dt = (2 * math.pi / S) / (Q_E * fld.B0 / (gamma * M_P))
         ens = Ensemble(ctx, fld, x, p, pusher="fast")
         mu = DeviceBuffer(ctx, (S // stride, n), np.float32)
@@ -422,9 +423,11 @@ def dmumu_from_phase_split(ctx, F32, L, pos, mom, lags, log_fn):
     """D_mumu from a device-resident FP32 phase-space cache (positions and
     momenta, (nsave, 3, n)), as run_combined_full does per batch: mu of every
     sample reconstructed into a device buffer (reconstruct_mu_chunk, B
-    trilinear from the FP32 planar field), then the pair walk over it.  Times
-    both steps and sets the buffer's HBM round trip (one write, one read)
-    beside them: the most that computing mu inside the pair walk could save."""
+    trilinear from the FP32 planar field) in the sample-major layout, then the
+    pair walk over it; and the same two steps through the reference's mu_chunk
+    layout (each particle's series contiguous).  Times each step and sets the
+    buffer's HBM round trip (one write, one read) beside them: the most that
+    computing mu inside the pair walk could save."""
     import numpy as np
 
     from particle_transport_amd import dmumu
@@ -432,26 +435,30 @@ def dmumu_from_phase_split(ctx, F32, L, pos, mom, lags, log_fn):
     from particle_transport_amd.runner import Field
     nsave, _, n = pos.shape
     fld = Field(ctx, F32, L, "f32_planar")
-    mu = DeviceBuffer(ctx, (n, nsave), np.float32)
     edges = np.linspace(0.0, 1.0, 41)
-    for rep in range(2):  # the first pass also builds device tables
-        t0 = time.perf_counter()
-        dmumu.reconstruct_mu_chunk(ctx, fld, pos, mom, out=mu)
-        rec_s = time.perf_counter() - t0
-        acc = dmumu.new_accumulators(len(lags), 40)
-        t0 = time.perf_counter()
-        dmumu.process_mu_chunk_dmumu(ctx, mu, lags, edges, acc, "sliding", True, layout="time_fastest")
-        walk_s = time.perf_counter() - t0
+    res = {"shape": f"{n} particles x {nsave} samples FP32 phase space, {len(lags)} lags, 40 bins"}
+    for name, sample_major in (("sample_major", True), ("reference_layout", False)):
+        mu = DeviceBuffer(ctx, (nsave, n) if sample_major else (n, nsave), np.float32)
+        for rep in range(2):  # the first pass also builds device tables
+            t0 = time.perf_counter()
+            dmumu.reconstruct_mu_chunk(ctx, fld, pos, mom, out=mu, sample_major=sample_major)
+            rec_s = time.perf_counter() - t0
+            acc = dmumu.new_accumulators(len(lags), 40)
+            t0 = time.perf_counter()
+            dmumu.process_mu_chunk_dmumu(ctx, mu, lags, edges, acc, "sliding", True,
+                                         layout="particle_fastest" if sample_major else "time_fastest")
+            walk_s = time.perf_counter() - t0
+        mu.free()
+        res[name] = {"reconstruct_s": rec_s, "walk_s": walk_s}
+        log_fn(f"[bench] D_mumu from phase space ({name}): reconstruct {rec_s * 1e3:.2f} ms, walk {walk_s * 1e3:.2f} ms")
     fld.close()
-    mu.free()
     buf = n * nsave * 4
     rt_s = 2 * buf / (HBM_PEAK_GBS * 1e9)
-    res = {"shape": f"{n} particles x {nsave} samples FP32 phase space, {len(lags)} lags, 40 bins",
-           "reconstruct_s": rec_s, "walk_s": walk_s, "mu_buffer_bytes": buf,
-           "round_trip_s_at_hbm_peak": rt_s, "round_trip_fraction": rt_s / (rec_s + walk_s),
-           "phase_space_bytes_read": 2 * 3 * n * nsave * 4}
-    log_fn(f"[bench] D_mumu from phase space: reconstruct {rec_s * 1e3:.2f} ms, walk {walk_s * 1e3:.2f} ms, "
-           f"mu round trip {rt_s * 1e3:.3f} ms at the HBM peak")
+    sm = res["sample_major"]
+    res.update({"mu_buffer_bytes": buf, "round_trip_s_at_hbm_peak": rt_s,
+                "round_trip_fraction": rt_s / (sm["reconstruct_s"] + sm["walk_s"]),
+                "phase_space_bytes_read": 2 * 3 * n * nsave * 4})
+    log_fn(f"[bench] mu buffer round trip {rt_s * 1e3:.3f} ms at the HBM peak")
     return res
 
 
@@ -749,7 +756,7 @@ def main():
     total = max(500, args.warmup + args.steps) * S + 1  # the campaign: 500 T_g0 (more if asked for)
     nsave = S // stride
     if args.pmc_probe:
-        pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total, formats, args.sort_every)
+        pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total, formats, args.sort_every, args.cell_tables)
         ctx.close()
         return
     if args.stats_probe:

@@ -242,6 +242,16 @@ int pt_ens_last_timing(const pt_ens *ens, double *kernel_ms, int64_t *launches);
 int pt_mu_from_phase(pt_ctx *ctx, const pt_field *field, int64_t chunk, int64_t nsave,
                      int src_dtype, int src_on_device, const void *positions,
                      const void *momenta, int out_dtype, int out_on_device, void *mu_out);
+/* Same reconstruction with a choice of output layout: sample_major = 0 is
+ * pt_mu_from_phase's (the reference's mu_chunk, each particle's series
+ * contiguous); sample_major = 1 writes C [nsave][chunk], the layout in which
+ * the pair walk (pt_dmumu_accumulate with stride_s = chunk, stride_p = 1) reads the series
+ * coalesced.  No reference counterpart: run_combined_full's device path uses
+ * it between the two steps. */
+int pt_mu_from_phase_layout(pt_ctx *ctx, const pt_field *field, int64_t chunk, int64_t nsave,
+                            int src_dtype, int src_on_device, const void *positions,
+                            const void *momenta, int out_dtype, int out_on_device, void *mu_out,
+                            int sample_major);
 
 /* ---- lag statistics ----------------------------------------------------- */
 

@@ -27,7 +27,7 @@ EXPORTS = (
     "pt_field_create", "pt_field_destroy", "pt_field_cell_table", "pt_field_mean_abs_b", "pt_mean_abs_b",
     "pt_ens_create", "pt_ens_create_injected", "pt_inject", "pt_ens_destroy", "pt_ens_advance",
     "pt_ens_get_state", "pt_ens_last_timing", "pt_ens_sort",
-    "pt_mu_from_phase", "pt_dmumu_accumulate", "pt_dmumu_accumulate_noalloc", "pt_dpp_accumulate",
+    "pt_mu_from_phase", "pt_mu_from_phase_layout", "pt_dmumu_accumulate", "pt_dmumu_accumulate_noalloc", "pt_dpp_accumulate",
     "pt_dpp_accumulate_pmag",
     "pt_dpp_accumulate_dprel",
     "pt_energy_snapshots", "pt_buffer_alloc", "pt_buffer_free", "pt_memcpy", "pt_host_alloc", "pt_host_free",
@@ -100,6 +100,7 @@ def load():
         "pt_ens_last_timing": (_int, [_vp, _dp, _i64p]),
         "pt_ens_sort": (_int, [_vp]),
         "pt_mu_from_phase": (_int, [_vp, _vp, _i64, _i64, _int, _int, _vp, _vp, _int, _int, _vp]),
+        "pt_mu_from_phase_layout": (_int, [_vp, _vp, _i64, _i64, _int, _int, _vp, _vp, _int, _int, _vp, _int]),
         "pt_dmumu_accumulate": (_int, [_vp, _fp, _int, _i64, _i64, _i64, _i64, _i64p, _i64p, _i64, _dp, _i64,
                                        _int, _int, _i64p, _dp, _dp, _dp, _i64p, _i64p, _dp, _dp]),
         "pt_dmumu_accumulate_noalloc": (_int, [_vp, _fp, _i64, _i64, _i64, _i64, _i64p, _i64p, _i64, _dp, _i64,

@@ -417,16 +417,19 @@ __device__ __forceinline__ void add_coarse(const FieldView &fv, int32_t gi, int3
 }
 
 // Cell polynomial of one component from its 8 corner values in storage
-// units: scale already carries the physical factor.  16-bit
-// bricks then add their coarse part, loaded only after the residual
-// coefficients are formed (the barrier keeps the scheduler from holding both
-// sets of loads in flight, which costs registers in the reload path).
-template <int STORE>
+// units times `scale`.  16-bit bricks then add their coarse part.  kSerial:
+// its loads are issued only after the residual coefficients are formed (the
+// barrier keeps the scheduler from holding both sets of loads in flight);
+// the per-lane reload of all six components needs that to stay within the
+// register budget, while a helper lane of the cooperative pass (one
+// component) keeps both in flight, one memory round trip instead of two.
+// Scheduling only: the arithmetic is the same either way.
+template <int STORE, bool kSerial = true>
 __device__ __forceinline__ void cell_coeffs(const FieldView &fv, int32_t gi, int32_t gj, int32_t gk, int comp,
                                             const float s[8], float scale, float a[8]) {
     coeffs(s, scale, a);
     if constexpr (kHierarchical<STORE>) {
-        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (kSerial) __builtin_amdgcn_sched_barrier(0);
         add_coarse(fv, gi, gj, gk, comp, scale, a);
     }
 }
@@ -632,7 +635,7 @@ __device__ __forceinline__ void coop_reload(int &need, const int32_t cell[3], co
             const int4 t = lds->slot[entry];
             float sv[8], c8[8];
             load_component_corners<STORE>(fv, t.x, t.y, t.z, comp, sv);
-            cell_coeffs<STORE>(fv, t.x, t.y, t.z, comp, sv, lds->scale[comp], c8);
+            cell_coeffs<STORE, false>(fv, t.x, t.y, t.z, comp, sv, lds->scale[comp], c8);
             if constexpr (kHierarchical<STORE>) apply_factor(c8, lds->fac[comp]);
             if (t.w) shift_coeffs(c8, (t.w & 1) ? 1.0f : 0.0f, (t.w & 2) ? 1.0f : 0.0f, (t.w & 4) ? 1.0f : 0.0f);
             float4 *dst = reinterpret_cast<float4 *>(lds->coef[entry][comp]);

@@ -751,6 +751,10 @@ __global__ void k_energy(const float *__restrict__ mom, int64_t total, int64_t n
 
 // ----------------------------------------------------- mu from phase space -
 
+// trilinear_sample (phase_space_gpu_runner.jl:247-276) in compute precision
+// T, split in two: the 1-based lower corner (i0, j0, k0), clamped as the
+// reference clamps it, with the fractions (tx, ty, tz); and the
+// interpolation of the cell's 8 corner values (corner ci = dx + 2 dy + 4 dz).
 template <class T>
 struct PlanarGeom {
     T lo[3], hi[3], d[3];
@@ -758,49 +762,82 @@ struct PlanarGeom {
 };
 
 template <class T>
-__device__ T trilinear_t(const T *__restrict__ arr, T x, T y, T z, const PlanarGeom<T> &g) {
-    const T fx = (x - g.lo[0]) / g.d[0] + T(1);
-    const T fy = (y - g.lo[1]) / g.d[1] + T(1);
-    const T fz = (z - g.lo[2]) / g.d[2] + T(1);
-    const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    auto cf = [](T f, int64_t hi) -> int64_t {
-        T fl = floor(f);
-        if (fl < T(1)) return 1;
-        if (fl > static_cast<T>(hi)) return hi;
-        return static_cast<int64_t>(fl);
-    };
-    const int64_t i0 = cf(fx, nx - 1), j0 = cf(fy, ny - 1), k0 = cf(fz, nz - 1);
-    const T tx = fx - static_cast<T>(i0), ty = fy - static_cast<T>(j0), tz = fz - static_cast<T>(k0);
-    const int64_t base = (i0 - 1) + nx * ((j0 - 1) + ny * (k0 - 1));
-    const int64_t sy = nx, sz = nx * ny;
-    const T c00 = arr[base] * (1 - tx) + arr[base + 1] * tx;
-    const T c10 = arr[base + sy] * (1 - tx) + arr[base + 1 + sy] * tx;
-    const T c01 = arr[base + sz] * (1 - tx) + arr[base + 1 + sz] * tx;
-    const T c11 = arr[base + sy + sz] * (1 - tx) + arr[base + 1 + sy + sz] * tx;
+__device__ __forceinline__ void trilinear_cell(T x, T y, T z, const PlanarGeom<T> &g, int64_t ijk[3], T t[3]) {
+    const T f[3] = {(x - g.lo[0]) / g.d[0] + T(1), (y - g.lo[1]) / g.d[1] + T(1), (z - g.lo[2]) / g.d[2] + T(1)};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const T fl = floor(f[c]);
+        const int64_t hi = g.n[c] - 1;
+        ijk[c] = fl < T(1) ? 1 : (fl > static_cast<T>(hi) ? hi : static_cast<int64_t>(fl));
+        t[c] = f[c] - static_cast<T>(ijk[c]);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T trilinear_corners(const T v[8], const T t[3]) {
+    const T tx = t[0], ty = t[1], tz = t[2];
+    const T c00 = v[0] * (1 - tx) + v[1] * tx;
+    const T c10 = v[2] * (1 - tx) + v[3] * tx;
+    const T c01 = v[4] * (1 - tx) + v[5] * tx;
+    const T c11 = v[6] * (1 - tx) + v[7] * tx;
     const T c0 = c00 * (1 - ty) + c10 * ty;
     const T c1 = c01 * (1 - ty) + c11 * ty;
     return c0 * (1 - tz) + c1 * tz;
 }
 
+// mu for the samples [s0, s0 + seg) of one particle (reconstruct_mu_kernel!,
+// compute_delta_mu2_curve.jl:151-184).  A thread walks one particle's
+// samples, so consecutive samples mostly fall in the same cell: the 24 corner
+// values of Bx, By, Bz stay in registers and are reloaded only when the
+// cell changes.  Lanes of a wave hold consecutive particles, so the position
+// and momentum reads ([s][3][chunk]) are coalesced, as are the mu writes in
+// the sample-major layout ([s][chunk]; layout 1); layout 0 is the
+// reference's mu_chunk, Julia (nsave, chunk), each particle's series
+// contiguous.
 template <class T, class S, class O>
-__global__ void k_mu_from_phase(const S *__restrict__ pos, const S *__restrict__ mom, int64_t chunk,
-                                int64_t nsave, const T *__restrict__ F, PlanarGeom<T> g, O *__restrict__ mu) {
+__global__ void __launch_bounds__(256) k_mu_from_phase(const S *__restrict__ pos, const S *__restrict__ mom,
+                                                       int64_t chunk, int64_t nsave, int32_t seg,
+                                                       const T *__restrict__ F, PlanarGeom<T> g,
+                                                       O *__restrict__ mu, int sample_major) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= chunk * nsave) return;
-    const int64_t s = t / chunk, p = t % chunk;
+    const int64_t nseg = (nsave + seg - 1) / seg;
+    if (t >= chunk * nseg) return;
+    const int64_t p = t % chunk;
+    const int64_t s0 = (t / chunk) * seg;
+    const int64_t s1 = s0 + seg < nsave ? s0 + seg : nsave;
     const int64_t nn = g.n[0] * g.n[1] * g.n[2];
-    const T x = static_cast<T>(pos[(s * 3 + 0) * chunk + p]);
-    const T y = static_cast<T>(pos[(s * 3 + 1) * chunk + p]);
-    const T z = static_cast<T>(pos[(s * 3 + 2) * chunk + p]);
-    const T px = static_cast<T>(mom[(s * 3 + 0) * chunk + p]);
-    const T py = static_cast<T>(mom[(s * 3 + 1) * chunk + p]);
-    const T pz = static_cast<T>(mom[(s * 3 + 2) * chunk + p]);
-    T value = T(NAN);
-    if (!(isnan(x) || isnan(y) || isnan(z) || isnan(px) || isnan(py) || isnan(pz))) {
-        if (g.lo[0] <= x && x <= g.hi[0] && g.lo[1] <= y && y <= g.hi[1] && g.lo[2] <= z && z <= g.hi[2]) {
-            const T bx = trilinear_t(F, x, y, z, g);
-            const T by = trilinear_t(F + nn, x, y, z, g);
-            const T bz = trilinear_t(F + 2 * nn, x, y, z, g);
+    const int64_t sy = g.n[0], sz = g.n[0] * g.n[1];
+    int64_t cur[3] = {-1, -1, -1};
+    T v[3][8];
+    for (int64_t s = s0; s < s1; ++s) {
+        const T x = static_cast<T>(pos[(s * 3 + 0) * chunk + p]);
+        const T y = static_cast<T>(pos[(s * 3 + 1) * chunk + p]);
+        const T z = static_cast<T>(pos[(s * 3 + 2) * chunk + p]);
+        const T px = static_cast<T>(mom[(s * 3 + 0) * chunk + p]);
+        const T py = static_cast<T>(mom[(s * 3 + 1) * chunk + p]);
+        const T pz = static_cast<T>(mom[(s * 3 + 2) * chunk + p]);
+        T value = T(NAN);
+        if (!(isnan(x) || isnan(y) || isnan(z) || isnan(px) || isnan(py) || isnan(pz)) && g.lo[0] <= x &&
+            x <= g.hi[0] && g.lo[1] <= y && y <= g.hi[1] && g.lo[2] <= z && z <= g.hi[2]) {
+            int64_t ijk[3];
+            T tt[3];
+            trilinear_cell(x, y, z, g, ijk, tt);
+            if (ijk[0] != cur[0] || ijk[1] != cur[1] || ijk[2] != cur[2]) {
+                const int64_t base = (ijk[0] - 1) + g.n[0] * ((ijk[1] - 1) + g.n[1] * (ijk[2] - 1));
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T *a = F + c * nn + base;
+#pragma unroll
+                    for (int ci = 0; ci < 8; ++ci)
+                        v[c][ci] = a[((ci & 1) ? 1 : 0) + ((ci & 2) ? sy : 0) + ((ci & 4) ? sz : 0)];
+                }
+                cur[0] = ijk[0];
+                cur[1] = ijk[1];
+                cur[2] = ijk[2];
+            }
+            const T bx = trilinear_corners(v[0], tt);
+            const T by = trilinear_corners(v[1], tt);
+            const T bz = trilinear_corners(v[2], tt);
             // pitch_mu_from_pB, compute_delta_mu2_curve.jl:101-110
             const T pn = sqrt(px * px + py * py + pz * pz);
             const T bn = sqrt(bx * bx + by * by + bz * bz);
@@ -809,8 +846,8 @@ __global__ void k_mu_from_phase(const S *__restrict__ pos, const S *__restrict__
                 value = m < T(-1) ? T(-1) : (m > T(1) ? T(1) : m);
             }
         }
+        mu[sample_major ? s * chunk + p : s + nsave * p] = static_cast<O>(value);
     }
-    mu[s + nsave * p] = static_cast<O>(value);
 }
 
 template <class F>
@@ -917,6 +954,38 @@ struct DmumuDev {
 };
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t{255}; }
+
+// Selected series gathered sample-major: dst[s * n_sel + j] = the sample s of
+// selected particle j (src column idx[j], or j), from a source whose series
+// are contiguous (stride_s = 1, stride_p between particles).  A block moves
+// a tile of 32 particles x 64 samples through LDS, so the reads (64
+// consecutive samples of one particle per wave) and the writes (32
+// consecutive particles per half-wave) are both coalesced.
+__global__ void __launch_bounds__(256) k_series_sample_major(const float *__restrict__ src, int64_t nsave,
+                                                             int64_t n_sel, int64_t sp,
+                                                             const int64_t *__restrict__ idx,
+                                                             float *__restrict__ dst) {
+    __shared__ float tile[32][65];
+    const int64_t j0 = static_cast<int64_t>(blockIdx.x) * 32, s0 = static_cast<int64_t>(blockIdx.y) * 64;
+    const int t = static_cast<int>(threadIdx.x);
+    const int ls = t & 63, lp = t >> 6;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int pj = lp + 4 * k;
+        const int64_t j = j0 + pj, s = s0 + ls;
+        float v = 0.0f;
+        if (j < n_sel && s < nsave) v = src[(idx ? idx[j] : j) * sp + s];
+        tile[pj][ls] = v;
+    }
+    __syncthreads();
+    const int wp = t & 31, ws = t >> 5;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int si = ws + 8 * k;
+        const int64_t j = j0 + wp, s = s0 + si;
+        if (j < n_sel && s < nsave) dst[s * n_sel + j] = tile[wp][si];
+    }
+}
 
 DmumuPlan dmumu_plan(pt_ctx *ctx, const float *mu, int64_t nsave, int64_t n_sel, int64_t stride_s,
                      int64_t stride_p, const int64_t *particle_index, const int64_t *lag_steps, int64_t nlags,
@@ -1081,6 +1150,14 @@ extern "C" {
 int pt_mu_from_phase(pt_ctx *ctx, const pt_field *field, int64_t chunk, int64_t nsave,
                      int src_dtype, int src_on_device, const void *positions,
                      const void *momenta, int out_dtype, int out_on_device, void *mu_out) {
+    return pt_mu_from_phase_layout(ctx, field, chunk, nsave, src_dtype, src_on_device, positions, momenta,
+                                   out_dtype, out_on_device, mu_out, 0);
+}
+
+int pt_mu_from_phase_layout(pt_ctx *ctx, const pt_field *field, int64_t chunk, int64_t nsave,
+                            int src_dtype, int src_on_device, const void *positions,
+                            const void *momenta, int out_dtype, int out_on_device, void *mu_out,
+                            int sample_major) {
     return guarded([&] {
         PT_REQUIRE(ctx && field && positions && momenta && mu_out, "pt_mu_from_phase: null argument");
         PT_REQUIRE(chunk >= 1 && nsave >= 1, "pt_mu_from_phase: empty chunk");
@@ -1102,7 +1179,12 @@ int pt_mu_from_phase(pt_ctx *ctx, const pt_field *field, int64_t chunk, int64_t 
         }
         const size_t osz = out_dtype == PT_F64 ? 8 : 4;
         void *dout = out_on_device ? mu_out : ho.alloc(static_cast<size_t>(chunk * nsave) * osz);
-        const unsigned blocks = static_cast<unsigned>((chunk * nsave + 255) / 256);
+        // samples per thread: long enough that a thread reuses its cell's
+        // corners, short enough to give the chip some thousands of waves
+        const int32_t seg = 32;
+        const int64_t threads = chunk * ((nsave + seg - 1) / seg);
+        PT_REQUIRE(threads / 256 < (1LL << 31), "pt_mu_from_phase: chunk too large");
+        const unsigned blocks = static_cast<unsigned>((threads + 255) / 256);
         auto run = [&](auto tag_t, auto tag_s, auto tag_o) {
             using T = decltype(tag_t);
             using S = decltype(tag_s);
@@ -1118,8 +1200,8 @@ int pt_mu_from_phase(pt_ctx *ctx, const pt_field *field, int64_t chunk, int64_t 
                 g.n[c] = field->g.n[c];
             }
             k_mu_from_phase<T, S, O><<<blocks, 256, 0, st>>>(static_cast<const S *>(dp), static_cast<const S *>(dm),
-                                                          chunk, nsave, static_cast<const T *>(field->data), g,
-                                                          static_cast<O *>(dout));
+                                                          chunk, nsave, seg, static_cast<const T *>(field->data), g,
+                                                          static_cast<O *>(dout), sample_major ? 1 : 0);
         };
         const bool f64 = field->storage == PT_STORE_F64_PLANAR;
         if (f64) {
@@ -1158,10 +1240,26 @@ int pt_dmumu_accumulate(pt_ctx *ctx, const float *mu, int mu_on_device, int64_t 
         if (n_sel == 0) return;
         PT_HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t st = ctx->stream;
-        DevBuf hmu(ctx), hidx(ctx), hlag(ctx), hlo(ctx), hout(ctx), hfin(ctx);
+        DevBuf hmu(ctx), hidx(ctx), hlag(ctx), hlo(ctx), hout(ctx), hfin(ctx), htr(ctx);
         DmumuDev D;
         D.mu = to_device(mu, mu_on_device, P.mu_count, hmu, st);
         D.idx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
+        // Series stored contiguously per particle (the reference's mu_chunk):
+        // one lane per particle would then read a different cache line for
+        // every sample, so the selected series are first gathered
+        // sample-major (a 4-byte-per-sample copy, against the ~nlags/4 reads
+        // of each sample the walk makes).  The walk's per-particle order is
+        // unchanged, so the accumulators are the same bit for bit.
+        const size_t tr_bytes = static_cast<size_t>(n_sel) * static_cast<size_t>(nsave) * sizeof(float);
+        if (stride_s == 1 && nsave > 1 && n_sel > 1 && tr_bytes <= (size_t{4} << 30) && (nsave + 63) / 64 <= 65535) {
+            float *tm = static_cast<float *>(htr.alloc(tr_bytes));
+            const dim3 grid(static_cast<unsigned>((n_sel + 31) / 32), static_cast<unsigned>((nsave + 63) / 64));
+            k_series_sample_major<<<grid, 256, 0, st>>>(D.mu, nsave, n_sel, stride_p, D.idx, tm);
+            PT_HIP_CHECK(hipGetLastError());
+            P = dmumu_plan(ctx, tm, nsave, n_sel, n_sel, 1, nullptr, lag_steps, nlags, mu_edges, n_bins, start_mode);
+            D.mu = tm;
+            D.idx = nullptr;
+        }
         D.lags = to_device(P.lags, 0, static_cast<size_t>(nlags), hlag, st);
         D.lo = to_device(P.lo, 0, static_cast<size_t>(n_bins), hlo, st);
         dmumu_carve(P, static_cast<char *>(hout.alloc(P.work_bytes)), D);

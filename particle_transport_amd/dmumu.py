@@ -389,14 +389,16 @@ def run_combined_from_mu_cache(cfg, ctx, mu_cache, t_s, t_norm, t_gyroperiods=No
                             u.centers, u.visits, u.pidx)
 
 
-def reconstruct_mu_chunk(ctx, field, positions, momenta, out=None):
+def reconstruct_mu_chunk(ctx, field, positions, momenta, out=None, sample_major=False):
     """reconstruct_mu_chunk_gpu (compute_delta_mu2_curve.jl:186-225): mu for
     every (sample, particle) of a phase-space chunk, B trilinear from a planar
     field in the field's precision.  positions/momenta: numpy (nsave, 3,
     chunk) (Julia (chunk, 3, nsave)) or DeviceBuffers of that shape.  Writes
     the reference's mu_chunk, Julia (nsave, chunk) = numpy (chunk, nsave)
     float32, into `out` (a DeviceBuffer, kept on the device) or returns it as
-    a host array."""
+    a host array.  sample_major=True writes numpy (nsave, chunk) instead, the
+    layout the pair walk reads coalesced (process_mu_chunk_dmumu with
+    layout="particle_fastest")."""
     keep = []
 
     def src(a):
@@ -415,14 +417,16 @@ def reconstruct_mu_chunk(ctx, field, positions, momenta, out=None):
     if (pdt, pdev) != (mdt, mdev):
         raise PipelineError("positions and momenta must share dtype and device")
     nsave, _, chunk = pshape
+    shape = (nsave, chunk) if sample_major else (chunk, nsave)
     if out is None:
-        host = np.empty((chunk, nsave), dtype=np.float32)
+        host = np.empty(shape, dtype=np.float32)
         dst, on_dev = ptr(host), 0
     else:
-        if out.dtype != np.float32 or out.shape != (chunk, nsave):
-            raise PipelineError("out must be a float32 DeviceBuffer shaped (chunk, nsave)")
+        if out.dtype != np.float32 or out.shape != shape:
+            raise PipelineError(f"out must be a float32 DeviceBuffer shaped {shape}")
         host, dst, on_dev = None, ctypes.c_void_p(out.ptr), 1
-    check(_lib.load().pt_mu_from_phase(ctx.h, field.h, chunk, nsave, pdt, pdev, pp, mp, _lib.PT_F32, on_dev, dst))
+    check(_lib.load().pt_mu_from_phase_layout(ctx.h, field.h, chunk, nsave, pdt, pdev, pp, mp, _lib.PT_F32, on_dev,
+                                              dst, 1 if sample_major else 0))
     return out if host is None else host
 
 
@@ -447,12 +451,13 @@ def run_combined_full(cfg, ctx, field, positions, momenta, t_s, t_norm, t_gyrope
     start_mode, bin_abs = dmumu_start_mode(cfg), mu_bin_abs(cfg)
     T = np.float64 if field.storage == "f64_planar" else np.float32
     batch = gpu_batch_particles(cfg, u.chunk, nsteps * 6 * np.dtype(T).itemsize)
-    buf = _lib.DeviceBuffer(ctx, (min(batch, len(u.pidx)), nsteps), np.float32)
+    buf = _lib.DeviceBuffer(ctx, (nsteps, min(batch, len(u.pidx))), np.float32)
 
     def one_batch(pos, mom):
+        # sample-major on the device: the pair walk then reads it coalesced
         mu = reconstruct_mu_chunk(ctx, field, np.ascontiguousarray(pos, dtype=T), np.ascontiguousarray(mom, dtype=T),
-                                  out=buf.view((pos.shape[-1], nsteps)))
-        process_mu_chunk_dmumu(ctx, mu, u.lag_steps, u.edges, u.acc, start_mode, bin_abs, layout="time_fastest")
+                                  out=buf.view((nsteps, pos.shape[-1])), sample_major=True)
+        process_mu_chunk_dmumu(ctx, mu, u.lag_steps, u.edges, u.acc, start_mode, bin_abs, layout="particle_fastest")
 
     srcs = [a if hasattr(a, "read_box") else h5.ArrayDataset(np.asarray(a)) for a in (positions, momenta)]
     try:

@@ -439,22 +439,34 @@ def test_dpp_from_relative_cache(ctx):
 
 
 @pytest.mark.parametrize("storage", ["f64_planar", "f32_planar"])
-def test_mu_from_phase_matches_oracle(ctx, storage):
-    F, x, p, dt, g = make_case(n=200)
-    o = oracle.run(F, g, x.copy(), p.copy(), dt, 1, 300, 300, 30, True, 1)
+@pytest.mark.parametrize("steps,stride", [(300, 30), (1200, 5)], ids=["10_samples", "240_samples"])
+def test_mu_from_phase_matches_oracle(ctx, storage, steps, stride):
+    """Both output layouts against the oracle's reconstruct_mu_chunk; the
+    long case spans several 32-sample thread segments, and the particles move
+    across cells within a segment (the cached corners are reloaded)."""
+    F, x, p, dt, g = make_case(n=200, energy_gev=1e4)
+    o = oracle.run(F, g, x.copy(), p.copy(), dt, 1, steps, steps, stride, True, 1)
     pos, mom = o["pos"].astype(np.float32), o["mom"].astype(np.float32)
     pos[3, :, 7] = np.nan
+    pos[5, 1, 9] = -1.0  # outside the box: NaN like the reference
     fld = Field(ctx, F, L_BOX, storage)
     from particle_transport_amd import _lib
-    import ctypes
     nsave, _, n = pos.shape
     out = np.empty((n, nsave), dtype=np.float32)
     _lib.check(_lib.load().pt_mu_from_phase(ctx.h, fld.h, n, nsave, _lib.PT_F32, 0, _lib.ptr(pos), _lib.ptr(mom),
                                             _lib.PT_F32, 0, _lib.ptr(out)))
+    sm = np.empty((nsave, n), dtype=np.float32)
+    _lib.check(_lib.load().pt_mu_from_phase_layout(ctx.h, fld.h, n, nsave, _lib.PT_F32, 0, _lib.ptr(pos),
+                                                   _lib.ptr(mom), _lib.PT_F32, 0, _lib.ptr(sm), 1))
+    assert np.array_equal(sm, out.T, equal_nan=True)
     ref = oracle.reconstruct_mu_chunk(pos.astype(np.float64), mom.astype(np.float64), F[:3], g)
     assert np.array_equal(np.isnan(out), np.isnan(ref))
+    assert np.isnan(out[9, 5]) and np.isnan(out[7, 3])
     tol = 1e-12 if storage == "f64_planar" else 2e-5  # FP32: the reference's compute_precision=Float32
     assert np.nanmax(np.abs(out - ref)) <= tol + 1e-7  # + float32 output rounding
+    if nsave > 64:
+        cells = np.floor(o["pos"] / g.dx)
+        assert np.any(np.diff(cells, axis=0) != 0)  # cell changes inside the series
     fld.close()
 
 
