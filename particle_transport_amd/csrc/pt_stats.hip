@@ -1179,9 +1179,11 @@ int pt_mu_from_phase_layout(pt_ctx *ctx, const pt_field *field, int64_t chunk, i
         }
         const size_t osz = out_dtype == PT_F64 ? 8 : 4;
         void *dout = out_on_device ? mu_out : ho.alloc(static_cast<size_t>(chunk * nsave) * osz);
-        // samples per thread: long enough that a thread reuses its cell's
-        // corners, short enough to give the chip some thousands of waves
-        const int32_t seg = 32;
+        // samples per thread: long enough that the 24 corner loads at a
+        // segment's start (a cache line each) cost less than the segment's
+        // own phase-space reads, short enough to leave the chip thousands
+        // of waves
+        const int32_t seg = 128;
         const int64_t threads = chunk * ((nsave + seg - 1) / seg);
         PT_REQUIRE(threads / 256 < (1LL << 31), "pt_mu_from_phase: chunk too large");
         const unsigned blocks = static_cast<unsigned>((threads + 255) / 256);

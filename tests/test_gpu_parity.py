@@ -442,7 +442,7 @@ def test_dpp_from_relative_cache(ctx):
 @pytest.mark.parametrize("steps,stride", [(300, 30), (1200, 5)], ids=["10_samples", "240_samples"])
 def test_mu_from_phase_matches_oracle(ctx, storage, steps, stride):
     """Both output layouts against the oracle's reconstruct_mu_chunk; the
-    long case spans several 32-sample thread segments, and the particles move
+    long case spans two 128-sample thread segments, and the particles move
     across cells within a segment (the cached corners are reloaded)."""
     F, x, p, dt, g = make_case(n=200, energy_gev=1e4)
     o = oracle.run(F, g, x.copy(), p.copy(), dt, 1, steps, steps, stride, True, 1)
