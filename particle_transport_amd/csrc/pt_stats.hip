@@ -5,15 +5,20 @@
 //  compute_dpp_full.jl:247-288.  One lane walks one particle's time series;
 //  a workgroup covers 256 particles x G lags.  The mu-bin of the pair start
 //  changes rarely along a trajectory, so each lane keeps a run accumulator
-//  (count, sum dmu, sum dmu^2) per lag for its current bin in registers and
-//  flushes it into LDS-privatised bins (ds_add) only when the bin changes.
+//  (count, sum dmu, sum dmu^2) per lag for its current bin in registers,
+//  moves it to a one-run stash when the bin changes, and the wave flushes
+//  its lanes' stashes into LDS-privatised bins (ds_add) together, when one
+//  of them is needed again.
 //  Per-particle means are merged across particles with Chan's formula in a
 //  fixed tree (wavefront shuffles, then LDS), per-workgroup partials are
 //  merged in workgroup order, so the Welford statistics are deterministic.
 //  The per-bin sums are accumulated as 128-bit fixed-point integers (2^-80
 //  units), whose addition is exact and associative, so they are the same
-//  whatever order the lanes and workgroups flush in.  Integer counts are
-//  exact; the remaining sums are FP64.
+//  whatever order the lanes and workgroups flush in.  A flush converts a run
+//  sum with integer operations on the double's bits and adds it to the LDS
+//  bins as three carry-free words with non-returning atomics; each workgroup
+//  folds its words into the global 128-bit bins once, at its end.  Integer
+//  counts are exact; the remaining sums are FP64.
 #include <algorithm>
 #include <cmath>
 #include <mutex>
@@ -29,7 +34,7 @@ namespace {
 constexpr int kTPB = 256;
 constexpr int kG = 8;          // lags per workgroup (D_pp)
 constexpr int kGmu = 4;        // lags per workgroup (D_mumu: its run accumulators take more registers)
-constexpr int kMaxBins = 256;  // LDS budget: kGmu * bins * 24 B
+constexpr int kMaxBins = 256;  // LDS budget: kGmu * bins * 56 B (57 KB at 256 bins)
 
 struct Welford {
     double n, mean, m2;
@@ -119,19 +124,28 @@ __host__ inline unsigned tile_group_blocks(int nbx, int ngy) {
 // fraction bits; each converted run sum is truncated by less than 2^-80.
 constexpr double kFixUnit = 0x1p-80;
 
-// v -> {lo, hi}: magnitude truncated toward zero, then negated if v < 0.
+// v -> trunc(v 2^80) toward zero as a 128-bit two's-complement integer
+// {lo, hi}, from the double's bits without branches: |v| 2^80 = m 2^sh with
+// m the 53-bit significand and sh = exponent - 995, so the magnitude is m
+// shifted left by sh or right by -sh (the bits shifted out below 2^0 are the
+// truncation), then the sign is applied as ~x + 1.  Needs |v| < 2^36
+// (sh <= 63); zeros and subnormals have sh = -995 and give 0.
 __device__ __forceinline__ void fix_from_double(double v, unsigned long long &lo, unsigned long long &hi) {
-    const double a = fabs(v) * 0x1p16;             // |v| 2^80 / 2^64
-    const double h = floor(a);
-    const unsigned long long uh = static_cast<unsigned long long>(h);
-    const unsigned long long ul = static_cast<unsigned long long>((a - h) * 0x1p64);  // a - h exact, in [0, 1)
-    if (v < 0.0) {
-        lo = ~ul + 1ull;
-        hi = ~uh + (ul == 0ull ? 1ull : 0ull);
-    } else {
-        lo = ul;
-        hi = uh;
-    }
+    const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(v));
+    const int e = static_cast<int>((bits >> 52) & 0x7ffull);
+    const unsigned long long m = (bits & 0xfffffffffffffull) | (1ull << 52);
+    const int sh = e - 995;
+    const int ls = sh > 0 ? sh : 0;                // left shift, 0..63
+    const int rs = sh < 0 ? (sh > -63 ? -sh : 63) : 0;  // right shift, 0..63 (m < 2^53: 63 clears it)
+    unsigned long long l = (m << ls) >> rs;
+    unsigned long long h = (m >> 1) >> (63 - ls);  // m >> (64 - ls), and 0 for ls = 0
+    const unsigned long long sgn = static_cast<unsigned long long>(static_cast<long long>(bits) >> 63);
+    l ^= sgn;
+    h ^= sgn;
+    const unsigned long long l1 = l + (sgn & 1ull);
+    h += l1 < l ? 1ull : 0ull;
+    lo = l1;
+    hi = h;
 }
 
 // *p += {lo, hi} (p[0] = lo word, p[1] = hi word) with atomics: the carry out
@@ -153,6 +167,31 @@ __host__ inline double fix_to_double(unsigned long long lo, unsigned long long h
 // [count x nbins][s1 x 2 nbins][s2 x 2 nbins].
 constexpr int kBinWords = 5;
 
+// A workgroup's LDS bins hold each fixed-point sum as three words: the sums
+// of the addends' bits 0-31 and bits 32-63 (below 2^32 each, so their sums
+// are exact while a word takes fewer than 2^32 adds: 256 lanes x fewer than
+// 2^24 samples) and of bits 64-127 modulo 2^64.  Adding needs no carry and
+// no returned value; lds_fix_total puts the three words back together
+// modulo 2^128.  A lag's LDS bins: [count x nbins][s1 x 3 nbins][s2 x 3 nbins].
+// A flushed run sum is below 4 x 2^24 = 2^26 in magnitude (a run is shorter
+// than the series), inside fix_from_double's range.
+constexpr int kLdsBinWords = 7;
+constexpr int64_t kMaxDmumuSamples = int64_t{1} << 24;
+
+__device__ __forceinline__ void lds_fix_add(unsigned long long *w, double v) {
+    unsigned long long lo, hi;
+    fix_from_double(v, lo, hi);
+    atomicAdd(&w[0], lo & 0xffffffffull);
+    atomicAdd(&w[1], lo >> 32);
+    atomicAdd(&w[2], hi);
+}
+
+__device__ __forceinline__ void lds_fix_total(const unsigned long long *w, unsigned long long &lo,
+                                              unsigned long long &hi) {
+    lo = w[0] + (w[1] << 32);
+    hi = w[2] + (w[1] >> 32) + (lo < w[0] ? 1ull : 0ull);
+}
+
 __global__ void __launch_bounds__(kTPB)
 k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, int64_t sp,
         const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags,
@@ -164,12 +203,12 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     extern __shared__ unsigned char smem[];
     // LDS bins of the group's kGmu lags, laid out as g_bins per lag
     unsigned long long *b_bins = reinterpret_cast<unsigned long long *>(smem);
-    float *lo = reinterpret_cast<float *>(b_bins + kGmu * kBinWords * nbins);
+    float *lo = reinterpret_cast<float *>(b_bins + kGmu * kLdsBinWords * nbins);
     __shared__ Welford sh_w[kTPB / 64];
     __shared__ double sh_d[kTPB / 64];
     __shared__ long long sh_i[kTPB / 64];
 
-    for (int i = threadIdx.x; i < kGmu * kBinWords * nbins; i += kTPB) b_bins[i] = 0ull;
+    for (int i = threadIdx.x; i < kGmu * kLdsBinWords * nbins; i += kTPB) b_bins[i] = 0ull;
     for (int i = threadIdx.x; i < nbins; i += kTPB) lo[i] = lo_g[i];
     __syncthreads();
 
@@ -186,41 +225,68 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     const bool active = j < n_sel;
 
     // Run accumulators of the current start bin (every pair is added; a run
-    // whose start was outside the binned range is dropped at the flush) and
+    // whose start was outside the binned range is dropped when it closes),
     // the particle's totals over all valid pairs (sum dmu^2, count), folded
-    // in at each flush.
-    unsigned rc[kGmu], pc[kGmu];
-    double r1[kGmu], r2[kGmu], ps[kGmu];
+    // in when a run closes, and a stash holding one closed run in range
+    // until its LDS flush.  A lane's bin changes rarely, so flushing each
+    // run as it closes would run the flush (fixed-point conversions and LDS
+    // atomics) for one or two lanes of the wave at a time; with the stash a
+    // closed run costs a few register moves, and the wave flushes the
+    // stashes of all its lanes together, only when a lane closes a run while
+    // its stash is taken.
+    unsigned rc[kGmu], pc[kGmu], sc[kGmu];
+    double r1[kGmu], r2[kGmu], ps[kGmu], s1[kGmu], s2[kGmu];
 #pragma unroll
     for (int g = 0; g < kGmu; ++g) {
         rc[g] = 0;
         pc[g] = 0;
-        r1[g] = r2[g] = ps[g] = 0.0;
+        sc[g] = 0;
+        r1[g] = r2[g] = ps[g] = s1[g] = s2[g] = 0.0;
     }
-    int cur_bin = -1;
+    int cur_bin = -1, st_bin = -1;
     float cur_lo = INFINITY, cur_hi = -INFINITY;
+    // first guess of a value's bin for evenly spaced edges
+    const float lo0 = lo[0], inv_w = static_cast<float>(nbins) / (hi_last - lo0);
     // starts of the current run visited on the finite fast path, where every
     // lag's pair is valid: their count is added to each rc[g] at the flush
     unsigned run = 0;
 
-    auto flush = [&]() {
+    // LDS adds of one run in bin `bin`: pair counts and both fixed-point sums
+    auto emit = [&](int bin, const unsigned *c, const double *a1, const double *a2) {
+#pragma unroll
+        for (int g = 0; g < kGmu; ++g) {
+            if (c[g]) {
+                unsigned long long *bl = b_bins + g * kLdsBinWords * nbins;
+                atomicAdd(&bl[bin], static_cast<unsigned long long>(c[g]));
+                lds_fix_add(&bl[nbins + 3 * bin], a1[g]);
+                lds_fix_add(&bl[4 * nbins + 3 * bin], a2[g]);
+            }
+        }
+    };
+    auto flush_stash = [&]() {
+        if (st_bin >= 0) {
+            emit(st_bin, sc, s1, s2);
+            st_bin = -1;
+        }
+    };
+    // close the current run into the particle's totals and, in range, into
+    // the stash (which the caller has emptied)
+    auto close_run = [&]() {
+        const bool keep = cur_bin >= 0;
 #pragma unroll
         for (int g = 0; g < kGmu; ++g) {
             rc[g] += run;
-            if (cur_bin >= 0 && rc[g]) {
-                unsigned long long *bl = b_bins + g * kBinWords * nbins;
-                atomicAdd(&bl[cur_bin], static_cast<unsigned long long>(rc[g]));
-                unsigned long long l, h;
-                fix_from_double(r1[g], l, h);
-                fix_atomic_add(&bl[nbins + 2 * cur_bin], l, h);
-                fix_from_double(r2[g], l, h);
-                fix_atomic_add(&bl[3 * nbins + 2 * cur_bin], l, h);
-            }
             pc[g] += rc[g];
             ps[g] += r2[g];
+            if (keep) {
+                sc[g] = rc[g];
+                s1[g] = r1[g];
+                s2[g] = r2[g];
+            }
             rc[g] = 0;
             r1[g] = r2[g] = 0.0;
         }
+        if (keep) st_bin = cur_bin;
         run = 0;
     };
 
@@ -262,8 +328,12 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         // holds mu[s + lag[g]] (any value where `valid[g]` is false).
         auto rebin = [&](float a) {
             const float v = bin_abs ? fabsf(a) : a;
-            if (!(v >= cur_lo && v < cur_hi)) {
-                flush();
+            const bool change = !(v >= cur_lo && v < cur_hi);
+            // wave-uniform: a lane must close an in-range run while its
+            // stash is taken, so every lane with a stash flushes it now
+            if (__builtin_amdgcn_ballot_w64(change && cur_bin >= 0 && st_bin >= 0)) flush_stash();
+            if (change) {
+                close_run();
                 if (v < lo[0]) {
                     cur_bin = -1;
                     cur_lo = -INFINITY;
@@ -273,7 +343,17 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
                     cur_lo = hi_last;
                     cur_hi = INFINITY;
                 } else {
-                    int b0 = 0, b1 = nbins - 1;  // largest b with lo[b] <= v
+                    // largest b with lo[b] <= v: the guess, else a binary
+                    // search on the side of it that holds b
+                    int g0 = static_cast<int>((v - lo0) * inv_w);
+                    g0 = g0 < 0 ? 0 : (g0 > nbins - 1 ? nbins - 1 : g0);
+                    int b0 = 0, b1 = nbins - 1;
+                    if (lo[g0] > v)
+                        b1 = g0 - 1;
+                    else if (g0 + 1 < nbins && lo[g0 + 1] <= v)
+                        b0 = g0 + 1;
+                    else
+                        b0 = b1 = g0;
                     while (b0 < b1) {
                         const int mid = (b0 + b1 + 1) >> 1;
                         if (lo[mid] <= v)
@@ -376,7 +456,9 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
             }
             visit(row[0], b, valid);
         }
-        flush();
+        flush_stash();
+        close_run();
+        flush_stash();
     }
 
 #pragma unroll
@@ -403,12 +485,15 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
     for (int i = threadIdx.x; i < kGmu * nbins; i += kTPB) {
         const int g = i / nbins, b = i % nbins;
         const int li = gy * kGmu + g;
-        const unsigned long long *bl = b_bins + g * kBinWords * nbins;
+        const unsigned long long *bl = b_bins + g * kLdsBinWords * nbins;
         if (li < nlags && bl[b]) {
             unsigned long long *gl = g_bins + static_cast<int64_t>(li) * kBinWords * nbins;
             atomicAdd(&gl[b], bl[b]);
-            fix_atomic_add(&gl[nbins + 2 * b], bl[nbins + 2 * b], bl[nbins + 2 * b + 1]);
-            fix_atomic_add(&gl[3 * nbins + 2 * b], bl[3 * nbins + 2 * b], bl[3 * nbins + 2 * b + 1]);
+            unsigned long long l, h;
+            lds_fix_total(&bl[nbins + 3 * b], l, h);
+            fix_atomic_add(&gl[nbins + 2 * b], l, h);
+            lds_fix_total(&bl[4 * nbins + 3 * b], l, h);
+            fix_atomic_add(&gl[3 * nbins + 2 * b], l, h);
         }
     }
 }
@@ -992,6 +1077,7 @@ DmumuPlan dmumu_plan(pt_ctx *ctx, const float *mu, int64_t nsave, int64_t n_sel,
                      const double *mu_edges, int64_t n_bins, int start_mode) {
     PT_REQUIRE(ctx && mu && lag_steps && mu_edges, "pt_dmumu_accumulate: null argument");
     PT_REQUIRE(nsave >= 1 && n_sel >= 0 && nlags >= 1, "pt_dmumu_accumulate: empty input");
+    PT_REQUIRE(nsave < kMaxDmumuSamples, "pt_dmumu_accumulate: series longer than 2^24 samples");
     PT_REQUIRE(n_bins >= 1 && n_bins <= kMaxBins, "pt_dmumu_accumulate: n_bins must be in [1, 256]");
     PT_REQUIRE(start_mode == 0 || start_mode == 1, "Unknown dmumu_start_mode");
     PT_REQUIRE(nlags <= kMaxLagsPerCall, "pt_dmumu_accumulate: too many lags");
@@ -1051,7 +1137,7 @@ void dmumu_enqueue(const DmumuPlan &P, const DmumuDev &D, int bin_abs, int start
     k_fill_u64<<<static_cast<unsigned>((nw + 255) / 256), 256, 0, st>>>(D.bins, nw);
     PT_HIP_CHECK(hipGetLastError());
     launch_finite_prefix(D.mu, P.nsave, P.n_sel, P.ss, P.sp, D.idx, D.fin, st);
-    const size_t shm = static_cast<size_t>(kGmu) * static_cast<size_t>(P.nbins) * kBinWords * 8 +
+    const size_t shm = static_cast<size_t>(kGmu) * static_cast<size_t>(P.nbins) * kLdsBinWords * 8 +
                        static_cast<size_t>(P.nbins) * 4;
     k_dmumu<<<tile_group_blocks(P.nbx, P.ngy), kTPB, shm, st>>>(
         D.mu, P.nsave, P.n_sel, P.ss, P.sp, D.idx, D.lags, static_cast<int>(P.nlags), D.lo,

@@ -744,6 +744,95 @@ def test_dmumu_bin_sums_deterministic_and_order_independent(ctx):
     _check_dmumu(runs[0], oacc)
 
 
+def _fixed_point_bins(mu, lags, edges, bin_abs):
+    """The bin sums as the kernel defines them, in exact integer arithmetic:
+    each particle's starts are split into runs of one start bin; a run's sums
+    of dmu and dmu^2 over each lag are formed in FP64 in start order (dmu^2
+    added with one rounding, as a fused multiply-add); each run sum is
+    truncated toward zero to a multiple of 2^-80; the bin's total is the exact
+    sum of those integers modulo 2^128, read back as hi 2^-16 + lo 2^-80 in
+    FP64.  Runs whose start lies outside the edges are dropped."""
+    from fractions import Fraction
+    nl, nb = len(lags), len(edges) - 1
+    cnt = np.zeros((nl, nb), np.int64)
+    tot = [[[0, 0] for _ in range(nb)] for _ in range(nl)]
+
+    def key(a):
+        v = abs(float(a)) if bin_abs else float(a)
+        if v < edges[0]:
+            return -1
+        if v > edges[-1]:
+            return -2
+        return min(int(np.searchsorted(edges, v, side="right")) - 1, nb - 1)
+
+    def fix(v):
+        q = Fraction(v) * 2 ** 80
+        return int(q) if q >= 0 else -int(-q)
+
+    nsave = mu.shape[1]
+    for row in mu:
+        for g, lag in enumerate(lags):
+            cur, rc, r1, r2 = None, 0, 0.0, 0.0
+
+            def flush():
+                if cur is not None and cur >= 0 and rc:
+                    cnt[g, cur] += rc
+                    tot[g][cur][0] += fix(r1)
+                    tot[g][cur][1] += fix(r2)
+
+            for s in range(nsave - 1):
+                a = row[s]
+                if not np.isfinite(a):
+                    continue
+                k = key(a)
+                if k != cur:
+                    flush()
+                    cur, rc, r1, r2 = k, 0, 0.0, 0.0
+                if s + lag < nsave and np.isfinite(row[s + lag]):
+                    d = float(row[s + lag]) - float(a)
+                    r1 += d
+                    r2 = float(Fraction(d) * Fraction(d) + Fraction(r2))
+                    rc += 1
+            flush()
+
+    def to_double(t):
+        t %= 1 << 128
+        lo, hi = t & ((1 << 64) - 1), t >> 64
+        if hi >= 1 << 63:
+            hi -= 1 << 64
+        return float(hi) * 2.0 ** -16 + float(lo) * 2.0 ** -80
+
+    s1 = np.array([[to_double(tot[g][b][0]) for b in range(nb)] for g in range(nl)])
+    s2 = np.array([[to_double(tot[g][b][1]) for b in range(nb)] for g in range(nl)])
+    return cnt, s1, s2
+
+
+def test_dmumu_bin_sums_equal_exact_fixed_point_sums(ctx):
+    """Bitwise equality of the bin sums with an exact-integer restatement of
+    their fixed-point definition: pins the conversion of run sums to 2^-80
+    units (truncation of both signs, zeros), the carry-free LDS words and the
+    128-bit carries across workgroups (three tiles of 256 particles).  Signed
+    mu, NaN samples, runs below and above the edges, constant series (long
+    runs) and lags reaching the end of the series."""
+    rng = np.random.default_rng(77)
+    n, nsave = 600, 90
+    mu = np.clip(np.cumsum(rng.normal(0, 0.08, (n, nsave)), axis=1) + rng.uniform(-1, 1, (n, 1)), -1, 1)
+    mu = mu.astype(np.float32)
+    mu[rng.random(mu.shape) < 0.01] = np.nan
+    mu[5, :] = np.float32(0.3)
+    mu[300, :] = -np.float32(0.7)
+    mu[301, ::2] = np.float32(0.25)
+    lags = [1, 2, 5, 17, 89]
+    for bin_abs, edges in ((True, np.linspace(0.1, 0.9, 7)), (False, np.linspace(-0.8, 0.95, 9))):
+        acc = dmumu.new_accumulators(len(lags), len(edges) - 1)
+        dmumu.process_mu_chunk_dmumu(ctx, mu, lags, edges, acc, "sliding", bin_abs)
+        cnt, s1, s2 = _fixed_point_bins(mu, lags, edges, bin_abs)
+        assert np.array_equal(acc.dmumu_counts, cnt)
+        assert np.array_equal(acc.dmumu_sum_delta, s1)
+        assert np.array_equal(acc.dmumu_sum_delta2, s2)
+        assert (s1 < 0).any() and (s1 > 0).any()
+
+
 @pytest.mark.parametrize("pusher,storage", [("fast", "f32_brick"), ("plain_f32", "f32_planar")])
 def test_periodic_particles_on_the_upper_faces_wrap(ctx, pusher, storage):
     """Periodic box, particles injected exactly on x == L (or y, z, or all
