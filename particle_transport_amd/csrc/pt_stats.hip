@@ -192,6 +192,25 @@ __device__ __forceinline__ void lds_fix_total(const unsigned long long *w, unsig
     hi = w[2] + (w[1] >> 32) + (lo < w[0] ? 1ull : 0ull);
 }
 
+// Raw buffer loads over one uniform row: the lane's 32-bit byte offset comes
+// in a VGPR and a uniform 32-bit offset (a lag's row) in an SGPR, so the
+// walk's loads need no per-lane 64-bit address arithmetic.  Descriptor word
+// 3 = 0x00020000 (gfx9-family raw access, 32-bit data format); the record
+// count admits every offset below 2^31, which the host guarantees.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void *row) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(row), 0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, double) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    return __hiloint2double(static_cast<int>(v[1]), static_cast<int>(v[0]));
+}
+
+// kBuf: the finite fast path loads through row_rsrc (every lane and lag
+// offset below 2^31 bytes); otherwise through 64-bit addresses.
+template <bool kBuf>
 __global__ void __launch_bounds__(kTPB)
 k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, int64_t sp,
         const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags,
@@ -329,8 +348,11 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
         auto rebin = [&](float a) {
             const float v = bin_abs ? fabsf(a) : a;
             const bool change = !(v >= cur_lo && v < cur_hi);
-            // wave-uniform: a lane must close an in-range run while its
-            // stash is taken, so every lane with a stash flushes it now
+            // wave-uniform branches: the common sample, where no lane of the
+            // wave changes bin, costs one ballot
+            if (!__builtin_amdgcn_ballot_w64(change)) return;
+            // a lane must close an in-range run while its stash is taken:
+            // every lane with a stash flushes it now
             if (__builtin_amdgcn_ballot_w64(change && cur_bin >= 0 && st_bin >= 0)) flush_stash();
             if (change) {
                 close_run();
@@ -403,11 +425,21 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
             auto at = [&](const float *uniform_row) {
                 return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(uniform_row) + lane_bytes);
             };
+            uint32_t soff[kGmu];
+#pragma unroll
+            for (int g = 0; g < kGmu; ++g) soff[g] = static_cast<uint32_t>(off[g] * 4);
             auto load = [&](int64_t t, float &a, float *b) {
                 const float *r = mu + t * ss;
-                a = at(r);
+                if constexpr (kBuf) {
+                    const __amdgpu_buffer_rsrc_t rs = row_rsrc(r);
+                    a = buf_load(rs, lane_bytes, 0u, 0.0f);
 #pragma unroll
-                for (int g = 0; g < kGmu; ++g) b[g] = at(r + off[g]);
+                    for (int g = 0; g < kGmu; ++g) b[g] = buf_load(rs, lane_bytes, soff[g], 0.0f);
+                } else {
+                    a = at(r);
+#pragma unroll
+                    for (int g = 0; g < kGmu; ++g) b[g] = at(r + off[g]);
+                }
             };
             float a0, a1, a2, b0[kGmu], b1[kGmu], b2[kGmu];
             load(0, a0, b0);
@@ -618,7 +650,7 @@ __global__ void k_pmag(const float *__restrict__ mom, int64_t nsave, int64_t tot
 // T = float, kDelta = true: a relative series delta = |p|/p_ref - 1; the
 // kernel sums d = delta(s+lag) - delta(s) (exact in FP64) and d^2 only, and the
 // host scales them by p_ref and p_ref/p0.
-template <class T, bool kDelta>
+template <class T, bool kDelta, bool kBuf>
 __global__ void __launch_bounds__(kTPB)
 k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
       const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags, double inv_p0,
@@ -693,20 +725,32 @@ k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
         if (s_fin > 0) {
             // every pair valid: the count is the number of starts
             const uint32_t lane_bytes = static_cast<uint32_t>(col * static_cast<int64_t>(sizeof(T)));
+            // samples stay in their stored type until used (a float series
+            // holds half the registers of its FP64 values)
             auto at = [&](const T *uniform_row) {
-                return static_cast<double>(
-                    *reinterpret_cast<const T *>(reinterpret_cast<const char *>(uniform_row) + lane_bytes));
+                return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(uniform_row) + lane_bytes);
             };
-            auto load = [&](int64_t t, double &a, double *b) {
-                const T *r = pm + t * total;
-                a = at(r);
+            uint32_t soff[kG];
 #pragma unroll
-                for (int g = 0; g < kG; ++g) b[g] = at(r + off[g]);
+            for (int g = 0; g < kG; ++g) soff[g] = static_cast<uint32_t>(off[g] * static_cast<int64_t>(sizeof(T)));
+            auto load = [&](int64_t t, T &a, T *b) {
+                const T *r = pm + t * total;
+                if constexpr (kBuf) {
+                    const __amdgpu_buffer_rsrc_t rs = row_rsrc(r);
+                    a = buf_load(rs, lane_bytes, 0u, T{});
+#pragma unroll
+                    for (int g = 0; g < kG; ++g) b[g] = buf_load(rs, lane_bytes, soff[g], T{});
+                } else {
+                    a = at(r);
+#pragma unroll
+                    for (int g = 0; g < kG; ++g) b[g] = at(r + off[g]);
+                }
             };
-            auto visit_finite = [&](double pst, const double *pe) {
+            auto visit_finite = [&](T st, const T *pe) {
+                const double pst = static_cast<double>(st);
 #pragma unroll
                 for (int g = 0; g < kG; ++g) {
-                    const double d = pe[g] - pst;
+                    const double d = static_cast<double>(pe[g]) - pst;
                     s1[g] += d;
                     s2[g] = fma(d, d, s2[g]);
                     if constexpr (!kDelta) {
@@ -716,7 +760,7 @@ k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
                     }
                 }
             };
-            double a0, a1, a2, b0[kG], b1[kG], b2[kG];
+            T a0, a1, a2, b0[kG], b1[kG], b2[kG];
             load(0, a0, b0);
             if (s_fin > 1) load(1, a1, b1);
             for (;;) {  // three samples in flight in rotating registers
@@ -736,7 +780,7 @@ k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
         }
         // every lag in range: sample s+1 is loaded before sample s is used
         if (s < s_main) {
-            double an = row[0], bn[kG];
+            T an = row[0], bn[kG];
 #pragma unroll
             for (int g = 0; g < kG; ++g) bn[g] = row[off[g]];
             for (; s < s_main; ++s) {
@@ -1020,7 +1064,7 @@ struct DmumuPlan {
     int64_t nsave, n_sel, ss, sp, nlags, nbins;
     int64_t max_col;
     size_t mu_count;         // extent of mu the kernels may touch (elements)
-    int nbx, ngy, lane32;
+    int nbx, ngy, lane32, buf;
     const int64_t *lags;     // caller's host lag steps
     float lo[kMaxBins];      // float thresholds equivalent to the FP64 edges
     float hi_last;
@@ -1108,6 +1152,10 @@ DmumuPlan dmumu_plan(pt_ctx *ctx, const float *mu, int64_t nsave, int64_t n_sel,
     P.ngy = static_cast<int>((nlags + kGmu - 1) / kGmu);
     // the fast path's 32-bit lane offsets (bytes) cover every particle
     P.lane32 = P.max_col * stride_p < (int64_t{1} << 30) && nsave < (int64_t{1} << 31) ? 1 : 0;
+    // buffer loads: the lane's and the largest lag's byte offsets below 2^31
+    int64_t lag_top = 0;
+    for (int64_t q = 0; q < nlags; ++q) lag_top = std::max(lag_top, std::min(lag_steps[q], nsave));
+    P.buf = P.lane32 && (P.max_col * stride_p + lag_top * stride_s) < (int64_t{1} << 29) ? 1 : 0;
     const size_t nl = static_cast<size_t>(nlags), nb = static_cast<size_t>(n_bins);
     P.bins_words = nl * nb * kBinWords;
     P.parts = nl * static_cast<size_t>(P.nbx);
@@ -1139,7 +1187,8 @@ void dmumu_enqueue(const DmumuPlan &P, const DmumuDev &D, int bin_abs, int start
     launch_finite_prefix(D.mu, P.nsave, P.n_sel, P.ss, P.sp, D.idx, D.fin, st);
     const size_t shm = static_cast<size_t>(kGmu) * static_cast<size_t>(P.nbins) * kLdsBinWords * 8 +
                        static_cast<size_t>(P.nbins) * 4;
-    k_dmumu<<<tile_group_blocks(P.nbx, P.ngy), kTPB, shm, st>>>(
+    auto *kern = P.buf ? k_dmumu<true> : k_dmumu<false>;
+    kern<<<tile_group_blocks(P.nbx, P.ngy), kTPB, shm, st>>>(
         D.mu, P.nsave, P.n_sel, P.ss, P.sp, D.idx, D.lags, static_cast<int>(P.nlags), D.lo,
         static_cast<int>(P.nbins), P.hi_last, bin_abs ? 1 : 0, start_mode, D.bins, D.part_d, D.part_i, P.nbx,
         P.ngy, D.fin, P.lane32);
@@ -1438,6 +1487,8 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         PT_REQUIRE(particle_index || n_sel <= total, "pt_dpp_accumulate: n_sel exceeds total");
         std::vector<int64_t> lags(lag_steps, lag_steps + nlags);
         for (int64_t l : lags) PT_REQUIRE(l >= 1, "pt_dpp_accumulate: lag steps must be >= 1");
+        int64_t lag_top = 0;
+        for (int64_t l : lags) lag_top = std::max(lag_top, std::min(l, nsave));
         if (particle_index)
             for (int64_t j = 0; j < n_sel; ++j)
                 PT_REQUIRE(particle_index[j] >= 0 && particle_index[j] < total, "pt_dpp_accumulate: particle index out of range");
@@ -1476,9 +1527,14 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
             launch_finite_prefix(pm, nsave, nsel_c, row_len, int64_t{1}, idx, fin, st);
             // the fast path's 32-bit lane offsets (bytes) cover every column
             const int lane32 = row_len < (int64_t{1} << 29) && nsave < (int64_t{1} << 31) ? 1 : 0;
-            k_dpp<T, kDelta><<<tile_group_blocks(nbx, ngy), kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag,
-                                                                static_cast<int>(nlags), 1.0 / p0, part_d, part_i,
-                                                                nbx, ngy, fin, lane32);
+            // buffer loads: the last column's and the largest lag's byte
+            // offsets below 2^31
+            const bool buf = lane32 && (row_len + lag_top * row_len) * static_cast<int64_t>(sizeof(T)) <
+                                           (int64_t{1} << 31);
+            auto *kern = buf ? k_dpp<T, kDelta, true> : k_dpp<T, kDelta, false>;
+            kern<<<tile_group_blocks(nbx, ngy), kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag,
+                                                              static_cast<int>(nlags), 1.0 / p0, part_d, part_i, nbx,
+                                                              ngy, fin, lane32);
             PT_HIP_CHECK(hipGetLastError());
             k_merge_dpp<<<static_cast<unsigned>(nlags), kTPB, 0, st>>>(part_d, part_i, static_cast<int>(nlags), nbx,
                                                                        out_d, out_i);
