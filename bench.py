@@ -26,8 +26,10 @@ Beside the headline number the JSON line carries:
                 (SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_{F32,F64}) times the kernel
                 rate from HIP-event launch durations on the library's stream,
                 over the published 157.3 TFLOP/s; beside it the VALU issue
-                fraction and the HBM convention (algorithmic bytes per
-                particle-step 48*s_f + (56+56)/K + 4/20, K pushes per launch);
+                fraction, the issue rates this box reaches for the push's
+                instruction forms (libpt_calib.so) and the HBM convention
+                (algorithmic bytes per particle-step 48*s_f + (56+56)/K + 4/20,
+                K pushes per launch);
   cpu_baseline  the CPU oracle (C restatement of the reference pusher, FP64)
                 on a bounded sample of the same workload in 16 host threads,
                 with a one-core figure beside it.
@@ -161,6 +163,34 @@ def pmc_probe(ctx, F32, L, x, p, gamma, S, stride, total, formats=FORMATS, sort_
     a.free()
     b.free()
     log("[probe] copies done")
+
+
+CALIB_KINDS = {"fma_vgpr": (0, 2000, 64), "fma_sgpr": (1, 1000, 64), "add_vgpr": (2, 2000, 64),
+               "rsq": (3, 500, 64), "fma_f64": (4, 2000, 32)}
+
+
+def valu_calibration():
+    """Vector-ALU issue rates of this box (libpt_calib.so, csrc/pt_calib.hip):
+    wave64 instructions per ns per CU for each instruction form, eight waves
+    per SIMD (2048 workgroups of 256 lanes on 256 CUs), eight independent
+    chains per lane; and the FP32 FMA rate they give in TFLOP/s."""
+    import ctypes
+    path = os.path.join(ROOT, "particle_transport_amd", "libpt_calib.so")
+    lib = ctypes.CDLL(path)
+    lib.pt_calib_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    lib.pt_calib_valu.restype = ctypes.c_int
+    blocks, cus = 2048, 256
+    out = {}
+    for name, (kind, iters, per_iter) in CALIB_KINDS.items():
+        ms = ctypes.c_float()
+        rc = lib.pt_calib_valu(kind, blocks, iters, ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError(f"pt_calib_valu({name}) failed with hipError {rc}")
+        out[name] = blocks * 4 * iters * per_iter / (ms.value * 1e6) / cus
+    out["fma_tflops"] = out["fma_vgpr"] * cus * 128 / 1e3
+    out["unit"] = "wave64 instructions per ns per CU"
+    out["source"] = "libpt_calib.so (csrc/pt_calib.hip) on this box: 8 waves per SIMD, 8 independent chains per lane"
+    return out
 
 
 def cpu_threads():
@@ -751,6 +781,7 @@ def main():
         assert F32[0].shape == (n_nodes,) * 3, "field file does not match --cells"
     else:
         F32 = make_fields(n_nodes, 1.0, 64.0, seed=2024)[0]
+    calib = None if (args.pmc_probe or args.stats_probe) else valu_calibration()
     ctx = Context(0)
     x, p, gamma = make_particles(n, L, args.energy_gev, seed=99)
     total = max(500, args.warmup + args.steps) * S + 1  # the campaign: 500 T_g0 (more if asked for)
@@ -898,6 +929,8 @@ def main():
                      "valu_instructions_per_particle_step": valu_ps,
                      "valu_issue_frac": valu_ps * kernel_rate / VALU_ISSUE_PEAK if counters else None,
                      "counter_source": os.path.relpath(PROFILE_COUNTERS, ROOT) if counters else None,
+                     "measured_valu_ceiling": calib,
+                     "frac_of_measured_fma_ceiling": flops / calib["fma_tflops"] if counters and calib else None,
                      "kernel_particle_steps_per_s": kernel_rate, "avg_launch_ms": avg_launch_s * 1e3,
                      "hbm_convention": {
                          "bytes_per_particle_step": b_alg, "achieved_gbs": achieved, "peak_gbs": HBM_PEAK_GBS,
@@ -911,7 +944,11 @@ def main():
                              "kernel and workload (every issued wave64 instruction, helpers and idle lanes "
                              "included) read from counter_source; achieved = that FLOP count x the kernel "
                              "rate measured here; valu_issue_frac = VALU instructions issued over the "
-                             "two-per-clock-per-CU rate the published FP32 peak implies"},
+                             "two-per-clock-per-CU rate the published FP32 peak implies; measured_valu_ceiling = "
+                             "what this box issues for the instruction forms the push uses (an FMA whose "
+                             "operands are all VGPRs, one with an SGPR operand, add, rsq, FP64 FMA) with "
+                             "enough independent work per wave, and frac_of_measured_fma_ceiling = achieved "
+                             "over the FMA one in FLOP/s"},
         "formats": {f: {"value": timing[f]["value"], "avg_launch_ms": timing[f]["avg_launch_s"] * 1e3,
                         "dmumu_pass": None if check is None else check[FORMAT_KEY[f]]["pass"]} for f in formats},
         "plain_f32": {"value": plain_rate, "kernel_particle_steps_per_s": plain_kernel,

@@ -10,6 +10,9 @@ Per-file flags:
                                         hipMalloc memory)
 All files: packed FP32 instructions disabled (-target-feature -packed-fp32-ops).
 
+libpt_calib.so (csrc/pt_calib.hip) holds the vector-ALU issue-rate kernels
+bench.py times; it is not part of the boundary.
+
 libpt_h5.so (csrc/pt_h5.cpp, host C++ over the HDF5 C library) is built beside
 it with g++: HDF5 from $PT_HDF5_DIR (default /opt/conda), found at run time
 through an rpath; libstdc++ is linked statically so that loading it never
@@ -82,6 +85,23 @@ def build_h5(verbose: bool = False) -> str:
     return H5_LIB
 
 
+CALIB_LIB = os.path.join(HERE, "libpt_calib.so")
+
+
+def build_calib(verbose: bool = False) -> str:
+    """libpt_calib.so: the vector-ALU issue-rate kernels bench.py times
+    (csrc/pt_calib.hip; not part of the libpt_hip.so boundary)."""
+    tmp = CALIB_LIB + ".tmp"
+    cmd = [hipcc(), *COMMON, "-shared", os.path.join(CSRC, "pt_calib.hip"), "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for pt_calib.hip:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CALIB_LIB)
+    if verbose:
+        print("built", CALIB_LIB)
+    return CALIB_LIB
+
+
 def build_library(verbose: bool = False) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
@@ -100,4 +120,5 @@ def build_library(verbose: bool = False) -> str:
 if __name__ == "__main__":
     build_library(verbose=True)
     build_h5(verbose=True)
+    build_calib(verbose=True)
     sys.exit(0)

@@ -661,6 +661,21 @@ __device__ __forceinline__ void coop_reload(int &need, const int32_t cell[3], co
     }
 }
 
+// Occupancy of the kernels that run the step loop (the single-campaign
+// pusher and the serving tick): four waves per SIMD (at most 128 VGPRs) on
+// stored nodes, the budget at which no storage spills.  With five (at most
+// 102 VGPRs) the compiler kept launch constants of the FP32 storages in
+// scratch (a 64-byte spill per lane written at every launch) and, for 16-bit
+// bricks, values the push itself reads (a 16-byte reload every push);
+// measured per 200-push launch, five waves against four: 4,194,304 protons
+// at 100 GeV, FP32 bricks 2.10 against 2.10 ms, BF16 2.6 against 2.2 ms,
+// FP16 3.6 against 2.2 ms; 1,048,576 at 100 TeV, FP32 bricks 1.74 against
+// 1.59 ms.  A cell table keeps five waves: its per-lane record loads gain
+// more from the fifth wave than its spill of launch constants costs (10 TeV:
+// 0.80 against 0.84 ms).
+template <int STORE>
+constexpr int kPushWaves = STORE == kStoreCells ? 5 : 4;
+
 // Cadence of one launch: steps si0 .. si0+K-1; a save happens where
 // (si-1) % stride == 0.  phase0 = (si0-1) % stride and slot0 = the index of
 // the first save at or after si0 are computed once on the host, so the loop

@@ -109,7 +109,7 @@ k_push_plain(SplitState s, uint8_t *__restrict__ alive_arr, const float *__restr
 
 // ------------------------------------------------------------ fast ---------
 
-// The production kernel's body; the kernels below differ only in occupancy.
+// The production kernel's body.
 template <int STORE, bool CUBIC>
 __device__ __forceinline__ void push_fast(SplitState s, uint8_t *__restrict__ alive_arr, const FieldView &fv,
                                           const SplitGeom &g, const Phys &ph, int64_t n, const Cadence &cad,
@@ -162,21 +162,10 @@ __device__ __forceinline__ void push_fast(SplitState s, uint8_t *__restrict__ al
         unsigned long long *deaths, const int32_t *__restrict__ perm
 #define PT_FAST_ARGS s, alive_arr, fv, g, ph, n, cad, out_kind, out_f64, out_a, out_b, periodic, first_step, deaths, perm
 
-// FP32 fields: five waves per SIMD (at most 102 VGPRs).  The launch
-// constants the compiler then keeps in scratch are read only on the rare
-// paths (coefficient reload, cell change, save), never in the push itself.
+// Waves per SIMD: kPushWaves (pt_push_core.hpp).
 template <int STORE, bool CUBIC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_push_fast(PT_FAST_PARAMS) {
-    push_fast<STORE, CUBIC>(PT_FAST_ARGS);
-}
-
-// 16-bit fields: four waves per SIMD.  Their reload also adds the coarse
-// part, and under the five-wave register budget the compiler spilled values
-// the push itself reads (a 16-byte reload every push; measured 2.6 ms per
-// 200-push launch of 4,194,304 particles for BF16 and 3.6 ms for FP16,
-// against 2.2 ms for both with four waves and no spills).
-template <int STORE, bool CUBIC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_push_fast16(PT_FAST_PARAMS) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kPushWaves<STORE>, 8)))
+k_push_fast(PT_FAST_PARAMS) {
     push_fast<STORE, CUBIC>(PT_FAST_ARGS);
 }
 #undef PT_FAST_PARAMS
@@ -202,18 +191,10 @@ void make_geom_phys(const pt_field *f, const PushArgs &a, SplitGeom &g, Phys &ph
 // take the variant that forms the position step factor once per push.
 template <int STORE, class... Args>
 void launch_fast(const SplitGeom &g, unsigned blocks, hipStream_t stream, Args... args) {
-    const bool cubic = g.k[0] == g.k[1] && g.k[0] == g.k[2];
-    if constexpr (kHierarchical<STORE>) {
-        if (cubic)
-            k_push_fast16<STORE, true><<<blocks, 256, 0, stream>>>(args...);
-        else
-            k_push_fast16<STORE, false><<<blocks, 256, 0, stream>>>(args...);
-    } else {
-        if (cubic)
-            k_push_fast<STORE, true><<<blocks, 256, 0, stream>>>(args...);
-        else
-            k_push_fast<STORE, false><<<blocks, 256, 0, stream>>>(args...);
-    }
+    if (g.k[0] == g.k[1] && g.k[0] == g.k[2])
+        k_push_fast<STORE, true><<<blocks, 256, 0, stream>>>(args...);
+    else
+        k_push_fast<STORE, false><<<blocks, 256, 0, stream>>>(args...);
 }
 
 }  // namespace

@@ -124,17 +124,10 @@ __device__ __forceinline__ void serve_tick(const DevReq *__restrict__ reqs, int 
     R.alive[p] = alive ? 1 : 0;
 }
 
-// One tick over all running requests: five waves per SIMD on FP32 fields,
-// four on 16-bit fields (whose reload adds the coarse part; see the
-// single-campaign kernels in pt_push_fast.hip).
+// One tick over all running requests (waves per SIMD: kPushWaves).
 template <int STORE, bool CUBIC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kPushWaves<STORE>, 8)))
 k_serve(const DevReq *__restrict__ reqs, int nreq) {
-    serve_tick<STORE, CUBIC>(reqs, nreq);
-}
-template <int STORE, bool CUBIC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
-k_serve16(const DevReq *__restrict__ reqs, int nreq) {
     serve_tick<STORE, CUBIC>(reqs, nreq);
 }
 
@@ -497,17 +490,10 @@ void complete_all(pt_serve *srv) {
 template <int STORE>
 void launch_tick(pt_serve *srv, const DevReq *table, int nreq, int64_t blocks, bool cubic) {
     const unsigned grid = static_cast<unsigned>(blocks);
-    if constexpr (kHierarchical<STORE>) {
-        if (cubic)
-            k_serve16<STORE, true><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
-        else
-            k_serve16<STORE, false><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
-    } else {
-        if (cubic)
-            k_serve<STORE, true><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
-        else
-            k_serve<STORE, false><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
-    }
+    if (cubic)
+        k_serve<STORE, true><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
+    else
+        k_serve<STORE, false><<<grid, 256, 0, srv->ctx->stream>>>(table, nreq);
     PT_HIP_CHECK(hipGetLastError());
 }
 

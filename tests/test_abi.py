@@ -1,5 +1,6 @@
 """The C-ABI libraries load and export every symbol include/pt_hip.h and
-include/pt_h5.h declare (no compute calls: this runs without a GPU)."""
+include/pt_h5.h declare, and bench.py's calibration library loads (no compute
+calls: this runs without a GPU)."""
 import ctypes
 import os
 import re
@@ -43,6 +44,17 @@ def test_h5_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(h5.LIB_PATH)
     assert not [s for s in syms if not hasattr(lib, s)]
     assert set(syms) == set(h5.EXPORTS)
+
+
+def test_calibration_library_loads():
+    """libpt_calib.so (bench.py's vector-ALU issue-rate kernels) is built and
+    exports its one entry point; argument errors come back before any launch."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "particle_transport_amd", "libpt_calib.so"))
+    assert hasattr(lib, "pt_calib_valu")
+    lib.pt_calib_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    ms = ctypes.c_float()
+    assert lib.pt_calib_valu(99, 1, 1, ctypes.byref(ms)) != 0
+    assert lib.pt_calib_valu(0, 0, 1, ctypes.byref(ms)) != 0
 
 
 def test_ctypes_structs_match_the_header(tmp_path):
