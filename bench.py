@@ -176,6 +176,9 @@ def valu_calibration():
     chains per lane; and the FP32 FMA rate they give in TFLOP/s."""
     import ctypes
     path = os.path.join(ROOT, "particle_transport_amd", "libpt_calib.so")
+    if not os.path.exists(path):
+        log(f"[bench] {path} not built (python -m particle_transport_amd.build): no measured VALU ceiling")
+        return None
     lib = ctypes.CDLL(path)
     lib.pt_calib_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     lib.pt_calib_valu.restype = ctypes.c_int
