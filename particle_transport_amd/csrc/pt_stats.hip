@@ -32,7 +32,11 @@ namespace pt {
 namespace {
 
 constexpr int kTPB = 256;
-constexpr int kG = 8;          // lags per workgroup (D_pp)
+// Lags per workgroup of the D_pp walk: eight over float32 series, four over
+// FP64 series, whose samples and sums take twice the registers (eight FP64
+// lags held the walk to three waves per SIMD; four give five).
+template <class T>
+constexpr int kDppLags = sizeof(T) == 8 ? 4 : 8;
 constexpr int kGmu = 4;        // lags per workgroup (D_mumu: its run accumulators take more registers)
 constexpr int kMaxBins = 256;  // LDS budget: kGmu * bins * 56 B (57 KB at 256 bins)
 
@@ -656,6 +660,7 @@ k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
       const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags, double inv_p0,
       double *__restrict__ part_d, long long *__restrict__ part_i, int nbx, int ngy,
       const int32_t *__restrict__ fin, int lane32) {
+    constexpr int kG = kDppLags<T>;
     const TileGroup tg = tile_group(ngy);
     if (tg.tile >= nbx) return;
     __shared__ double sh_d[kTPB / 64];
@@ -1509,7 +1514,6 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
             const int64_t cap = std::max<int64_t>(kTPB, ((int64_t{1} << 31) / (8 * nsave)) / kTPB * kTPB);
             chunk = std::min(n_sel, cap);
         }
-        const int ngy = static_cast<int>((nlags + kG - 1) / kG);
         const int nbx_max = static_cast<int>((chunk + kTPB - 1) / kTPB);
         const size_t nl = static_cast<size_t>(nlags), parts = nl * static_cast<size_t>(nbx_max);
         char *base = static_cast<char *>(hout.alloc(parts * (32 + 8) + nl * (32 + 8)));
@@ -1524,6 +1528,7 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         auto run = [&](const auto *pm, int64_t row_len, int64_t nsel_c, const int64_t *idx) {
             using T = std::remove_cv_t<std::remove_pointer_t<decltype(pm)>>;
             const int nbx = static_cast<int>((nsel_c + kTPB - 1) / kTPB);
+            const int ngy = static_cast<int>((nlags + kDppLags<T> - 1) / kDppLags<T>);
             launch_finite_prefix(pm, nsave, nsel_c, row_len, int64_t{1}, idx, fin, st);
             // the fast path's 32-bit lane offsets (bytes) cover every column
             const int lane32 = row_len < (int64_t{1} << 29) && nsave < (int64_t{1} << 31) ? 1 : 0;
