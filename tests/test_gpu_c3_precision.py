@@ -103,7 +103,8 @@ def test_c3_dmumu_within_one_percent_of_fp64(c3_runs, fmt):
     rel_b, z_b, ok_b = compare(curves[fmt], curves["fp64"], "bin")
     rel_l, z_l, ok_l = compare(curves[fmt], curves["fp64"], "lag")
     print(f"\n{fmt}: |mu| bins: median rel err {np.median(rel_b):.2e}, max {rel_b.max():.2e}, "
-          f"max err/sigma {z_b.max():.2f}; per bin: " + " ".join(f"{v:.1e}" for v in rel_b))
+          f"max err/sigma {z_b.max():.2f}; per bin: " + " ".join(f"{v:.1e}" for v in rel_b)
+          + "; per bin err/sigma: " + " ".join(f"{v:.2f}" for v in z_b))
     print(f"{fmt}: lags ({len(lags)}, 1..{lags[-1]} samples): median rel err {np.median(rel_l):.2e}, "
           f"max {rel_l.max():.2e}, max err/sigma {z_l.max():.2f}")
     assert len(rel_b) == 20
