@@ -445,17 +445,23 @@ k_dmumu(const float *__restrict__ mu, int64_t nsave, int64_t n_sel, int64_t ss, 
                     for (int g = 0; g < kGmu; ++g) b[g] = at(r + off[g]);
                 }
             };
+            // Every load is issued unconditionally, past the end clamped to
+            // the last start (a row already loaded): the number of loads in
+            // flight is then the same on every path, and the compiler waits
+            // only for the oldest sample's loads instead of draining them all.
+            const int64_t t_last = s_fin - 1;
+            auto ahead = [&](int64_t t) { return t < t_last ? t : t_last; };
             float a0, a1, a2, b0[kGmu], b1[kGmu], b2[kGmu];
             load(0, a0, b0);
-            if (s_fin > 1) load(1, a1, b1);
+            load(ahead(1), a1, b1);
             for (;;) {
-                if (s + 2 < s_fin) load(s + 2, a2, b2);
+                load(ahead(s + 2), a2, b2);
                 visit_finite(a0, b0);
                 if (++s == s_fin) break;
-                if (s + 2 < s_fin) load(s + 2, a0, b0);
+                load(ahead(s + 2), a0, b0);
                 visit_finite(a1, b1);
                 if (++s == s_fin) break;
-                if (s + 2 < s_fin) load(s + 2, a1, b1);
+                load(ahead(s + 2), a1, b1);
                 visit_finite(a2, b2);
                 if (++s == s_fin) break;
             }
@@ -765,17 +771,21 @@ k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
                     }
                 }
             };
+            // three samples in flight in rotating registers, every load
+            // issued (clamped to the last start) as in k_dmumu
+            const int64_t t_last = s_fin - 1;
+            auto ahead = [&](int64_t t) { return t < t_last ? t : t_last; };
             T a0, a1, a2, b0[kG], b1[kG], b2[kG];
             load(0, a0, b0);
-            if (s_fin > 1) load(1, a1, b1);
-            for (;;) {  // three samples in flight in rotating registers
-                if (s + 2 < s_fin) load(s + 2, a2, b2);
+            load(ahead(1), a1, b1);
+            for (;;) {
+                load(ahead(s + 2), a2, b2);
                 visit_finite(a0, b0);
                 if (++s == s_fin) break;
-                if (s + 2 < s_fin) load(s + 2, a0, b0);
+                load(ahead(s + 2), a0, b0);
                 visit_finite(a1, b1);
                 if (++s == s_fin) break;
-                if (s + 2 < s_fin) load(s + 2, a1, b1);
+                load(ahead(s + 2), a1, b1);
                 visit_finite(a2, b2);
                 if (++s == s_fin) break;
             }
