@@ -344,7 +344,7 @@ def device_particle_groups(ctx, sources, pidx0, chunk, dtypes, budget_bytes=256 
     srcs = [s if hasattr(s, "read_box") else h5.ArrayDataset(np.asarray(s)) for s in sources]
     bufs = [None] * len(srcs)
     try:
-        for arrays, chunks in h5.iter_particle_groups(srcs, pidx0, chunk, budget_bytes):
+        for arrays, chunks in h5.iter_particle_groups(srcs, pidx0, chunk, budget_bytes, reuse=True):
             views = []
             for k, (a, dt) in enumerate(zip(arrays, dtypes)):
                 a = np.ascontiguousarray(a, dtype=dt)
@@ -460,10 +460,21 @@ def run_combined_full(cfg, ctx, field, positions, momenta, t_s, t_norm, t_gyrope
         process_mu_chunk_dmumu(ctx, mu, u.lag_steps, u.edges, u.acc, start_mode, bin_abs, layout="particle_fastest")
 
     srcs = [a if hasattr(a, "read_box") else h5.ArrayDataset(np.asarray(a)) for a in (positions, momenta)]
+    # One batch per group: the group's array (read on the worker thread while
+    # the GPU works) is then the batch itself, contiguous, and goes to the
+    # device without a second gather on the host (a scattered selection, whose
+    # group spans more columns than it selects, is still gathered).
+    per_col = max(int(np.prod(s.shape[:-1])) * np.dtype(s.dtype).itemsize for s in srcs)
     try:
-        for (P, M), batches in h5.iter_particle_groups(srcs, u.pidx - 1, batch):
+        for (P, M), batches in h5.iter_particle_groups(srcs, u.pidx - 1, batch, budget_bytes=batch * per_col,
+                                                       reuse=True):
             for _, local in batches:
-                one_batch(P[:, :, local], M[:, :, local])
+                whole = len(local) == P.shape[-1] and (len(local) == 0 or (
+                    local[0] == 0 and local[-1] == len(local) - 1))
+                if whole:
+                    one_batch(P, M)
+                else:
+                    one_batch(P[:, :, local], M[:, :, local])
     finally:
         buf.free()
     return _finish_products(cfg, u.acc, u.lag_steps, u.grid, u.tau_s, u.tau_norm, u.t_s, u.t_norm, u.edges,

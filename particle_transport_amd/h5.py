@@ -123,10 +123,15 @@ class H5Dataset:
         a = self[...]
         return a if dtype is None else a.astype(dtype)
 
-    def read_box(self, start, count, dtype=None):
-        """numpy-order start/count; returns a C-ordered array of shape count."""
-        dt = np.dtype(dtype or self.dtype)
-        out = np.empty(tuple(int(c) for c in count), dtype=dt)
+    def read_box(self, start, count, dtype=None, out=None):
+        """numpy-order start/count; returns a C-ordered array of shape count
+        (`out`, a C-contiguous array of that shape, when given)."""
+        dt = np.dtype(dtype or (out.dtype if out is not None else self.dtype))
+        shape = tuple(int(c) for c in count)
+        if out is None:
+            out = np.empty(shape, dtype=dt)
+        elif out.shape != shape or out.dtype != dt or not out.flags.c_contiguous:
+            raise H5Error("read_box: out must be a C-contiguous array of the box's shape and dtype")
         _check(load().pt_h5_read_box(self.file._h, _name(self.name), _CODE[dt], _julia(start), _julia(count),
                                      out.ctypes.data_as(_vp)))
         return out
@@ -146,6 +151,41 @@ class H5Dataset:
         consecutive indices, as read_particle_batch does
         (compute_delta_mu2_dmumu_full.jl:411-437)."""
         return _read_columns(self, cols, dtype)
+
+
+_COPY_THREADS = 8
+_copy_pool = None
+
+
+def _copy_rows(dst, src):
+    """dst[...] = src (same shape), the rows of the first axis split over a
+    few threads: a box of a cadence cache is many short strided rows, which
+    one thread copies at a fraction of the memory bandwidth."""
+    global _copy_pool
+    n = dst.shape[0] if dst.ndim else 0
+    k = min(_COPY_THREADS, n)
+    if k < 2 or dst.nbytes < (8 << 20):
+        np.copyto(dst, src, casting="same_kind")
+        return
+    if _copy_pool is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _copy_pool = ThreadPoolExecutor(_COPY_THREADS)
+    b = np.linspace(0, n, k + 1).astype(int)
+    list(_copy_pool.map(lambda r: np.copyto(dst[r[0]:r[1]], src[r[0]:r[1]], casting="same_kind"),
+                        zip(b[:-1], b[1:])))
+
+
+def _reusable(count, dtype):
+    """A flat host buffer of `count` elements, page-locked when the library
+    can allocate it so (no device present: pageable)."""
+    try:
+        from ._lib import PinnedArray, PTError
+    except ImportError:  # the HIP library is not built: pageable
+        return np.empty(max(1, count), dtype=dtype)
+    try:
+        return PinnedArray((max(1, count),), dtype).array
+    except (PTError, OSError):
+        return np.empty(max(1, count), dtype=dtype)
 
 
 def _read_columns(ds, cols, dtype=None):
@@ -175,9 +215,15 @@ class ArrayDataset:
         self.dtype = np.dtype(array.dtype)
         self.ndim = len(self.shape)
 
-    def read_box(self, start, count, dtype=None):
+    def read_box(self, start, count, dtype=None, out=None):
         sl = tuple(slice(int(s), int(s) + int(c)) for s, c in zip(start, count))
-        return np.ascontiguousarray(self.a[sl], dtype=dtype or self.dtype)
+        src = self.a[sl]
+        if out is None:
+            out = np.empty(src.shape, dtype=dtype or self.dtype)
+        elif out.shape != src.shape or not out.flags.c_contiguous:
+            raise H5Error("read_box: out must be a C-contiguous array of the box's shape")
+        _copy_rows(out, src)
+        return out
 
     def read_columns(self, cols, dtype=None):
         return _read_columns(self, cols, dtype)
@@ -340,7 +386,7 @@ def scalar_bool_or(file, name, default):
     return default
 
 
-def iter_particle_groups(datasets, pidx0, chunk, budget_bytes=256 << 20, prefetch=True):
+def iter_particle_groups(datasets, pidx0, chunk, budget_bytes=256 << 20, prefetch=True, reuse=False):
     """Stream the selected particles of cadence datasets (numpy shapes
     (nsave, n) or (nsave, 3, n); particles on the last axis) in groups of
     whole analysis chunks.
@@ -355,7 +401,13 @@ def iter_particle_groups(datasets, pidx0, chunk, budget_bytes=256 << 20, prefetc
 
     Yields (arrays, chunks): one array per dataset holding the group's
     columns, and per analysis chunk (offset into pidx0, its column indices
-    into those arrays)."""
+    into those arrays).
+
+    reuse=True: a group's arrays are views of two sets of buffers allocated
+    once (page-locked when the host allows, so a device upload from them is
+    a straight DMA) and used in turn: a caller must be done with a group's
+    arrays when it asks for the group after next, as a caller that uploads
+    or consumes each group before asking for the next one is."""
     from concurrent.futures import ThreadPoolExecutor
 
     pidx0 = np.asarray(pidx0, dtype=np.int64)
@@ -381,13 +433,23 @@ def iter_particle_groups(datasets, pidx0, chunk, budget_bytes=256 << 20, prefetc
             plans.append(("runs", g0, c1, 0, 0))
         c0 = c1
 
-    def read(plan):
+    bufs = None
+    if reuse:
+        span = max((hi - lo + 1 for kind, _, _, lo, hi in plans if kind == "span"), default=0)
+        if span:
+            bufs = [[_reusable(int(np.prod(ds.shape[:-1])) * span, ds.dtype) for ds in datasets] for _ in range(2)]
+
+    def read(plan, slot=0):
         kind, g0, g1, lo, hi = plan
         if kind == "span":
             arrays = []
-            for ds in datasets:
+            for k, ds in enumerate(datasets):
                 lead = tuple(ds.shape[:-1])
-                arrays.append(ds.read_box((0,) * len(lead) + (lo,), lead + (hi - lo + 1,)))
+                count = lead + (hi - lo + 1,)
+                out = None
+                if bufs is not None:
+                    out = bufs[slot][k][:int(np.prod(count))].reshape(count)
+                arrays.append(ds.read_box((0,) * len(lead) + (lo,), count, out=out))
             base = lambda a, b: pidx0[a:b] - lo  # noqa: E731
         else:
             arrays = [ds.read_columns(pidx0[g0:g1]) for ds in datasets]
@@ -396,12 +458,12 @@ def iter_particle_groups(datasets, pidx0, chunk, budget_bytes=256 << 20, prefetc
         return arrays, chunks
 
     if not prefetch:
-        for plan in plans:
-            yield read(plan)
+        for k, plan in enumerate(plans):
+            yield read(plan, k & 1)
         return
     with ThreadPoolExecutor(max_workers=1) as ex:
-        fut = ex.submit(read, plans[0]) if plans else None
+        fut = ex.submit(read, plans[0], 0) if plans else None
         for k in range(len(plans)):
             cur = fut.result()
-            fut = ex.submit(read, plans[k + 1]) if k + 1 < len(plans) else None
+            fut = ex.submit(read, plans[k + 1], (k + 1) & 1) if k + 1 < len(plans) else None
             yield cur

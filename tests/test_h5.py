@@ -130,13 +130,14 @@ def test_particle_groups_follow_the_selection(tmp_path, selection):
     with h5.H5File(path, "r") as f:
         for src in (f.dataset("momenta"), h5.ArrayDataset(a)):
             for budget in (1 << 30, 20_000, 1):  # one span, several spans, per-chunk runs
-                seen = []
-                for (block,), chunks in h5.iter_particle_groups([src], pidx0, chunk, budget):
-                    for c0, local in chunks:
-                        want = pidx0[c0:c0 + chunk]
-                        assert np.array_equal(block[..., local], a[..., want])
-                        seen.append(c0)
-                assert seen == list(range(0, len(pidx0), chunk))
+                for reuse in (False, True):
+                    seen = []
+                    for (block,), chunks in h5.iter_particle_groups([src], pidx0, chunk, budget, reuse=reuse):
+                        for c0, local in chunks:
+                            want = pidx0[c0:c0 + chunk]
+                            assert np.array_equal(block[..., local], a[..., want])
+                            seen.append(c0)
+                    assert seen == list(range(0, len(pidx0), chunk))
 
 
 def _setup(cfg, F):
@@ -160,6 +161,25 @@ def _write_cache(path, kind, s, nsave, n, alive=None):
             for name in ("positions", "momenta"):
                 f.create_dataset(name, (nsave, 3, n), np.float32)
         cache_h5.finalize_cache(f, CFG, s, 10.0, alive, kind, np.float32)
+
+
+def test_read_box_into_given_buffers():
+    """A box large enough to be copied on several threads, read into a fresh
+    array and into a caller's buffer, equals the slice; a buffer of the wrong
+    shape is refused."""
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((20, 3, 70_000)).astype(np.float32)
+    src = h5.ArrayDataset(a)
+    start, count = (0, 0, 1_234), (20, 3, 60_001)
+    want = a[:, :, 1_234:61_235]
+    got = src.read_box(start, count)
+    assert got.flags.c_contiguous and np.array_equal(got, want)
+    out = np.full(count, np.nan, np.float32)
+    assert src.read_box(start, count, out=out) is out and np.array_equal(out, want)
+    wide = src.read_box(start, count, dtype=np.float64)
+    assert wide.dtype == np.float64 and np.array_equal(wide, want.astype(np.float64))
+    with pytest.raises(h5.H5Error):
+        src.read_box(start, count, out=np.empty((20, 3, 60_000), np.float32))
 
 
 def test_cache_metadata_and_validators(tmp_path):
