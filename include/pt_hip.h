@@ -98,6 +98,14 @@ int pt_device_memory(const pt_ctx *ctx, int64_t *free_bytes, int64_t *total_byte
 int pt_buffer_alloc(pt_ctx *ctx, int64_t bytes, void **dev_ptr);
 int pt_buffer_free(pt_ctx *ctx, void *dev_ptr, int64_t bytes);
 int pt_memcpy(pt_ctx *ctx, void *dst, const void *src, int64_t bytes, int direction);
+/* Strided copy: `rows` rows of `width` bytes, row r from src + r * src_pitch to
+ * dst + r * dst_pitch, by the copy engine (hipMemcpy2DAsync), then synchronised.
+ * Replaces the host gather plus CuArray upload of a batch of particle columns
+ * (read_particle_batch, compute_delta_mu2_dmumu_full.jl:411-437 and
+ * compute_dpp_full.jl:200-220, called at :1137-1140 and :507): a contiguous run of particles of a cadence
+ * cache in page-locked host memory goes to the device without the host reading it. */
+int pt_memcpy_2d(pt_ctx *ctx, void *dst, int64_t dst_pitch, const void *src, int64_t src_pitch, int64_t width,
+                 int64_t rows, int direction);
 /* Page-locked host memory for fast cache downloads (hipHostMalloc). */
 int pt_host_alloc(int64_t bytes, void **host_ptr);
 int pt_host_free(void *host_ptr);

@@ -30,7 +30,7 @@ EXPORTS = (
     "pt_mu_from_phase", "pt_mu_from_phase_layout", "pt_dmumu_accumulate", "pt_dmumu_accumulate_noalloc", "pt_dpp_accumulate",
     "pt_dpp_accumulate_pmag",
     "pt_dpp_accumulate_dprel",
-    "pt_energy_snapshots", "pt_buffer_alloc", "pt_buffer_free", "pt_memcpy", "pt_host_alloc", "pt_host_free",
+    "pt_energy_snapshots", "pt_buffer_alloc", "pt_buffer_free", "pt_memcpy", "pt_memcpy_2d", "pt_host_alloc", "pt_host_free",
     "pt_serve_create", "pt_serve_destroy", "pt_serve_add_field", "pt_serve_submit", "pt_serve_step",
     "pt_serve_drain", "pt_serve_take_done", "pt_serve_poll", "pt_serve_fetch", "pt_serve_copy_cache", "pt_serve_release",
     "pt_serve_stats",
@@ -111,6 +111,7 @@ def load():
         "pt_buffer_alloc": (_int, [_vp, _i64, ctypes.POINTER(_vp)]),
         "pt_buffer_free": (_int, [_vp, _vp, _i64]),
         "pt_memcpy": (_int, [_vp, _vp, _vp, _i64, _int]),
+        "pt_memcpy_2d": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _int]),
         "pt_host_alloc": (_int, [_i64, ctypes.POINTER(_vp)]),
         "pt_host_free": (_int, [_vp]),
     }
@@ -174,6 +175,19 @@ class DeviceBuffer:
         assert arr.nbytes == self.nbytes
         check(load().pt_memcpy(self.ctx.h, ctypes.c_void_p(self.ptr), ptr(arr), self.nbytes, 0))
 
+    def upload_columns(self, src, lo):
+        """self (shape lead + (w,)) = src[..., lo:lo + w] for a C-contiguous
+        host array src of shape lead + (n,): one strided copy by the copy
+        engine, which reads page-locked memory (is_pinned) in place."""
+        if not (isinstance(src, np.ndarray) and src.flags.c_contiguous and src.dtype == self.dtype):
+            raise ValueError("upload_columns: src must be a C-contiguous array of the buffer's dtype")
+        w, n, isz = self.shape[-1], src.shape[-1], self.dtype.itemsize
+        if tuple(src.shape[:-1]) != tuple(self.shape[:-1]) or not 0 <= lo <= n - w:
+            raise ValueError("upload_columns: columns out of range or shapes differ")
+        rows = int(np.prod(self.shape[:-1]))
+        check(load().pt_memcpy_2d(self.ctx.h, ctypes.c_void_p(self.ptr), w * isz,
+                                  ctypes.c_void_p(src.ctypes.data + lo * isz), n * isz, w * isz, rows, 0))
+
     def to_host(self):
         out = np.empty(self.shape, dtype=self.dtype)
         check(load().pt_memcpy(self.ctx.h, ptr(out), ctypes.c_void_p(self.ptr), self.nbytes, 1))
@@ -190,7 +204,21 @@ class DeviceBuffer:
         self.ptr = None
 
 
+_pinned = {}  # start address -> bytes of each live PinnedArray allocation
+
+
+def is_pinned(arr):
+    """True when all of arr's memory lies inside one live PinnedArray
+    allocation (page-locked, readable by the device's copy engine)."""
+    if not isinstance(arr, np.ndarray) or arr.size == 0:
+        return False
+    from numpy.lib.array_utils import byte_bounds
+    lo, hi = byte_bounds(arr)
+    return any(p <= lo and hi <= p + n for p, n in list(_pinned.items()))
+
+
 def _host_free(ptr_value):
+    _pinned.pop(ptr_value, None)
     try:
         load().pt_host_free(ctypes.c_void_p(ptr_value))
     except Exception:  # interpreter shutdown: the process releases it anyway
@@ -209,6 +237,7 @@ class PinnedArray:
         p = ctypes.c_void_p()
         check(load().pt_host_alloc(nbytes, ctypes.byref(p)))
         self.ptr = p.value
+        _pinned[self.ptr] = max(nbytes, 1)
         buf = (ctypes.c_char * max(nbytes, 1)).from_address(self.ptr)
         weakref.finalize(buf, _host_free, self.ptr)  # every view keeps buf alive
         self.array = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(self.shape))).reshape(self.shape)

@@ -609,6 +609,23 @@ int pt_memcpy(pt_ctx *ctx, void *dst, const void *src, int64_t bytes, int direct
     });
 }
 
+int pt_memcpy_2d(pt_ctx *ctx, void *dst, int64_t dst_pitch, const void *src, int64_t src_pitch, int64_t width,
+                 int64_t rows, int direction) {
+    return guarded([&] {
+        PT_REQUIRE(ctx && dst && src, "pt_memcpy_2d: null argument");
+        PT_REQUIRE(width >= 0 && rows >= 0, "pt_memcpy_2d: negative size");
+        PT_REQUIRE(dst_pitch >= width && src_pitch >= width, "pt_memcpy_2d: a pitch is narrower than a row");
+        PT_REQUIRE(direction >= 0 && direction <= 2, "pt_memcpy_2d: direction must be 0, 1 or 2");
+        if (width == 0 || rows == 0) return;
+        const hipMemcpyKind k = direction == 0 ? hipMemcpyHostToDevice
+                                : direction == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        PT_HIP_CHECK(hipSetDevice(ctx->device));
+        PT_HIP_CHECK(hipMemcpy2DAsync(dst, static_cast<size_t>(dst_pitch), src, static_cast<size_t>(src_pitch),
+                                      static_cast<size_t>(width), static_cast<size_t>(rows), k, ctx->stream));
+        PT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 int pt_host_alloc(int64_t bytes, void **host_ptr) {
     return guarded([&] {
         PT_REQUIRE(host_ptr && bytes >= 0, "pt_host_alloc: bad argument");

@@ -336,23 +336,61 @@ def gpu_batch_particles(cfg, chunk, bytes_per_particle, budget_bytes=256 << 20):
     return max(chunk, (min(max(want, chunk), cap) // chunk) * chunk)
 
 
+def _direct_sources(sources, dtypes):
+    """True when every source is a C-contiguous page-locked host array
+    (a cache the runner keeps in host memory) of the wanted dtype: the
+    device's copy engine then reads a group's columns in place."""
+    return all(isinstance(s, np.ndarray) and s.flags.c_contiguous and s.dtype == np.dtype(dt) and _lib.is_pinned(s)
+               for s, dt in zip(sources, dtypes))
+
+
 def device_particle_groups(ctx, sources, pidx0, chunk, dtypes, budget_bytes=256 << 20):
     """h5.iter_particle_groups over host sources (numpy arrays, memory maps
     or h5.H5Datasets), each group uploaded once into reusable device buffers:
     yields (device views, chunks).  The next group is read from the host
-    source while the GPU works on the current one."""
+    source while the GPU works on the current one.  Page-locked sources of
+    the wanted dtype skip the host read: a group's span of columns goes to
+    the device in one strided copy per source."""
+    bufs = [None] * len(sources)
+
+    def dev(k, shape, dt):
+        n = max(1, int(np.prod(shape)))
+        if bufs[k] is None or bufs[k].nbytes < n * np.dtype(dt).itemsize:
+            if bufs[k] is not None:
+                bufs[k].free()
+            bufs[k] = _lib.DeviceBuffer(ctx, (n,), dt)
+        return bufs[k].view(shape)
+
+    if _direct_sources(sources, dtypes):
+        pidx0 = np.asarray(pidx0, dtype=np.int64)
+        per_col = max(int(np.prod(s.shape[:-1])) * s.dtype.itemsize for s in sources)
+        try:
+            for plan in h5.plan_particle_groups(pidx0, chunk, max(1, int(budget_bytes) // max(1, per_col))):
+                kind, g0, g1, lo, hi = plan
+                views = []
+                for k, (s, dt) in enumerate(zip(sources, dtypes)):
+                    if kind == "span":
+                        v = dev(k, tuple(s.shape[:-1]) + (hi - lo + 1,), dt)
+                        v.upload_columns(s, lo)
+                    else:
+                        a = np.ascontiguousarray(np.take(s, pidx0[g0:g1], axis=-1), dtype=dt)
+                        v = dev(k, a.shape, dt)
+                        v.upload(a)
+                    views.append(v)
+                yield views, h5.plan_chunks(plan, pidx0, chunk)
+        finally:
+            for b in bufs:
+                if b is not None:
+                    b.free()
+        return
+
     srcs = [s if hasattr(s, "read_box") else h5.ArrayDataset(np.asarray(s)) for s in sources]
-    bufs = [None] * len(srcs)
     try:
         for arrays, chunks in h5.iter_particle_groups(srcs, pidx0, chunk, budget_bytes, reuse=True):
             views = []
             for k, (a, dt) in enumerate(zip(arrays, dtypes)):
                 a = np.ascontiguousarray(a, dtype=dt)
-                if bufs[k] is None or bufs[k].nbytes < a.nbytes:
-                    if bufs[k] is not None:
-                        bufs[k].free()
-                    bufs[k] = _lib.DeviceBuffer(ctx, (max(1, a.size),), dt)
-                v = bufs[k].view(a.shape)
+                v = dev(k, a.shape, dt)
                 v.upload(a)
                 views.append(v)
             yield views, chunks
@@ -439,7 +477,9 @@ def run_combined_full(cfg, ctx, field, positions, momenta, t_s, t_norm, t_gyrope
     (nsave, 3, n) as the phase-space cache stores them (Julia (n, 3, nsave)),
     e.g. memory maps; each chunk's columns are gathered on the host like
     read_particle_batch (:411-437), or h5.H5Datasets of a phase-space cache
-    file, streamed in groups of chunks.  field: a planar Field with B (the
+    file, streamed in groups of chunks.  A page-locked cache (the runner's
+    in-memory form) is not read by the host at all: the copy engine moves
+    each batch's columns straight to the device.  field: a planar Field with B (the
     reference loads B in compute_precision, Float32 by default).  Returns the
     product groups like run_combined_from_mu_cache."""
     nsteps, three, total = tuple(positions.shape)
@@ -454,27 +494,40 @@ def run_combined_full(cfg, ctx, field, positions, momenta, t_s, t_norm, t_gyrope
     buf = _lib.DeviceBuffer(ctx, (nsteps, min(batch, len(u.pidx))), np.float32)
 
     def one_batch(pos, mom):
+        if not isinstance(pos, _lib.DeviceBuffer):
+            pos, mom = np.ascontiguousarray(pos, dtype=T), np.ascontiguousarray(mom, dtype=T)
         # sample-major on the device: the pair walk then reads it coalesced
-        mu = reconstruct_mu_chunk(ctx, field, np.ascontiguousarray(pos, dtype=T), np.ascontiguousarray(mom, dtype=T),
-                                  out=buf.view((nsteps, pos.shape[-1])), sample_major=True)
+        mu = reconstruct_mu_chunk(ctx, field, pos, mom, out=buf.view((nsteps, pos.shape[-1])), sample_major=True)
         process_mu_chunk_dmumu(ctx, mu, u.lag_steps, u.edges, u.acc, start_mode, bin_abs, layout="particle_fastest")
 
-    srcs = [a if hasattr(a, "read_box") else h5.ArrayDataset(np.asarray(a)) for a in (positions, momenta)]
-    # One batch per group: the group's array (read on the worker thread while
-    # the GPU works) is then the batch itself, contiguous, and goes to the
-    # device without a second gather on the host (a scattered selection, whose
-    # group spans more columns than it selects, is still gathered).
-    per_col = max(int(np.prod(s.shape[:-1])) * np.dtype(s.dtype).itemsize for s in srcs)
+    def whole(local, width):
+        return len(local) == width and (len(local) == 0 or (local[0] == 0 and local[-1] == len(local) - 1))
+
+    # One batch per group: the group's arrays are then the batch itself and
+    # go to the device without a second gather (a scattered selection, whose
+    # group spans more columns than it selects, is still gathered on the host).
     try:
-        for (P, M), batches in h5.iter_particle_groups(srcs, u.pidx - 1, batch, budget_bytes=batch * per_col,
-                                                       reuse=True):
-            for _, local in batches:
-                whole = len(local) == P.shape[-1] and (len(local) == 0 or (
-                    local[0] == 0 and local[-1] == len(local) - 1))
-                if whole:
-                    one_batch(P, M)
-                else:
-                    one_batch(P[:, :, local], M[:, :, local])
+        if _direct_sources((positions, momenta), (T, T)):
+            # page-locked cache: the copy engine reads each group's columns in place
+            per_col = nsteps * 3 * np.dtype(T).itemsize
+            for (P, M), batches in device_particle_groups(ctx, (positions, momenta), u.pidx - 1, batch, (T, T),
+                                                          budget_bytes=batch * per_col):
+                for c0, local in batches:
+                    if whole(local, P.shape[-1]):
+                        one_batch(P, M)
+                    else:
+                        cols = u.pidx[c0:c0 + len(local)] - 1
+                        one_batch(np.take(positions, cols, axis=-1), np.take(momenta, cols, axis=-1))
+        else:
+            srcs = [a if hasattr(a, "read_box") else h5.ArrayDataset(np.asarray(a)) for a in (positions, momenta)]
+            per_col = max(int(np.prod(s.shape[:-1])) * np.dtype(s.dtype).itemsize for s in srcs)
+            for (P, M), batches in h5.iter_particle_groups(srcs, u.pidx - 1, batch, budget_bytes=batch * per_col,
+                                                           reuse=True):
+                for _, local in batches:
+                    if whole(local, P.shape[-1]):
+                        one_batch(P, M)
+                    else:
+                        one_batch(P[:, :, local], M[:, :, local])
     finally:
         buf.free()
     return _finish_products(cfg, u.acc, u.lag_steps, u.grid, u.tau_s, u.tau_norm, u.t_s, u.t_norm, u.edges,

@@ -412,7 +412,47 @@ def iter_particle_groups(datasets, pidx0, chunk, budget_bytes=256 << 20, prefetc
 
     pidx0 = np.asarray(pidx0, dtype=np.int64)
     per_col = max(int(np.prod(ds.shape[:-1])) * ds.dtype.itemsize for ds in datasets)
-    cap = max(1, int(budget_bytes) // max(1, per_col))
+    plans = plan_particle_groups(pidx0, chunk, max(1, int(budget_bytes) // max(1, per_col)))
+
+    bufs = None
+    if reuse:
+        span = max((hi - lo + 1 for kind, _, _, lo, hi in plans if kind == "span"), default=0)
+        if span:
+            bufs = [[_reusable(int(np.prod(ds.shape[:-1])) * span, ds.dtype) for ds in datasets] for _ in range(2)]
+
+    def read(plan, slot=0):
+        kind, g0, g1, lo, hi = plan
+        if kind == "span":
+            arrays = []
+            for k, ds in enumerate(datasets):
+                lead = tuple(ds.shape[:-1])
+                count = lead + (hi - lo + 1,)
+                out = None
+                if bufs is not None:
+                    out = bufs[slot][k][:int(np.prod(count))].reshape(count)
+                arrays.append(ds.read_box((0,) * len(lead) + (lo,), count, out=out))
+        else:
+            arrays = [ds.read_columns(pidx0[g0:g1]) for ds in datasets]
+        return arrays, plan_chunks(plan, pidx0, chunk)
+
+    if not prefetch:
+        for k, plan in enumerate(plans):
+            yield read(plan, k & 1)
+        return
+    with ThreadPoolExecutor(max_workers=1) as ex:
+        fut = ex.submit(read, plans[0], 0) if plans else None
+        for k in range(len(plans)):
+            cur = fut.result()
+            fut = ex.submit(read, plans[k + 1], (k + 1) & 1) if k + 1 < len(plans) else None
+            yield cur
+
+
+def plan_particle_groups(pidx0, chunk, cap):
+    """The groups of iter_particle_groups: consecutive whole chunks of pidx0
+    grouped while their index span stays within cap columns and the indices
+    ascend, as ("span", g0, g1, lo, hi) (chunks pidx0[g0:g1], columns lo..hi
+    inclusive), else one chunk as ("runs", g0, g1, 0, 0)."""
+    pidx0 = np.asarray(pidx0, dtype=np.int64)
     plans = []
     c0 = 0
     while c0 < len(pidx0):
@@ -432,38 +472,14 @@ def iter_particle_groups(datasets, pidx0, chunk, budget_bytes=256 << 20, prefetc
         else:
             plans.append(("runs", g0, c1, 0, 0))
         c0 = c1
+    return plans
 
-    bufs = None
-    if reuse:
-        span = max((hi - lo + 1 for kind, _, _, lo, hi in plans if kind == "span"), default=0)
-        if span:
-            bufs = [[_reusable(int(np.prod(ds.shape[:-1])) * span, ds.dtype) for ds in datasets] for _ in range(2)]
 
-    def read(plan, slot=0):
-        kind, g0, g1, lo, hi = plan
-        if kind == "span":
-            arrays = []
-            for k, ds in enumerate(datasets):
-                lead = tuple(ds.shape[:-1])
-                count = lead + (hi - lo + 1,)
-                out = None
-                if bufs is not None:
-                    out = bufs[slot][k][:int(np.prod(count))].reshape(count)
-                arrays.append(ds.read_box((0,) * len(lead) + (lo,), count, out=out))
-            base = lambda a, b: pidx0[a:b] - lo  # noqa: E731
-        else:
-            arrays = [ds.read_columns(pidx0[g0:g1]) for ds in datasets]
-            base = lambda a, b: np.arange(a - g0, b - g0, dtype=np.int64)  # noqa: E731
-        chunks = [(a, base(a, min(a + chunk, g1))) for a in range(g0, g1, chunk)]
-        return arrays, chunks
-
-    if not prefetch:
-        for k, plan in enumerate(plans):
-            yield read(plan, k & 1)
-        return
-    with ThreadPoolExecutor(max_workers=1) as ex:
-        fut = ex.submit(read, plans[0], 0) if plans else None
-        for k in range(len(plans)):
-            cur = fut.result()
-            fut = ex.submit(read, plans[k + 1], (k + 1) & 1) if k + 1 < len(plans) else None
-            yield cur
+def plan_chunks(plan, pidx0, chunk):
+    """Per analysis chunk of a group: (offset into pidx0, its column indices
+    into the group's arrays: columns lo..hi of a span, the gathered columns
+    of runs)."""
+    kind, g0, g1, lo, _ = plan
+    if kind == "span":
+        return [(a, pidx0[a:min(a + chunk, g1)] - lo) for a in range(g0, g1, chunk)]
+    return [(a, np.arange(a - g0, min(a + chunk, g1) - g0, dtype=np.int64)) for a in range(g0, g1, chunk)]
