@@ -20,7 +20,10 @@
 #     reconstruct_mu_step_kernel!,
 #     multimode_multienergy_cache_pipeline.jl:1467-1498
 #   reconstruct_mu_chunk_gpu, compute_delta_mu2_curve.jl:186-225
-#                                                           reconstruct_mu_chunk_gpu!
+#                                                           reconstruct_mu_chunk_gpu!,
+#                                                           reconstruct_mu_chunk_device!
+#   read_particle_batch + CuArray,                          upload_particle_range!
+#     compute_delta_mu2_dmumu_full.jl:411-437
 #   process_mu_chunk_dmumu_cpu!,                            process_mu_chunk_dmumu!
 #     compute_delta_mu2_dmumu_full.jl:656-704
 #   process_momenta_chunk_dpp_cpu!, compute_dpp_full.jl:247 process_momenta_chunk_dpp!
@@ -42,6 +45,7 @@ const PUSH_REF_F64, PUSH_PLAIN_F32, PUSH_FAST = Cint.(0:2)
 const OUT_NONE, OUT_PHASE, OUT_MU, OUT_PMAG, OUT_DPREL = Cint.(0:4)
 const REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_RELEASED = Cint.(0:3)
 
+abi_version() = ccall((:pt_abi_version, LIB), Cint, ())
 last_error() = unsafe_string(ccall((:pt_last_error, LIB), Cstring, ()))
 check(rc::Cint) = rc == 0 || error("libpt_hip: " * last_error())
 
@@ -107,6 +111,12 @@ function ctx_create(device::Integer=0)
 end
 ctx_destroy(ctx) = check(ccall((:pt_ctx_destroy, LIB), Cint, (Ptr{Cvoid},), ctx))
 
+function bytes_in_use(ctx)
+    b = Ref{Int64}(0)
+    check(ccall((:pt_ctx_bytes_in_use, LIB), Cint, (Ptr{Cvoid}, Ref{Int64}), ctx, b))
+    return b[]
+end
+
 function device_memory(ctx)
     f, t = Ref{Int64}(0), Ref{Int64}(0)
     check(ccall((:pt_device_memory, LIB), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}), ctx, f, t))
@@ -125,6 +135,32 @@ buffer_free(ctx, p, bytes::Integer) =
 memcpy!(ctx, dst, src, bytes::Integer, direction::Integer) =
     check(ccall((:pt_memcpy, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Cint),
                 ctx, dst, src, bytes, direction))
+
+# Page-locked host arrays (hipHostMalloc) for caches kept in host memory: the
+# copy engine reads them in place.  Release with host_free, not by the GC.
+function host_alloc(::Type{T}, dims::Integer...) where {T}
+    p = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pt_host_alloc, LIB), Cint, (Int64, Ref{Ptr{Cvoid}}), prod(dims) * sizeof(T), p))
+    return unsafe_wrap(Array, Ptr{T}(p[]), Int.(dims); own=false)
+end
+host_free(a::Array) = check(ccall((:pt_host_free, LIB), Cint, (Ptr{Cvoid},), pointer(a)))
+
+# read_particle_batch + CuArray (compute_delta_mu2_dmumu_full.jl:411-437;
+# compute_dpp_full.jl:200-220) for a contiguous range: particles
+# first:first+count-1 of a cadence cache (n, 3, nsave) or (n, nsave), best in
+# host_alloc memory, to a device buffer holding (count, 3, nsave) or
+# (count, nsave), by one strided copy of the copy engine.
+function upload_particle_range!(ctx, dev::Ptr{Cvoid}, cache::Array{T}, first::Integer, count::Integer) where {T}
+    n = size(cache, 1)
+    (first >= 1 && count >= 0 && first + count - 1 <= n) || error("upload_particle_range!: range outside the cache")
+    w = Int64(count) * sizeof(T)
+    GC.@preserve cache begin
+        check(ccall((:pt_memcpy_2d, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64, Int64, Cint),
+                    ctx, dev, w, pointer(cache, first), Int64(n) * sizeof(T), w, div(length(cache), n), 0))
+    end
+    return nothing
+end
 
 # ----------------------------------------------------------------- fields --
 
@@ -259,6 +295,18 @@ function reconstruct_mu_chunk_gpu!(mu::Matrix{T}, ctx, field, positions::Array{S
     return mu
 end
 
+# The same reconstruction from device phase space (upload_particle_range!)
+# into a device buffer, sample-major (C [nsave][chunk]): the layout
+# process_mu_cache_dmumu! walks with n = chunk.  μ stays in HBM between the
+# two passes (run_combined_full, compute_delta_mu2_dmumu_full.jl:1131-1167).
+function reconstruct_mu_chunk_device!(mu_dev::Ptr{Cvoid}, ctx, field, pos_dev::Ptr{Cvoid}, mom_dev::Ptr{Cvoid},
+                                      chunk::Integer, nsave::Integer, ::Type{S}) where {S<:Union{Float32,Float64}}
+    check(ccall((:pt_mu_from_phase_layout, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Cint),
+                ctx, field, chunk, nsave, dtype_code(S), 1, pos_dev, mom_dev, PT_F32, 1, mu_dev, 1))
+    return nothing
+end
+
 # ----------------------------------------------------------- statistics --
 
 # process_mu_chunk_dmumu_cpu! (compute_delta_mu2_dmumu_full.jl:656-704): the
@@ -298,6 +346,28 @@ function process_mu_cache_dmumu!(ctx, mu_dev::Ptr{Cvoid}, nsave::Integer, n::Int
                      Ptr{Float64}, Int64, Cint, Cint, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
                      Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}),
                     ctx, mu_dev, 1, nsave, n_sel, n, 1, C_NULL, lag_steps, length(lag_steps), mu_edges,
+                    length(mu_edges) - 1, mode, bin_abs, particle_counts, particle_means, particle_m2s,
+                    pair_sum_squares, pair_counts, dmumu_counts, dmumu_sum_delta, dmumu_sum_delta2))
+    end
+    return nothing
+end
+
+# process_mu_cache_dmumu! for callers that cannot allocate device memory
+# (code under graph capture, serving loops): no counterpart in the reference.
+# Device work areas live in the library's fixed 32 MiB array per GPU; limits
+# and the race between overlapping calls on one GPU as in include/pt_hip.h.
+function process_mu_cache_dmumu_noalloc!(ctx, mu_dev::Ptr{Cvoid}, nsave::Integer, n::Integer, n_sel::Integer,
+                                         lag_steps::Vector{Int64}, mu_edges::Vector{Float64}, acc,
+                                         start_mode::Symbol, bin_abs::Bool)
+    mode = start_mode == :injection ? Cint(1) : Cint(0)
+    (; particle_counts, particle_means, particle_m2s, pair_sum_squares, pair_counts, dmumu_counts,
+       dmumu_sum_delta, dmumu_sum_delta2) = acc
+    GC.@preserve lag_steps mu_edges acc begin
+        check(ccall((:pt_dmumu_accumulate_noalloc, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Int64, Int64, Ptr{Int64}, Ptr{Int64}, Int64,
+                     Ptr{Float64}, Int64, Cint, Cint, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                     Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}),
+                    ctx, mu_dev, nsave, n_sel, n, 1, C_NULL, lag_steps, length(lag_steps), mu_edges,
                     length(mu_edges) - 1, mode, bin_abs, particle_counts, particle_means, particle_m2s,
                     pair_sum_squares, pair_counts, dmumu_counts, dmumu_sum_delta, dmumu_sum_delta2))
     end

@@ -30,6 +30,49 @@ def test_binding_covers_declarations_and_abi_version():
     assert L.pt_abi_version() == 1
 
 
+def _balanced(text, i):
+    """text[i] == "(": the text inside the matching parenthesis."""
+    depth = 0
+    for j in range(i, len(text)):
+        depth += {"(": 1, ")": -1}.get(text[j], 0)
+        if depth == 0:
+            return text[i + 1:j]
+    raise ValueError("unbalanced")
+
+
+def _top_level_items(inner):
+    items, depth, cur = [], 0, ""
+    for ch in inner:
+        depth += ch in "({[" and 1 or (ch in ")}]" and -1 or 0)
+        if ch == "," and depth == 0:
+            items.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    items.append(cur)
+    return [x.strip() for x in items if x.strip()]
+
+
+def test_julia_module_binds_every_declaration_with_its_arity():
+    """julia/PTHip.jl (the reference-side ccall layer) binds each entry point
+    include/pt_hip.h declares, with one argument type per C parameter."""
+    hdr = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "pt_hip.h")).read(), flags=re.S)
+    arity = {}
+    for m in re.finditer(r"\b(pt_[a-z0-9_]+)\s*\(", hdr):
+        params = _top_level_items(_balanced(hdr, m.end() - 1))
+        arity[m.group(1)] = 0 if params == ["void"] else len(params)
+    jl = open(os.path.join(ROOT, "julia", "PTHip.jl")).read()
+    bound = {}
+    for m in re.finditer(r"ccall\(\(:(pt_[a-z0-9_]+), LIB\)", jl):
+        call = _top_level_items(_balanced(jl, m.start() + len("ccall")))
+        types = call[2]
+        assert types.startswith("("), (m.group(1), types)
+        bound.setdefault(m.group(1), set()).add(len(_top_level_items(types[1:-1])))
+    assert set(arity) == set(bound), sorted(set(arity) ^ set(bound))
+    for name, n in arity.items():
+        assert bound[name] == {n}, (name, n, bound[name])
+
+
 def test_errors_are_reported_not_thrown():
     L = _lib.load()
     # a null out-pointer is rejected with a message before any device call
