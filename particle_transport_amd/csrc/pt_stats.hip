@@ -655,15 +655,17 @@ __global__ void k_pmag(const float *__restrict__ mom, int64_t nsave, int64_t tot
 }
 
 // D_pp pairs over a series [save][total]: the walk of k_dmumu without bins.
-// T = double, kDelta = false: the FP64 |p| series; inv_p0 = 1/p0 (the
-// normalised sums differ from Delta p / p0 by at most an ulp per term).
-// T = float, kDelta = true: a relative series delta = |p|/p_ref - 1; the
-// kernel sums d = delta(s+lag) - delta(s) (exact in FP64) and d^2 only, and the
-// host scales them by p_ref and p_ref/p0.
+// Per lag it sums d = series(s+lag) - series(s) and d^2 over the valid pairs.
+// T = double, kDelta = false: the FP64 |p| series, d = Delta p; the host forms
+// the normalised sums as (sum d)/p0 and (sum d^2)/p0^2, the reference's
+// sum(Delta p/p0) and sum((Delta p/p0)^2) up to the rounding of the terms
+// (two FP64 accumulators per lag fewer in the walk).
+// T = float, kDelta = true: a relative series delta = |p|/p_ref - 1 (d exact
+// in FP64); the host scales the sums by p_ref and p_ref/p0.
 template <class T, bool kDelta, bool kBuf>
 __global__ void __launch_bounds__(kTPB)
 k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
-      const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags, double inv_p0,
+      const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags,
       double *__restrict__ part_d, long long *__restrict__ part_i, int nbx, int ngy,
       const int32_t *__restrict__ fin, int lane32) {
     constexpr int kG = kDppLags<T>;
@@ -680,11 +682,11 @@ k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
     }
     const int64_t j = static_cast<int64_t>(tg.tile) * kTPB + threadIdx.x;
     unsigned cnt[kG];
-    double s1[kG], s2[kG], s3[kG], s4[kG];
+    double s1[kG], s2[kG];
 #pragma unroll
     for (int g = 0; g < kG; ++g) {
         cnt[g] = 0;
-        s1[g] = s2[g] = s3[g] = s4[g] = 0.0;
+        s1[g] = s2[g] = 0.0;
     }
     int64_t lag_min = lag[0], lag_max = lag[0], off[kG];
 #pragma unroll
@@ -722,11 +724,6 @@ k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
                 cnt[g] += ok ? 1u : 0u;
                 s1[g] += d;
                 s2[g] = fma(d, d, s2[g]);
-                if constexpr (!kDelta) {
-                    const double dn = d * inv_p0;
-                    s3[g] += dn;
-                    s4[g] = fma(dn, dn, s4[g]);
-                }
             }
         };
         bool all[kG];
@@ -764,11 +761,6 @@ k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
                     const double d = static_cast<double>(pe[g]) - pst;
                     s1[g] += d;
                     s2[g] = fma(d, d, s2[g]);
-                    if constexpr (!kDelta) {
-                        const double dn = d * inv_p0;
-                        s3[g] += dn;
-                        s4[g] = fma(dn, dn, s4[g]);
-                    }
                 }
             };
             // three samples in flight in rotating registers, every load
@@ -830,18 +822,196 @@ k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
         const long long c = block_sum_i(static_cast<long long>(cnt[g]), sh_i);
         const double a = block_sum_d(s1[g], sh_d);
         const double b = block_sum_d(s2[g], sh_d);
-        const double cc = block_sum_d(s3[g], sh_d);
-        const double dd = block_sum_d(s4[g], sh_d);
         if (threadIdx.x == 0 && li < nlags) {
             const int64_t o = static_cast<int64_t>(li) * nbx + tg.tile;
             part_i[o] = c;
-            part_d[o * 4 + 0] = a;
-            part_d[o * 4 + 1] = b;
-            part_d[o * 4 + 2] = cc;
-            part_d[o * 4 + 3] = dd;
+            part_d[o * 2 + 0] = a;
+            part_d[o * 2 + 1] = b;
         }
     }
 }
+
+// D_pp walk over LDS tiles.  A workgroup stages the whole series of kP
+// selected particles in LDS (rows [save][kP] of T, read from HBM once) and
+// walks every lag over them: wave w takes lag groups w, w + 4, ...; lane
+// (k, p) = (lane / kP, lane % kP) walks the k-th of 64 / kP runs of
+// consecutive starts of particle p.  The run length is odd, so the lanes of a
+// read fall on distinct LDS banks.  (k_dpp above re-reads each sample from
+// the L2 once per lag; here the L2 sees it once per workgroup.)  A workgroup
+// takes tiles first, first + gridDim.x, ... (first: below) and adds each
+// lag's wave sum into its own slot part_*[lag * gridDim.x + blockIdx.x] in
+// tile order, so the result does not depend on scheduling.  Tiles whose particles are finite at
+// every sample (fin == nsave) skip the finiteness tests; their counts are
+// np * (nsave - lag).
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Lags per wave pass of k_dpp_tile: with eight, the float series' walk held
+// 134 VGPRs (three waves per SIMD) and ran 15% slower; the FP64 walk is bound by
+// LDS bandwidth either way (five reads of eight bytes per four pairs).
+constexpr int kTileLags = 4;
+
+template <class T, bool kDelta, int kP>
+__global__ void __launch_bounds__(kTPB)
+k_dpp_tile(const T *__restrict__ pm, int32_t nsave, int64_t total, int64_t n_sel,
+           const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags,
+           double *__restrict__ part_d, long long *__restrict__ part_i, const int32_t *__restrict__ fin) {
+    constexpr int kG = kTileLags;
+    constexpr int kK = 64 / kP;
+    static_assert(kP >= 1 && kP <= 64 && (kP & (kP - 1)) == 0, "kP: a power of two up to 64");
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *rows = reinterpret_cast<T *>(smem);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int p = lane & (kP - 1), k = lane / kP;
+    const int ngroups = (nlags + kG - 1) / kG;
+    const int64_t ntiles = (n_sel + kP - 1) / kP;
+    const int nel = nsave * kP;
+    // Consecutive tiles share cache lines (a row of kP < 16 FP64 values is
+    // part of a line), so they run on one XCD at the same time: within each
+    // run of 32 workgroups, the four on XCD x (= blockIdx % 8) take tiles
+    // 4x .. 4x + 3.  A bijection on [0, gridDim.x).
+    const int b = static_cast<int>(blockIdx.x);
+    const int full = static_cast<int>(gridDim.x) & ~31;
+    const int first = b < full ? (b & ~31) + 4 * (b & 7) + ((b & 31) >> 3) : b;
+    for (int64_t tile = first; tile < ntiles; tile += gridDim.x) {
+        const int64_t j0 = tile * kP;
+        const int np = n_sel - j0 < kP ? static_cast<int>(n_sel - j0) : kP;
+        __syncthreads();  // the previous tile's walk is done with the rows
+        // stage: eight loads in flight per thread before their LDS writes
+        for (int e0 = threadIdx.x; e0 < nel; e0 += 8 * kTPB) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * kTPB;
+                const int q = e & (kP - 1);
+                v[u] = T(0);
+                if (e < nel && q < np) {
+                    const int64_t col = pidx ? pidx[j0 + q] : j0 + q;
+                    v[u] = pm[static_cast<int64_t>(e / kP) * total + col];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (e0 + u * kTPB < nel) rows[e0 + u * kTPB] = v[u];
+        }
+        const bool finite = __syncthreads_and(threadIdx.x >= np || fin[j0 + threadIdx.x] >= nsave) != 0;
+        const T *col = rows + p;
+        for (int gy = wave; gy < ngroups; gy += kTPB / 64) {
+            int32_t off[kG];
+            bool valid[kG];
+            int32_t lmin = nsave, lmax = 0;
+#pragma unroll
+            for (int g = 0; g < kG; ++g) {
+                const int li = gy * kG + g;
+                const int64_t lv = li < nlags ? lags[li] : nsave;
+                valid[g] = lv < nsave;
+                off[g] = valid[g] ? static_cast<int32_t>(lv) : 0;
+                if (valid[g]) {
+                    lmin = off[g] < lmin ? off[g] : lmin;
+                    lmax = off[g] > lmax ? off[g] : lmax;
+                }
+            }
+            double s1[kG], s2[kG];
+            unsigned cnt[kG];
+#pragma unroll
+            for (int g = 0; g < kG; ++g) {
+                s1[g] = s2[g] = 0.0;
+                cnt[g] = 0;
+            }
+            if (lmin < nsave) {
+                // starts [0, nsave - lmin); below nsave - lmax every lag is in range
+                const int s_end = nsave - lmin, s_main = nsave - lmax;
+                const int seg = ((s_end + kK - 1) / kK) | 1;
+                const int s_lo = k * seg < s_end ? k * seg : s_end;
+                const int s_hi = s_lo + seg < s_end ? s_lo + seg : s_end;
+                const int m_hi = s_hi < s_main ? s_hi : s_main;
+                const int t_lo = s_lo > s_main ? s_lo : s_main;
+                if (finite) {
+                    // a lag out of range pairs the start with itself: d = 0
+#pragma unroll 2
+                    for (int s = s_lo; s < m_hi; ++s) {
+                        const double a = static_cast<double>(col[s * kP]);
+#pragma unroll
+                        for (int g = 0; g < kG; ++g) {
+                            const double d = static_cast<double>(col[(s + off[g]) * kP]) - a;
+                            s1[g] += d;
+                            s2[g] = fma(d, d, s2[g]);
+                        }
+                    }
+                    for (int s = t_lo; s < s_hi; ++s) {
+                        const double a = static_cast<double>(col[s * kP]);
+#pragma unroll
+                        for (int g = 0; g < kG; ++g) {
+                            const int e = s + off[g] < nsave ? s + off[g] : s;
+                            const double d = static_cast<double>(col[e * kP]) - a;
+                            s1[g] += d;
+                            s2[g] = fma(d, d, s2[g]);
+                        }
+                    }
+                } else {
+                    const bool live = p < np;
+                    for (int s = s_lo; s < s_hi; ++s) {
+                        const T av = col[s * kP];
+                        const bool fa = live && isfinite(av);
+#pragma unroll
+                        for (int g = 0; g < kG; ++g) {
+                            const bool in = valid[g] && s + off[g] < nsave;
+                            const T bv = col[(in ? s + off[g] : s) * kP];
+                            const bool ok = fa && in && isfinite(bv);
+                            const double d = ok ? static_cast<double>(bv) - static_cast<double>(av) : 0.0;
+                            cnt[g] += ok ? 1u : 0u;
+                            s1[g] += d;
+                            s2[g] = fma(d, d, s2[g]);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < kG; ++g) {
+                const int li = gy * kG + g;
+                const double a = wave_sum_d(s1[g]);
+                const double b = wave_sum_d(s2[g]);
+                long long c;
+                if (finite) {
+                    c = valid[g] ? static_cast<long long>(np) * (nsave - off[g]) : 0;
+                } else {
+                    c = static_cast<long long>(cnt[g]);
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+                }
+                if (lane == 0 && li < nlags) {
+                    const int64_t o = static_cast<int64_t>(li) * gridDim.x + blockIdx.x;
+                    if (tile == first) {
+                        part_d[o * 2 + 0] = a;
+                        part_d[o * 2 + 1] = b;
+                        part_i[o] = c;
+                    } else {
+                        part_d[o * 2 + 0] += a;
+                        part_d[o * 2 + 1] += b;
+                        part_i[o] += c;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Particles per LDS tile of k_dpp_tile for a series of nsave samples of
+// `elem` bytes: the largest power of two up to 16 whose tile stays within
+// 40 KiB (four workgroups per CU), else up to 64 KiB; 0 when one particle's
+// series does not fit (the global walk k_dpp runs instead).
+inline int dpp_tile_particles(int64_t nsave, size_t elem) {
+    const int64_t row = nsave * static_cast<int64_t>(elem);
+    for (int64_t cap : {int64_t{40} << 10, int64_t{64} << 10})
+        for (int kp = 16; kp >= 1; kp >>= 1)
+            if (row * kp <= cap) return kp;
+    return 0;
+}
+
+constexpr int kDppTileBlocks = 2048;  // workgroups of a k_dpp_tile launch at most
 
 __global__ void __launch_bounds__(kTPB)
 k_merge_dpp(const double *__restrict__ part_d, const long long *__restrict__ part_i, int nlags, int nbx,
@@ -851,27 +1021,21 @@ k_merge_dpp(const double *__restrict__ part_d, const long long *__restrict__ par
     const int li = blockIdx.x;
     const int per = (nbx + kTPB - 1) / kTPB;
     const int b0 = threadIdx.x * per, b1 = b0 + per < nbx ? b0 + per : nbx;
-    double a = 0, b = 0, c = 0, d = 0;
+    double a = 0, b = 0;
     long long n = 0;
     for (int k = b0; k < b1; ++k) {
         const int64_t o = static_cast<int64_t>(li) * nbx + k;
         n += part_i[o];
-        a += part_d[o * 4 + 0];
-        b += part_d[o * 4 + 1];
-        c += part_d[o * 4 + 2];
-        d += part_d[o * 4 + 3];
+        a += part_d[o * 2 + 0];
+        b += part_d[o * 2 + 1];
     }
     a = block_sum_d(a, sh_d);
     b = block_sum_d(b, sh_d);
-    c = block_sum_d(c, sh_d);
-    d = block_sum_d(d, sh_d);
     n = block_sum_i(n, sh_i);
     if (threadIdx.x == 0 && li < nlags) {
         out_i[li] = n;
-        out_d[li * 4 + 0] = a;
-        out_d[li * 4 + 1] = b;
-        out_d[li * 4 + 2] = c;
-        out_d[li * 4 + 3] = d;
+        out_d[li * 2 + 0] = a;
+        out_d[li * 2 + 1] = b;
     }
 }
 
@@ -1524,14 +1688,14 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
             const int64_t cap = std::max<int64_t>(kTPB, ((int64_t{1} << 31) / (8 * nsave)) / kTPB * kTPB);
             chunk = std::min(n_sel, cap);
         }
-        const int nbx_max = static_cast<int>((chunk + kTPB - 1) / kTPB);
+        const int nbx_max = std::max(static_cast<int>((chunk + kTPB - 1) / kTPB), kDppTileBlocks);
         const size_t nl = static_cast<size_t>(nlags), parts = nl * static_cast<size_t>(nbx_max);
-        char *base = static_cast<char *>(hout.alloc(parts * (32 + 8) + nl * (32 + 8)));
+        char *base = static_cast<char *>(hout.alloc(parts * (16 + 8) + nl * (16 + 8)));
         auto *part_d = reinterpret_cast<double *>(base);
-        auto *part_i = reinterpret_cast<long long *>(part_d + parts * 4);
+        auto *part_i = reinterpret_cast<long long *>(part_d + parts * 2);
         auto *out_d = reinterpret_cast<double *>(part_i + parts);
-        auto *out_i = reinterpret_cast<long long *>(out_d + nl * 4);
-        std::vector<double> hd(nl * 4, 0.0), cd(nl * 4);
+        auto *out_i = reinterpret_cast<long long *>(out_d + nl * 2);
+        std::vector<double> hd(nl * 2, 0.0), cd(nl * 2);
         std::vector<long long> hi(nl, 0), ci(nl);
         DevBuf hfin(ctx);
         auto *fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(std::max(chunk, n_sel)) * 4));
@@ -1546,18 +1710,39 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
             // offsets below 2^31
             const bool buf = lane32 && (row_len + lag_top * row_len) * static_cast<int64_t>(sizeof(T)) <
                                            (int64_t{1} << 31);
-            auto *kern = buf ? k_dpp<T, kDelta, true> : k_dpp<T, kDelta, false>;
-            kern<<<tile_group_blocks(nbx, ngy), kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag,
-                                                              static_cast<int>(nlags), 1.0 / p0, part_d, part_i, nbx,
-                                                              ngy, fin, lane32);
+            int nparts = nbx;
+            const int kp = dpp_tile_particles(nsave, sizeof(T));
+            if (kp > 0) {
+                // LDS tiles of kp particles: every sample crosses the L2 once
+                const int64_t ntiles = (nsel_c + kp - 1) / kp;
+                nparts = static_cast<int>(std::min<int64_t>(ntiles, kDppTileBlocks));
+                const size_t lds = static_cast<size_t>(nsave) * kp * sizeof(T);
+                auto launch = [&](auto kernel) {
+                    kernel<<<static_cast<unsigned>(nparts), kTPB, lds, st>>>(
+                        pm, static_cast<int32_t>(nsave), row_len, nsel_c, idx, dlag, static_cast<int>(nlags), part_d,
+                        part_i, fin);
+                };
+                switch (kp) {
+                    case 16: launch(k_dpp_tile<T, kDelta, 16>); break;
+                    case 8: launch(k_dpp_tile<T, kDelta, 8>); break;
+                    case 4: launch(k_dpp_tile<T, kDelta, 4>); break;
+                    case 2: launch(k_dpp_tile<T, kDelta, 2>); break;
+                    default: launch(k_dpp_tile<T, kDelta, 1>); break;
+                }
+            } else {
+                auto *kern = buf ? k_dpp<T, kDelta, true> : k_dpp<T, kDelta, false>;
+                kern<<<tile_group_blocks(nbx, ngy), kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag,
+                                                                  static_cast<int>(nlags), part_d, part_i, nbx, ngy,
+                                                                  fin, lane32);
+            }
             PT_HIP_CHECK(hipGetLastError());
-            k_merge_dpp<<<static_cast<unsigned>(nlags), kTPB, 0, st>>>(part_d, part_i, static_cast<int>(nlags), nbx,
+            k_merge_dpp<<<static_cast<unsigned>(nlags), kTPB, 0, st>>>(part_d, part_i, static_cast<int>(nlags), nparts,
                                                                        out_d, out_i);
             PT_HIP_CHECK(hipGetLastError());
-            PT_HIP_CHECK(hipMemcpyAsync(cd.data(), out_d, nl * 32, hipMemcpyDeviceToHost, st));
+            PT_HIP_CHECK(hipMemcpyAsync(cd.data(), out_d, nl * 16, hipMemcpyDeviceToHost, st));
             PT_HIP_CHECK(hipMemcpyAsync(ci.data(), out_i, nl * 8, hipMemcpyDeviceToHost, st));
             PT_HIP_CHECK(hipStreamSynchronize(st));
-            for (size_t q = 0; q < nl * 4; ++q) hd[q] += cd[q];
+            for (size_t q = 0; q < nl * 2; ++q) hd[q] += cd[q];
             for (size_t q = 0; q < nl; ++q) hi[q] += ci[q];
         };
         if constexpr (!kMomenta) {
@@ -1577,15 +1762,15 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
             if (lags[li] >= nsave) continue;
             counts[li] += hi[li];
             if constexpr (kDelta) {
-                sum_delta_p[li] += p_ref * hd[li * 4 + 0];
-                sum_delta_p2[li] += p_ref * p_ref * hd[li * 4 + 1];
-                sum_delta_p_norm[li] += rn * hd[li * 4 + 0];
-                sum_delta_p_norm2[li] += rn * rn * hd[li * 4 + 1];
+                sum_delta_p[li] += p_ref * hd[li * 2 + 0];
+                sum_delta_p2[li] += p_ref * p_ref * hd[li * 2 + 1];
+                sum_delta_p_norm[li] += rn * hd[li * 2 + 0];
+                sum_delta_p_norm2[li] += rn * rn * hd[li * 2 + 1];
             } else {
-                sum_delta_p[li] += hd[li * 4 + 0];
-                sum_delta_p2[li] += hd[li * 4 + 1];
-                sum_delta_p_norm[li] += hd[li * 4 + 2];
-                sum_delta_p_norm2[li] += hd[li * 4 + 3];
+                sum_delta_p[li] += hd[li * 2 + 0];
+                sum_delta_p2[li] += hd[li * 2 + 1];
+                sum_delta_p_norm[li] += hd[li * 2 + 0] / p0;
+                sum_delta_p_norm2[li] += hd[li * 2 + 1] / p0 / p0;
             }
         }
     });

@@ -400,6 +400,51 @@ def test_dpp_from_fp64_pmag_cache_matches_oracle(ctx):
         assert np.allclose(getattr(acc, k), oacc[k], rtol=1e-11, atol=1e-12 * np.max(np.abs(oacc[k]))), k
 
 
+@pytest.mark.parametrize("nsave,n", [(40, 40_000), (700, 300), (2600, 70), (6000, 40), (9000, 9)])
+def test_dpp_tile_sizes_match_oracle(ctx, nsave, n):
+    """The D_pp walk stages each particle's whole series in LDS, 16, 4 or 1
+    particles per workgroup tile as the series length allows (40,000 particles
+    of 40 samples: workgroups that take several tiles; 9000 FP64 samples: too
+    long for a tile, the global walk).  Dead particles (NaN suffix) put some
+    tiles on the checked path; lags reach past the series.  FP64 |p| from FP32
+    momenta against the oracle, and the float32 relative series against an
+    FP64 brute force."""
+    from particle_transport_amd import _lib, serving
+    rng = np.random.default_rng(nsave)
+    mom = (rng.normal(0, 1e-18, (nsave, 3, n)) + 5e-17).astype(np.float32)
+    for i in np.nonzero(rng.random(n) < 0.05)[0]:
+        mom[rng.integers(1, nsave):, :, i] = np.nan
+    mom[:, :, 0] = mom[0, :, 0]  # keep particle 0 alive: p0 below
+    lags = sorted({1, 2, 3, nsave - 1, nsave, nsave + 7} | set(np.geomspace(4, nsave - 2, 8).astype(int).tolist()))
+    p0 = dpp.momentum_magnitude(*mom[0, :, 0])
+    acc = dpp.new_accumulators(len(lags))
+    dpp.process_momenta_chunk_dpp(ctx, mom, lags, p0, acc)
+    oacc = oracle.new_dpp_accumulators(len(lags))
+    oracle.dpp(mom, lags, p0, oacc)
+    assert np.array_equal(acc.counts, oacc["counts"]) and oacc["counts"].sum() > 0
+    for k in ("sum_delta_p", "sum_delta_p2", "sum_delta_p_norm", "sum_delta_p_norm2"):
+        assert np.allclose(getattr(acc, k), oacc[k], rtol=1e-11, atol=1e-12 * np.max(np.abs(oacc[k]))), k
+    pm = np.sqrt((mom.astype(np.float64) ** 2).sum(axis=1))
+    p_ref = float(np.nanmedian(pm))
+    delta = np.ascontiguousarray((pm / p_ref - 1.0).astype(np.float32))
+    buf = _lib.DeviceBuffer.from_host(ctx, delta)
+    try:
+        racc = serving.dpp_dprel_on_device(ctx, buf.ptr, nsave, n, lags, p_ref, p0)
+    finally:
+        buf.free()
+    for li, lag in enumerate(lags):
+        if lag >= nsave:
+            assert racc.counts[li] == 0 and racc.sum_delta_p2[li] == 0.0
+            continue
+        a, b = delta[:-lag].astype(np.float64), delta[lag:].astype(np.float64)
+        ok = np.isfinite(a) & np.isfinite(b)
+        d = (b - a)[ok]
+        assert racc.counts[li] == ok.sum()
+        for k, want in (("sum_delta_p", p_ref * d.sum()), ("sum_delta_p2", p_ref ** 2 * (d * d).sum())):
+            got = getattr(racc, k)[li]
+            assert abs(got - want) <= 1e-11 * abs(want) + 1e-13 * abs(p_ref ** 2 * (d * d).sum()), (k, lag)
+
+
 def test_dpp_from_relative_cache(ctx):
     """pt_dpp_accumulate_dprel over a float32 relative cache delta = |p|/p_ref - 1
     (the serving path's PT_OUT_DPREL): counts and sums equal a float64 brute
