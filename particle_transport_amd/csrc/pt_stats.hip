@@ -841,8 +841,7 @@ k_dpp(const T *__restrict__ pm, int64_t nsave, int64_t total, int64_t n_sel,
 // takes tiles first, first + gridDim.x, ... (first: below) and adds each
 // lag's wave sum into its own slot part_*[lag * gridDim.x + blockIdx.x] in
 // tile order, so the result does not depend on scheduling.  Tiles whose particles are finite at
-// every sample (fin == nsave) skip the finiteness tests; their counts are
-// np * (nsave - lag).
+// every sample skip the finiteness tests; their counts are np * (nsave - lag).
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -854,11 +853,15 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 // LDS bandwidth either way (five reads of eight bytes per four pairs).
 constexpr int kTileLags = 4;
 
-template <class T, bool kDelta, int kP>
+// kMom: the source is FP32 momenta [save][3][total] and the tile holds their
+// FP64 |p| (momentum_magnitude, evaluated while staging); otherwise the
+// source is the series itself, [save][total] of T.
+template <class T, bool kDelta, int kP, bool kMom>
 __global__ void __launch_bounds__(kTPB)
-k_dpp_tile(const T *__restrict__ pm, int32_t nsave, int64_t total, int64_t n_sel,
+k_dpp_tile(const void *__restrict__ src, int32_t nsave, int64_t total, int64_t n_sel,
            const int64_t *__restrict__ pidx, const int64_t *__restrict__ lags, int nlags,
-           double *__restrict__ part_d, long long *__restrict__ part_i, const int32_t *__restrict__ fin) {
+           double *__restrict__ part_d, long long *__restrict__ part_i) {
+    static_assert(!kMom || (std::is_same<T, double>::value && !kDelta), "momenta stage an FP64 |p| series");
     constexpr int kG = kTileLags;
     constexpr int kK = 64 / kP;
     static_assert(kP >= 1 && kP <= 64 && (kP & (kP - 1)) == 0, "kP: a power of two up to 64");
@@ -880,24 +883,34 @@ k_dpp_tile(const T *__restrict__ pm, int32_t nsave, int64_t total, int64_t n_sel
         const int64_t j0 = tile * kP;
         const int np = n_sel - j0 < kP ? static_cast<int>(n_sel - j0) : kP;
         __syncthreads();  // the previous tile's walk is done with the rows
-        // stage: eight loads in flight per thread before their LDS writes
-        for (int e0 = threadIdx.x; e0 < nel; e0 += 8 * kTPB) {
-            T v[8];
+        // stage: kU samples in flight per thread before their LDS writes
+        // (momenta: three loads each); the tile is finite when every staged
+        // sample is
+        constexpr int kU = kMom ? 4 : 8;
+        bool ok = true;
+        for (int e0 = threadIdx.x; e0 < nel; e0 += kU * kTPB) {
+            T v[kU];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < kU; ++u) {
                 const int e = e0 + u * kTPB;
                 const int q = e & (kP - 1);
                 v[u] = T(0);
                 if (e < nel && q < np) {
                     const int64_t col = pidx ? pidx[j0 + q] : j0 + q;
-                    v[u] = pm[static_cast<int64_t>(e / kP) * total + col];
+                    if constexpr (kMom)
+                        v[u] = pmag(static_cast<const float *>(src), e / kP, total, col);
+                    else
+                        v[u] = static_cast<const T *>(src)[static_cast<int64_t>(e / kP) * total + col];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (e0 + u * kTPB < nel) rows[e0 + u * kTPB] = v[u];
+            for (int u = 0; u < kU; ++u)
+                if (e0 + u * kTPB < nel) {
+                    rows[e0 + u * kTPB] = v[u];
+                    ok = ok && isfinite(v[u]);
+                }
         }
-        const bool finite = __syncthreads_and(threadIdx.x >= np || fin[j0 + threadIdx.x] >= nsave) != 0;
+        const bool finite = __syncthreads_and(ok) != 0;
         const T *col = rows + p;
         for (int gy = wave; gy < ngroups; gy += kTPB / 64) {
             int32_t off[kG];
@@ -1680,9 +1693,14 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         const Src *dm = to_device(momenta, on_device, static_cast<size_t>(nsave * comps * total), hm, st);
         const int64_t *didx = particle_index ? to_device(particle_index, 0, static_cast<size_t>(n_sel), hidx, st) : nullptr;
         const int64_t *dlag = to_device(lags.data(), 0, lags.size(), hlag, st);
-        // FP32 momenta: |p| once per (sample, particle) into an FP64 scratch
-        // series, a chunk of particles at a time (at most ~2 GiB).
+        // FP32 momenta: the walks read an FP64 |p| series
         constexpr bool kMomenta = !kDelta && !std::is_same<Src, double>::value;
+        using Series = std::conditional_t<kMomenta, double, Src>;
+        // LDS tiles of kp particles when one particle's series fits
+        const int kp = dpp_tile_particles(nsave, sizeof(Series));
+        // The global walk over FP32 momenta: |p| once per (sample, particle)
+        // into an FP64 scratch series, a chunk of particles at a time (at most
+        // ~2 GiB).
         int64_t chunk = n_sel;
         if constexpr (kMomenta) {
             const int64_t cap = std::max<int64_t>(kTPB, ((int64_t{1} << 31) / (8 * nsave)) / kTPB * kTPB);
@@ -1697,45 +1715,8 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
         auto *out_i = reinterpret_cast<long long *>(out_d + nl * 2);
         std::vector<double> hd(nl * 2, 0.0), cd(nl * 2);
         std::vector<long long> hi(nl, 0), ci(nl);
-        DevBuf hfin(ctx);
-        auto *fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(std::max(chunk, n_sel)) * 4));
-        auto run = [&](const auto *pm, int64_t row_len, int64_t nsel_c, const int64_t *idx) {
-            using T = std::remove_cv_t<std::remove_pointer_t<decltype(pm)>>;
-            const int nbx = static_cast<int>((nsel_c + kTPB - 1) / kTPB);
-            const int ngy = static_cast<int>((nlags + kDppLags<T> - 1) / kDppLags<T>);
-            launch_finite_prefix(pm, nsave, nsel_c, row_len, int64_t{1}, idx, fin, st);
-            // the fast path's 32-bit lane offsets (bytes) cover every column
-            const int lane32 = row_len < (int64_t{1} << 29) && nsave < (int64_t{1} << 31) ? 1 : 0;
-            // buffer loads: the last column's and the largest lag's byte
-            // offsets below 2^31
-            const bool buf = lane32 && (row_len + lag_top * row_len) * static_cast<int64_t>(sizeof(T)) <
-                                           (int64_t{1} << 31);
-            int nparts = nbx;
-            const int kp = dpp_tile_particles(nsave, sizeof(T));
-            if (kp > 0) {
-                // LDS tiles of kp particles: every sample crosses the L2 once
-                const int64_t ntiles = (nsel_c + kp - 1) / kp;
-                nparts = static_cast<int>(std::min<int64_t>(ntiles, kDppTileBlocks));
-                const size_t lds = static_cast<size_t>(nsave) * kp * sizeof(T);
-                auto launch = [&](auto kernel) {
-                    kernel<<<static_cast<unsigned>(nparts), kTPB, lds, st>>>(
-                        pm, static_cast<int32_t>(nsave), row_len, nsel_c, idx, dlag, static_cast<int>(nlags), part_d,
-                        part_i, fin);
-                };
-                switch (kp) {
-                    case 16: launch(k_dpp_tile<T, kDelta, 16>); break;
-                    case 8: launch(k_dpp_tile<T, kDelta, 8>); break;
-                    case 4: launch(k_dpp_tile<T, kDelta, 4>); break;
-                    case 2: launch(k_dpp_tile<T, kDelta, 2>); break;
-                    default: launch(k_dpp_tile<T, kDelta, 1>); break;
-                }
-            } else {
-                auto *kern = buf ? k_dpp<T, kDelta, true> : k_dpp<T, kDelta, false>;
-                kern<<<tile_group_blocks(nbx, ngy), kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag,
-                                                                  static_cast<int>(nlags), part_d, part_i, nbx, ngy,
-                                                                  fin, lane32);
-            }
-            PT_HIP_CHECK(hipGetLastError());
+        // merge the partials of one walk in workgroup order and add them up
+        auto collect = [&](int nparts) {
             k_merge_dpp<<<static_cast<unsigned>(nlags), kTPB, 0, st>>>(part_d, part_i, static_cast<int>(nlags), nparts,
                                                                        out_d, out_i);
             PT_HIP_CHECK(hipGetLastError());
@@ -1745,7 +1726,49 @@ int dpp_accumulate(pt_ctx *ctx, const Src *momenta, int on_device, int64_t nsave
             for (size_t q = 0; q < nl * 2; ++q) hd[q] += cd[q];
             for (size_t q = 0; q < nl; ++q) hi[q] += ci[q];
         };
-        if constexpr (!kMomenta) {
+        // The global walk (series longer than one LDS tile per particle).
+        DevBuf hfin(ctx);
+        int32_t *fin = nullptr;
+        auto run = [&](const auto *pm, int64_t row_len, int64_t nsel_c, const int64_t *idx) {
+            using T = std::remove_cv_t<std::remove_pointer_t<decltype(pm)>>;
+            const int nbx = static_cast<int>((nsel_c + kTPB - 1) / kTPB);
+            const int ngy = static_cast<int>((nlags + kDppLags<T> - 1) / kDppLags<T>);
+            if (!fin) fin = static_cast<int32_t *>(hfin.alloc(static_cast<size_t>(std::max(chunk, n_sel)) * 4));
+            launch_finite_prefix(pm, nsave, nsel_c, row_len, int64_t{1}, idx, fin, st);
+            // the fast path's 32-bit lane offsets (bytes) cover every column
+            const int lane32 = row_len < (int64_t{1} << 29) && nsave < (int64_t{1} << 31) ? 1 : 0;
+            // buffer loads: the last column's and the largest lag's byte
+            // offsets below 2^31
+            const bool buf = lane32 && (row_len + lag_top * row_len) * static_cast<int64_t>(sizeof(T)) <
+                                           (int64_t{1} << 31);
+            auto *kern = buf ? k_dpp<T, kDelta, true> : k_dpp<T, kDelta, false>;
+            kern<<<tile_group_blocks(nbx, ngy), kTPB, 0, st>>>(pm, nsave, row_len, nsel_c, idx, dlag,
+                                                              static_cast<int>(nlags), part_d, part_i, nbx, ngy, fin,
+                                                              lane32);
+            PT_HIP_CHECK(hipGetLastError());
+            collect(nbx);
+        };
+        if (kp > 0) {
+            // LDS tiles straight from the source (FP32 momenta: |p| while
+            // staging): every sample crosses the L2 once
+            const int64_t ntiles = (n_sel + kp - 1) / kp;
+            const int nparts = static_cast<int>(std::min<int64_t>(ntiles, kDppTileBlocks));
+            const size_t lds = static_cast<size_t>(nsave) * kp * sizeof(Series);
+            auto launch = [&](auto kernel) {
+                kernel<<<static_cast<unsigned>(nparts), kTPB, lds, st>>>(dm, static_cast<int32_t>(nsave), total, n_sel,
+                                                                         didx, dlag, static_cast<int>(nlags), part_d,
+                                                                         part_i);
+            };
+            switch (kp) {
+                case 16: launch(k_dpp_tile<Series, kDelta, 16, kMomenta>); break;
+                case 8: launch(k_dpp_tile<Series, kDelta, 8, kMomenta>); break;
+                case 4: launch(k_dpp_tile<Series, kDelta, 4, kMomenta>); break;
+                case 2: launch(k_dpp_tile<Series, kDelta, 2, kMomenta>); break;
+                default: launch(k_dpp_tile<Series, kDelta, 1, kMomenta>); break;
+            }
+            PT_HIP_CHECK(hipGetLastError());
+            collect(nparts);
+        } else if constexpr (!kMomenta) {
             run(dm, total, n_sel, didx);
         } else {
             auto *pm = static_cast<double *>(hpm.alloc(static_cast<size_t>(nsave * chunk) * 8));
