@@ -26,6 +26,13 @@ Parity pinning of each function (what independent check pins it):
   - ``inject``          positions equal numpy Philox doubles mapped onto the
                         box; fixed-mu gives p.b/(|p||b|) == mu0 and |p| ==
                         pscale; isotropic moments <u_z> = 0, <u_z^2> = 1/3.
+
+Parity unpinned (nothing available here fixes these bits):
+  - Julia's MersenneTwister streams (the reference's injection draws and its
+    random / block-random particle selection): the parity tests hand both
+    sides identical initial arrays and index sets instead.
+  - the bit pattern of Julia's pairwise ``mean`` behind ``mean_abs_b``: the
+    known answer pins the value, other fields are compared at 1e-15.
 """
 from __future__ import annotations
 

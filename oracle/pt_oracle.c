@@ -14,6 +14,10 @@
  *
  * Parity pinning (which check pins which function) is listed in the header
  * comment of each function; the tests live in tests/test_oracle.py.
+ * Parity unpinned: Julia's MersenneTwister streams (injection draws, random
+ * particle selection; the tests hand both sides identical arrays instead) and
+ * the bit pattern of Julia's pairwise mean in oracle_mean_abs_b (its known
+ * answer pins the value; other fields compare at 1e-15).
  */
 #include <math.h>
 #include <stdint.h>
