@@ -407,8 +407,8 @@ def test_dpp_tile_sizes_match_oracle(ctx, nsave, n):
     of 40 samples: workgroups that take several tiles; 9000 FP64 samples: too
     long for a tile, the global walk).  Dead particles (NaN suffix) put some
     tiles on the checked path; lags reach past the series.  FP64 |p| from FP32
-    momenta against the oracle, and the float32 relative series against an
-    FP64 brute force."""
+    momenta against the oracle (and bitwise equal over two calls), and the
+    float32 relative series against an FP64 brute force."""
     from particle_transport_amd import _lib, serving
     rng = np.random.default_rng(nsave)
     mom = (rng.normal(0, 1e-18, (nsave, 3, n)) + 5e-17).astype(np.float32)
@@ -419,6 +419,10 @@ def test_dpp_tile_sizes_match_oracle(ctx, nsave, n):
     p0 = dpp.momentum_magnitude(*mom[0, :, 0])
     acc = dpp.new_accumulators(len(lags))
     dpp.process_momenta_chunk_dpp(ctx, mom, lags, p0, acc)
+    again = dpp.new_accumulators(len(lags))
+    dpp.process_momenta_chunk_dpp(ctx, mom, lags, p0, again)
+    for k in ("counts", "sum_delta_p", "sum_delta_p2", "sum_delta_p_norm", "sum_delta_p_norm2"):
+        assert np.array_equal(getattr(acc, k), getattr(again, k)), k  # fixed merge order: bitwise run to run
     oacc = oracle.new_dpp_accumulators(len(lags))
     oracle.dpp(mom, lags, p0, oacc)
     assert np.array_equal(acc.counts, oacc["counts"]) and oacc["counts"].sum() > 0
