@@ -309,7 +309,10 @@ int pt_dmumu_accumulate_noalloc(pt_ctx *ctx, const float *mu, int64_t nsave, int
 /* Replaces process_momenta_chunk_dpp_cpu! (compute_dpp_full.jl:247-288).
  * momenta: float32, Julia (total,3,nsave) (C [s][3][total]) with the chunk's
  * selected columns given by particle_index (or 0..n_sel-1).  |p| in FP64
- * (momentum_magnitude, :232-234); sums in FP64; updated in place. */
+ * (momentum_magnitude, :232-234); sums in FP64; updated in place.  The
+ * p0-normalised sums are (sum Delta p)/p0 and (sum Delta p^2)/p0^2, equal to
+ * the reference's per-pair sums of Delta p/p0 up to the rounding of the terms.
+ * Deterministic: the same inputs give the same bits on every call. */
 int pt_dpp_accumulate(pt_ctx *ctx, const float *momenta, int on_device, int64_t nsave,
                       int64_t total, int64_t n_sel, const int64_t *particle_index,
                       const int64_t *lag_steps, int64_t nlags, double p0, int64_t *counts,
