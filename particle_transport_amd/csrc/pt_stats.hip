@@ -1014,11 +1014,12 @@ k_dpp_tile(const void *__restrict__ src, int32_t nsave, int64_t total, int64_t n
 
 // Particles per LDS tile of k_dpp_tile for a series of nsave samples of
 // `elem` bytes: the largest power of two up to 16 whose tile stays within
-// 40 KiB (four workgroups per CU), else up to 64 KiB; 0 when one particle's
+// 40 KiB (four workgroups per CU), else up to 60 KiB (with the kernel's
+// static LDS, inside a workgroup's default 64 KiB); 0 when one particle's
 // series does not fit (the global walk k_dpp runs instead).
 inline int dpp_tile_particles(int64_t nsave, size_t elem) {
     const int64_t row = nsave * static_cast<int64_t>(elem);
-    for (int64_t cap : {int64_t{40} << 10, int64_t{64} << 10})
+    for (int64_t cap : {int64_t{40} << 10, int64_t{60} << 10})
         for (int kp = 16; kp >= 1; kp >>= 1)
             if (row * kp <= cap) return kp;
     return 0;
