@@ -344,13 +344,76 @@ def _direct_sources(sources, dtypes):
                for s, dt in zip(sources, dtypes))
 
 
+def _direct_groups(ctx, sources, pidx0, chunk, dtypes, budget_bytes):
+    """device_particle_groups over page-locked sources: a worker thread has
+    the copy engine move the next group's columns into one of two sets of
+    device buffers, on a context (stream) of its own, while the caller's
+    kernels work on the current group.  A caller must be done with a group's
+    device views when it asks for the group after next (the statistics
+    passes synchronise before they return)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from .runner import Context
+
+    pidx0 = np.asarray(pidx0, dtype=np.int64)
+    per_col = max(int(np.prod(s.shape[:-1])) * s.dtype.itemsize for s in sources)
+    plans = h5.plan_particle_groups(pidx0, chunk, max(1, int(budget_bytes) // max(1, per_col)))
+    cctx = Context(getattr(ctx, "device", 0))
+    sets = [[None] * len(sources) for _ in range(2)]
+
+    def dev(slot, k, shape, dt):
+        n = max(1, int(np.prod(shape)))
+        b = sets[slot][k]
+        if b is None or b.nbytes < n * np.dtype(dt).itemsize:
+            if b is not None:
+                b.free()
+            b = sets[slot][k] = _lib.DeviceBuffer(cctx, (n,), dt)
+        return b.view(shape)
+
+    def upload(plan, slot):
+        kind, g0, g1, lo, hi = plan
+        views = []
+        for k, (s, dt) in enumerate(zip(sources, dtypes)):
+            if kind == "span":
+                v = dev(slot, k, tuple(s.shape[:-1]) + (hi - lo + 1,), dt)
+                v.upload_columns(s, lo)  # synchronous on the copy context
+            else:
+                a = np.ascontiguousarray(np.take(s, pidx0[g0:g1], axis=-1), dtype=dt)
+                v = dev(slot, k, a.shape, dt)
+                v.upload(a)
+            views.append(v)
+        return views, h5.plan_chunks(plan, pidx0, chunk)
+
+    fut = None
+    try:
+        with ThreadPoolExecutor(max_workers=1) as ex:
+            fut = ex.submit(upload, plans[0], 0) if plans else None
+            for k in range(len(plans)):
+                cur = fut.result()
+                fut = ex.submit(upload, plans[k + 1], (k + 1) & 1) if k + 1 < len(plans) else None
+                yield cur
+    finally:
+        if fut is not None:
+            fut.cancel()
+            try:
+                fut.result()
+            except Exception:  # the copy of a group nobody asked for
+                pass
+        for bs in sets:
+            for b in bs:
+                if b is not None:
+                    b.free()
+        cctx.close()
+
+
 def device_particle_groups(ctx, sources, pidx0, chunk, dtypes, budget_bytes=256 << 20):
     """h5.iter_particle_groups over host sources (numpy arrays, memory maps
     or h5.H5Datasets), each group uploaded once into reusable device buffers:
     yields (device views, chunks).  The next group is read from the host
     source while the GPU works on the current one.  Page-locked sources of
     the wanted dtype skip the host read: a group's span of columns goes to
-    the device in one strided copy per source."""
+    the device in one strided copy per source, the next group's while the
+    GPU works on the current one (_direct_groups)."""
     bufs = [None] * len(sources)
 
     def dev(k, shape, dt):
@@ -362,26 +425,7 @@ def device_particle_groups(ctx, sources, pidx0, chunk, dtypes, budget_bytes=256 
         return bufs[k].view(shape)
 
     if _direct_sources(sources, dtypes):
-        pidx0 = np.asarray(pidx0, dtype=np.int64)
-        per_col = max(int(np.prod(s.shape[:-1])) * s.dtype.itemsize for s in sources)
-        try:
-            for plan in h5.plan_particle_groups(pidx0, chunk, max(1, int(budget_bytes) // max(1, per_col))):
-                kind, g0, g1, lo, hi = plan
-                views = []
-                for k, (s, dt) in enumerate(zip(sources, dtypes)):
-                    if kind == "span":
-                        v = dev(k, tuple(s.shape[:-1]) + (hi - lo + 1,), dt)
-                        v.upload_columns(s, lo)
-                    else:
-                        a = np.ascontiguousarray(np.take(s, pidx0[g0:g1], axis=-1), dtype=dt)
-                        v = dev(k, a.shape, dt)
-                        v.upload(a)
-                    views.append(v)
-                yield views, h5.plan_chunks(plan, pidx0, chunk)
-        finally:
-            for b in bufs:
-                if b is not None:
-                    b.free()
+        yield from _direct_groups(ctx, sources, pidx0, chunk, dtypes, budget_bytes)
         return
 
     srcs = [s if hasattr(s, "read_box") else h5.ArrayDataset(np.asarray(s)) for s in sources]
