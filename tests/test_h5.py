@@ -163,6 +163,34 @@ def _write_cache(path, kind, s, nsave, n, alive=None):
         cache_h5.finalize_cache(f, CFG, s, 10.0, alive, kind, np.float32)
 
 
+def test_group_plans_cover_the_selection_in_order():
+    """plan_particle_groups / plan_chunks: whole chunks in selection order,
+    spans while the column span fits the cap and indices ascend, runs
+    otherwise; every selected index appears once, at its chunk's offset."""
+    assert h5.plan_particle_groups(np.zeros(0, np.int64), 4, 100) == []
+    pidx0 = np.arange(10, 30)
+    assert h5.plan_particle_groups(pidx0, 4, 100) == [("span", 0, 20, 10, 29)]
+    plans = h5.plan_particle_groups(pidx0, 4, 9)  # two chunks (8 columns) per span
+    assert [p[:3] for p in plans] == [("span", 0, 8), ("span", 8, 16), ("span", 16, 20)]
+    desc = pidx0[::-1].copy()
+    assert all(p[0] == "runs" for p in h5.plan_particle_groups(desc, 4, 100))
+    rng = np.random.default_rng(3)
+    sel = np.sort(rng.choice(1000, 300, replace=False))
+    for cap in (1, 50, 10_000):
+        plans = h5.plan_particle_groups(sel, 16, cap)
+        seen = []
+        for plan in plans:
+            kind, g0, g1, lo, hi = plan
+            if kind == "span":
+                assert hi - lo + 1 <= max(cap, sel[min(g0 + 16, g1) - 1] - sel[g0] + 1)
+            for c0, local in h5.plan_chunks(plan, sel, 16):
+                want = sel[c0:min(c0 + 16, g1)]
+                got = local + lo if kind == "span" else sel[g0:g1][local]
+                assert np.array_equal(got, want)
+                seen.append(c0)
+        assert seen == list(range(0, len(sel), 16))
+
+
 def test_read_box_into_given_buffers():
     """A box large enough to be copied on several threads, read into a fresh
     array and into a caller's buffer, equals the slice; a buffer of the wrong
