@@ -73,6 +73,23 @@ def test_julia_module_binds_every_declaration_with_its_arity():
         assert bound[name] == {n}, (name, n, bound[name])
 
 
+def test_pinned_registry_bounds():
+    """is_pinned: an array (or view) lies wholly inside one registered
+    page-locked allocation; arrays that straddle its end or live elsewhere do
+    not (the registry is filled by PinnedArray; a host array stands in here)."""
+    import numpy as np
+    host = np.zeros(1000, np.float32)
+    base, n = host.ctypes.data, 400 * 4  # register the first 400 elements only
+    _lib._pinned[base] = n
+    try:
+        assert _lib.is_pinned(host[:400]) and _lib.is_pinned(host[10:20]) and _lib.is_pinned(host[:400:3])
+        assert not _lib.is_pinned(host[:401]) and not _lib.is_pinned(host[500:])
+        assert not _lib.is_pinned(np.zeros(4, np.float32)) and not _lib.is_pinned(host[:0])
+    finally:
+        del _lib._pinned[base]
+    assert not _lib.is_pinned(host[:10])
+
+
 def test_errors_are_reported_not_thrown():
     L = _lib.load()
     # a null out-pointer is rejected with a message before any device call
